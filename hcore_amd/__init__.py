@@ -1,0 +1,289 @@
+"""hcore_amd — MI355X-native tile low-rank BLAS (HCORE drop-in).
+
+The product is the C-ABI shared library hcore_amd/lib/libhcore_amd.so
+(HIP kernels for gfx950 + host orchestration).  This module is a thin ctypes
+front end mirroring the reference's C API (HCORE_dgemm, HCORE_sgemm,
+HCORE_dsyrk, HCORE_dgemmcd; reference src/include/hcore_d.h,
+src/include/hcore_s.h) plus the batched-plan extension of include/hcore_hip.h.
+
+There is no CPU fallback: importing the library without the built .so raises.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "lib", "libhcore_amd.so")
+INCLUDE_DIR = os.path.join(ROOT, "include")
+
+# status codes (include/hcore_hip.h)
+OK, ERR_ZERO_RANK, ERR_SUM_RANKS, ERR_K_GT_MAXRK, ERR_R_GT_MAXRK, ERR_NUMERIC, \
+    ERR_RUNTIME, ERR_ARG = range(8)
+FP64, FP32 = 0, 1
+ON_ERROR_EXIT, ON_ERROR_RETURN = 0, 1
+
+
+class FlopCounter(C.Structure):
+    """ctypes mirror of flop_counter (include/flop_util_structs.h)."""
+    _fields_ = [("__pad1", C.c_ulong), ("__pad2", C.c_ulong), ("copy", C.c_ulong),
+                ("flop", C.c_ulong), ("tid", C.c_int), ("potrf", C.c_ulong),
+                ("trsm", C.c_ulong), ("syrk", C.c_ulong), ("update", C.c_ulong),
+                ("geqrfu", C.c_ulong), ("geqrfv", C.c_ulong), ("geqrfv_gemm", C.c_ulong),
+                ("gesvd", C.c_ulong), ("gesvd_gemm", C.c_ulong), ("gesvd_lasetu", C.c_ulong),
+                ("gesvd_lasetv", C.c_ulong), ("gesvd_lacpyu", C.c_ulong),
+                ("gesvd_lacpyv", C.c_ulong), ("gesvd_dscal", C.c_ulong), ("ormqru", C.c_ulong),
+                ("ormqru_laset", C.c_ulong), ("ormqru_lacpy", C.c_ulong), ("ormqrv", C.c_ulong),
+                ("ormqrv_laset", C.c_ulong), ("ormqrv_lacpy", C.c_ulong),
+                ("omatcopy", C.c_ulong), ("numthreads", C.c_int), ("__pad3", C.c_ulong),
+                ("__pad4", C.c_ulong)]
+
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_vp = C.c_void_p
+_pp = C.POINTER(C.c_void_p)
+_ip = C.POINTER(C.c_int)
+
+
+def lib():
+    """Load the built library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("hcore_amd: %s is not built; run __graft_entry__.build() "
+                               "(there is no CPU fallback)" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        i, d, f = C.c_int, C.c_double, C.c_float
+        fl = C.POINTER(FlopCounter)
+        L.HCORE_dgemm.argtypes = [i, i, i, i, d, _dp, _dp, _dp, i, _dp, _dp, _dp, i, d, _dp, _dp,
+                                  _dp, i, i, i, d, _dp, fl]
+        L.HCORE_dgemm.restype = None
+        L.HCORE_sgemm.argtypes = [i, i, i, i, f, _fp, _fp, _fp, i, _fp, _fp, _fp, i, f, _fp, _fp,
+                                  _fp, i, i, i, f, _fp, fl]
+        L.HCORE_sgemm.restype = None
+        L.HCORE_dsyrk.argtypes = [i, i, i, i, d, _dp, i, _dp, i, d, _dp, i, _dp, fl]
+        L.HCORE_dsyrk.restype = None
+        L.HCORE_dgemmcd.argtypes = [i, i, i, d, _dp, _dp, i, i, _dp, _dp, i, i, d, _dp, i, i,
+                                    _dp, fl]
+        L.HCORE_dgemmcd.restype = None
+        L.flop_counts.argtypes = [C.c_char, C.c_ulong, C.c_ulong, C.c_ulong, C.c_ulong]
+        L.flop_counts.restype = C.c_ulong
+        L.HCORE_hip_init.argtypes = [i]
+        L.HCORE_hip_set_error_policy.argtypes = [i]
+        L.HCORE_hip_plan_create.argtypes = [C.POINTER(_vp), i, i]
+        L.HCORE_hip_plan_add_dgemm.argtypes = [_vp, i, _pp, _pp, _ip, i, _pp, _pp, _ip, i, _pp,
+                                               _pp, _ip, i, _pp, _pp, i, d, d, i, i, d]
+        L.HCORE_hip_plan_add_dsyrk.argtypes = [_vp, i, _pp, _pp, _ip, i, _pp, i, d]
+        L.HCORE_hip_plan_finalize.argtypes = [_vp]
+        L.HCORE_hip_plan_execute.argtypes = [_vp, _vp]
+        L.HCORE_hip_plan_execute_graph.argtypes = [_vp, _vp]
+        L.HCORE_hip_plan_ranks_device.argtypes = [_vp]
+        L.HCORE_hip_plan_ranks_device.restype = _vp
+        L.HCORE_hip_plan_get_ranks.argtypes = [_vp, _ip]
+        L.HCORE_hip_plan_get_status.argtypes = [_vp, _ip]
+        L.HCORE_hip_plan_get_sigma.argtypes = [_vp, i, _dp, _ip]
+        L.HCORE_hip_plan_workspace_bytes.argtypes = [_vp]
+        L.HCORE_hip_plan_workspace_bytes.restype = C.c_longlong
+        L.HCORE_hip_plan_flops.argtypes = [_vp, C.POINTER(C.c_ulong), C.POINTER(C.c_ulong)]
+        L.HCORE_hip_plan_destroy.argtypes = [_vp]
+        L.HCORE_hip_plan_destroy.restype = None
+        _lib = L
+    return _lib
+
+
+def header_functions():
+    """Names of every function declared in include/*.h."""
+    names = []
+    for fn in sorted(os.listdir(INCLUDE_DIR)):
+        if not fn.endswith(".h"):
+            continue
+        txt = open(os.path.join(INCLUDE_DIR, fn)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b([A-Za-z_]\w*)\s*\(", txt, flags=re.M):
+            name = m.group(1)
+            if name in ("if", "while", "for", "defined") or name.startswith("__"):
+                continue
+            names.append(name)
+    return sorted(set(names))
+
+
+def set_error_policy(policy):
+    return lib().HCORE_hip_set_error_policy(policy)
+
+
+def last_status():
+    return lib().HCORE_hip_last_status()
+
+
+def flop_counts(op, a, b=0, c=0, d=0):
+    return int(lib().flop_counts(op.encode(), a, b, c, d))
+
+
+def _fortran(a, dtype):
+    return np.asfortranarray(np.asarray(a, dtype=dtype))
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_fp if a.dtype == np.float32 else _dp)
+
+
+def dgemm(alpha, AU, AV, BU, BV, beta, CU, CV, rk=0, maxrk=None, acc=1e-8, dtype=np.float64,
+          flops=None):
+    """HCORE_dgemm (or HCORE_sgemm for dtype float32) on host numpy factors.
+
+    Returns (CU_new, CV_new, rank).  CU/CV are copied into buffers with room
+    for K = rank(C) + rank(A) columns, updated in place by the library.
+    """
+    M = AU.shape[0]
+    ra, rb, rc = AU.shape[1], BU.shape[1], CU.shape[1]
+    if maxrk is None:
+        maxrk = M
+    cap = max(ra + rc, 1)
+    AU, AV, BU, BV = (_fortran(x, dtype) for x in (AU, AV, BU, BV))
+    CUw = np.zeros((M, cap), dtype=dtype, order="F")
+    CVw = np.zeros((M, cap), dtype=dtype, order="F")
+    CUw[:, :rc] = CU
+    CVw[:, :rc] = CV
+    rks = np.array([ra, rb, rc], dtype=dtype)
+    fc = flops if flops is not None else FlopCounter()
+    L = lib()
+    fn = L.HCORE_dgemm if dtype == np.float64 else L.HCORE_sgemm
+    fn(111, 112, M, M, alpha, _ptr(AU), _ptr(AV), _ptr(rks[0:1]), M, _ptr(BU), _ptr(BV),
+       _ptr(rks[1:2]), M, beta, _ptr(CUw), _ptr(CVw), _ptr(rks[2:3]), M, rk, maxrk, acc, None,
+       C.byref(fc))
+    r = int(rks[2])
+    return CUw[:, :r].copy(order="F"), CVw[:, :r].copy(order="F"), r
+
+
+def dsyrk(alpha, AU, AV, CD, flops=None):
+    """HCORE_dsyrk on host numpy arrays; returns the updated CD."""
+    AU, AV = _fortran(AU, np.float64), _fortran(AV, np.float64)
+    CDw = np.array(CD, dtype=np.float64, order="F", copy=True)
+    M, K = AU.shape
+    fc = flops if flops is not None else FlopCounter()
+    lib().HCORE_dsyrk(122, 111, M, K, alpha, _ptr(AU), M, _ptr(AV), M, 1.0, _ptr(CDw), M,
+                      None, C.byref(fc))
+    return CDw
+
+
+def dgemmcd(Au, Av, Bu, Bv, CD):
+    """HCORE_dgemmcd on host numpy arrays; returns CD - (Au Av^T)(Bu Bv^T)^T."""
+    Au, Av, Bu, Bv = (_fortran(x, np.float64) for x in (Au, Av, Bu, Bv))
+    CDw = np.array(CD, dtype=np.float64, order="F", copy=True)
+    M = Au.shape[0]
+    fc = FlopCounter()
+    lib().HCORE_dgemmcd(111, 112, M, -1.0, _ptr(Au), _ptr(Av), Au.shape[1], M, _ptr(Bu),
+                        _ptr(Bv), Bu.shape[1], M, 1.0, _ptr(CDw), M, M, None, C.byref(fc))
+    return CDw
+
+
+def _parr(ptrs):
+    return (C.c_void_p * max(len(ptrs), 1))(*[int(p) for p in ptrs])
+
+
+def _iarr(v):
+    a = np.ascontiguousarray(v, dtype=np.int32)
+    return a, a.ctypes.data_as(_ip)
+
+
+class Plan:
+    """Batched plan of one TLR trailing step (include/hcore_hip.h).
+
+    Pointers are integer device addresses (e.g. torch tensor .data_ptr()).
+    """
+
+    def __init__(self, M, precision=FP64):
+        self.L = lib()
+        h = C.c_void_p()
+        st = self.L.HCORE_hip_plan_create(C.byref(h), M, precision)
+        if st:
+            raise RuntimeError("HCORE_hip_plan_create failed: %d" % st)
+        self.h = h
+        self.M = M
+        self._keep = []
+        self.n = 0
+
+    def add_dgemm(self, AU, AV, ra, lda, BU, BV, rb, ldb, CU, CV, rc, ldc, alpha, beta, rk,
+                  maxrk, acc, CUout=None, CVout=None, ldco=0):
+        n = len(AU)
+        arrs = [_parr(x) for x in (AU, AV, BU, BV, CU, CV)]
+        (ra_a, ra_p), (rb_a, rb_p), (rc_a, rc_p) = _iarr(ra), _iarr(rb), _iarr(rc)
+        cuo = _parr(CUout) if CUout is not None else None
+        cvo = _parr(CVout) if CVout is not None else None
+        self._keep += arrs + [ra_a, rb_a, rc_a, cuo, cvo]
+        st = self.L.HCORE_hip_plan_add_dgemm(self.h, n, arrs[0], arrs[1], ra_p, lda, arrs[2],
+                                             arrs[3], rb_p, ldb, arrs[4], arrs[5], rc_p, ldc,
+                                             cuo, cvo, ldco, alpha, beta, rk, maxrk, acc)
+        if st:
+            raise RuntimeError("HCORE_hip_plan_add_dgemm failed: %d" % st)
+        self.n = n
+
+    def add_dsyrk(self, AU, AV, K, ldau, CD, ldcd, alpha):
+        n = len(AU)
+        arrs = [_parr(x) for x in (AU, AV, CD)]
+        k_a, k_p = _iarr(K)
+        self._keep += arrs + [k_a]
+        st = self.L.HCORE_hip_plan_add_dsyrk(self.h, n, arrs[0], arrs[1], k_p, ldau, arrs[2],
+                                             ldcd, alpha)
+        if st:
+            raise RuntimeError("HCORE_hip_plan_add_dsyrk failed: %d" % st)
+
+    def finalize(self):
+        st = self.L.HCORE_hip_plan_finalize(self.h)
+        if st:
+            raise RuntimeError("HCORE_hip_plan_finalize failed: %d" % st)
+
+    def execute(self, stream=0, graph=False):
+        fn = self.L.HCORE_hip_plan_execute_graph if graph else self.L.HCORE_hip_plan_execute
+        st = fn(self.h, C.c_void_p(stream))
+        if st:
+            raise RuntimeError("plan execute failed: %d" % st)
+
+    def ranks(self):
+        out = np.zeros(max(self.n, 1), dtype=np.int32)
+        st = self.L.HCORE_hip_plan_get_ranks(self.h, out.ctypes.data_as(_ip))
+        if st:
+            raise RuntimeError("get_ranks failed: %d" % st)
+        return out[:self.n]
+
+    def status(self):
+        out = np.zeros(max(self.n, 1), dtype=np.int32)
+        st = self.L.HCORE_hip_plan_get_status(self.h, out.ctypes.data_as(_ip))
+        if st:
+            raise RuntimeError("get_status failed: %d" % st)
+        return out[:self.n]
+
+    def sigma(self, u):
+        K = C.c_int(0)
+        self.L.HCORE_hip_plan_get_sigma(self.h, u, None, C.byref(K))
+        out = np.zeros(max(K.value, 1))
+        self.L.HCORE_hip_plan_get_sigma(self.h, u, out.ctypes.data_as(_dp), C.byref(K))
+        return out[:K.value]
+
+    def ranks_device_ptr(self):
+        return self.L.HCORE_hip_plan_ranks_device(self.h)
+
+    def workspace_bytes(self):
+        return int(self.L.HCORE_hip_plan_workspace_bytes(self.h))
+
+    def flops(self):
+        fu, fs = C.c_ulong(0), C.c_ulong(0)
+        st = self.L.HCORE_hip_plan_flops(self.h, C.byref(fu), C.byref(fs))
+        if st:
+            raise RuntimeError("plan flops failed: %d" % st)
+        return fu.value, fs.value
+
+    def destroy(self):
+        if self.h:
+            self.L.HCORE_hip_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

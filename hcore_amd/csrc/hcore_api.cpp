@@ -1,0 +1,876 @@
+// hcore_api.cpp — host side of the MI355X-native HCORE: library context,
+// device workspace pool, batched plans and the drop-in C entry points
+// HCORE_dgemm / HCORE_sgemm / HCORE_dsyrk / HCORE_dgemmcd.
+//
+// The drop-in entry points keep the reference's contracts (argument meaning,
+// in-place outputs, double-typed ranks, exit(-1) on fatal conditions) and run
+// every arithmetic step on the GPU through the same batched plan machinery
+// that a TLR Cholesky trailing step uses (one update = a batch of one).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <vector>
+
+#include "flop_counts.h"
+#include "hcore_d.h"
+#include "hcore_hip.h"
+#include "hcore_internal.hpp"
+#include "hcore_s.h"
+
+// ---- data symbols kept from the reference (hcore_dgemm.c:37-43 etc.) --------
+extern "C" {
+__attribute__((weak)) int use_scratch = 0;
+int use_trmm = 0;
+int gemm_print_index = 0;
+int gemm_print_mat = 0;
+int hc_nelm_limit = 512;
+int sgemm_use_trmm = 0;
+int sgemm_print_index = 0;
+int sgemm_print_mat = 0;
+int syrk_print_index = 0;
+int gemmcd_print_index = 0;
+int gemmcd_print_mat = 0;
+int gemmcd_use_scratch = 0;
+
+// Closed-form operation counts (reference src/misc/descutil/flop_counts.c:3-70),
+// including its float-precision thirds in the 'q' formula.
+unsigned long int flop_counts(char op, unsigned long int a, unsigned long int b,
+                              unsigned long int c, unsigned long int d) {
+  unsigned long int r = 0;
+  switch (op) {
+    case 'q':
+      r = 2 * a * b * b - (unsigned long int)(2 * b * b * b / 3.0f) + 2 * a * b +
+          (unsigned long int)(17 * b / 3.0f);
+      break;
+    case 'c': r = (unsigned long int)(a * a * a / 3 - a * a / 2.0 + a / 6); break;
+    case 't': r = (c == 1) ? b * a * a : (c == 2 ? a * b * b : 0); break;
+    case 'm': r = a * b * c * 2; break;
+    case 's': r = 22 * a * a * a; break;
+    case 'o':
+      if (d == 1) r = 4 * b * a * c - 2 * b * c * c + 3 * b * c;
+      else if (d == 2) r = 4 * b * a * c - 2 * a * c * c + 2 * a * c + b * c - c * c / 2 + c / 2;
+      break;
+    case 'r': r = (c == 1) ? b * a * a : (c == 2 ? a * b * b : 0); break;
+    default: break;
+  }
+  return r;
+}
+}  // extern "C"
+
+namespace hcore {
+
+// ------------------------------------------------------------------------------
+// context
+// ------------------------------------------------------------------------------
+struct Context {
+  std::mutex mu;
+  bool ready = false;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int policy = HCORE_HIP_ON_ERROR_EXIT;
+  int core_kl_d = 0, core_kl_f = 0;
+  // grow-only device arena used by the single-call entry points
+  char* arena = nullptr;
+  size_t arena_cap = 0;
+};
+static Context g_ctx;
+static thread_local int t_last_status = HCORE_HIP_OK;
+
+// HCORE_ON_ERROR=return selects the recording policy for the whole process.
+__attribute__((constructor)) static void policy_from_env() {
+  const char* pol = getenv("HCORE_ON_ERROR");
+  if (pol && strcmp(pol, "return") == 0) g_ctx.policy = HCORE_HIP_ON_ERROR_RETURN;
+}
+
+static int hip_fail(hipError_t e, const char* what) {
+  fprintf(stderr, "hcore_amd: HIP error in %s: %s\n", what, hipGetErrorString(e));
+  return HCORE_HIP_ERR_RUNTIME;
+}
+#define HCHECK(call)                                          \
+  do {                                                        \
+    hipError_t e_ = (call);                                   \
+    if (e_ != hipSuccess) return hip_fail(e_, #call);         \
+  } while (0)
+
+static int ctx_init_locked(int device) {
+  if (g_ctx.ready) return HCORE_HIP_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    fprintf(stderr, "hcore_amd: no HIP device available\n");
+    return HCORE_HIP_ERR_RUNTIME;
+  }
+  if (device < 0) HCHECK(hipGetDevice(&device));
+  HCHECK(hipSetDevice(device));
+  HCHECK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+  g_ctx.device = device;
+  g_ctx.core_kl_d = core_lds_max_k(sizeof(double));
+  g_ctx.core_kl_f = core_lds_max_k(sizeof(float));
+  prepare_kernels<double>();
+  prepare_kernels<float>();
+  g_ctx.ready = true;
+  return HCORE_HIP_OK;
+}
+
+static int ctx_ensure() {
+  if (g_ctx.ready) return HCORE_HIP_OK;
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  return ctx_init_locked(-1);
+}
+
+template <typename T>
+static int core_kl() { return sizeof(T) == 8 ? g_ctx.core_kl_d : g_ctx.core_kl_f; }
+
+// ------------------------------------------------------------------------------
+// host-side descriptor packing
+// ------------------------------------------------------------------------------
+struct Packer {
+  std::vector<char> buf;
+  template <typename X>
+  size_t add(const std::vector<X>& v) {
+    size_t off = (buf.size() + 255) & ~size_t(255);
+    buf.resize(off + v.size() * sizeof(X));
+    if (!v.empty()) memcpy(buf.data() + off, v.data(), v.size() * sizeof(X));
+    return off;
+  }
+};
+
+static void add_tasks(std::vector<GemmTask>& tasks, int prob, int m, int n) {
+  for (int tj = 0; tj < (n + 63) / 64; ++tj)
+    for (int ti = 0; ti < (m + 63) / 64; ++ti) tasks.push_back({prob, ti, tj});
+}
+
+}  // namespace hcore
+
+// ------------------------------------------------------------------------------
+// batched plan
+// ------------------------------------------------------------------------------
+struct hcore_hip_plan {
+  int M = 0;
+  int prec = HCORE_HIP_FP64;
+  bool finalized = false;
+  bool use_arena = false;  // workspace from the context arena (single calls)
+  // gemm group
+  int ng = 0;
+  std::vector<void*> AU, AV, BU, BV, CU, CV, CUo, CVo;
+  std::vector<int> ra, rb, rc;
+  int lda = 0, ldb = 0, ldc = 0, ldco = 0;
+  double alpha = 1, beta = 1, acc = 0;
+  int rk = 0, maxrk = 0;
+  // syrk group
+  int ns = 0;
+  std::vector<void*> sAU, sAV, sCD;
+  std::vector<int> sK;
+  int ldau = 0, ldcd = 0;
+  double salpha = -1;
+  // device memory
+  char* dmem = nullptr;  // owned unless use_arena
+  size_t dbytes = 0;
+  size_t ws_bytes = 0;
+  char* desc = nullptr;
+  int* d_rank = nullptr;
+  int* d_status = nullptr;
+  std::vector<size_t> sigma_off;  // per update, element offset into ws
+  char* ws = nullptr;
+  // descriptor offsets and counts
+  size_t o_stack = 0, o_gp = 0, o_tF = 0, o_tG = 0, o_tT = 0, o_tZ = 0, o_qr = 0, o_core_l = 0,
+         o_core_g = 0, o_apply = 0, o_sp = 0, o_tW = 0, o_tX = 0, o_tC = 0;
+  int n_tF = 0, n_tG = 0, n_tT = 0, n_tZ = 0, n_qr = 0, n_core_l = 0, n_core_g = 0, n_apply = 0,
+      n_tW = 0, n_tX = 0, n_tC = 0;
+  int kmax_l = 1, kmax_g = 1;
+  // graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  // last ranks (host) for flop accounting
+  std::vector<int> last_ranks;
+};
+
+namespace hcore {
+
+template <typename T>
+static int plan_finalize_t(hcore_hip_plan* p) {
+  const int M = p->M;
+  const int n = p->ng;
+  const int KL = core_kl<T>();
+  const size_t es = sizeof(T);
+  auto al = [](size_t x) { return (x + 63) & ~size_t(63); };  // elements, 512 B for FP64
+
+  // ---- workspace layout (elements of T) ----
+  std::vector<int> Kv(n);
+  std::vector<size_t> o_Us(n), o_Vs(n), o_tA(n), o_tB(n), o_TA(n), o_TB(n), o_F(n), o_Tc(n),
+      o_Tj(n), o_Zu(n), o_Zv(n), o_sg(n);
+  size_t off = 0;
+  for (int u = 0; u < n; ++u) {
+    const int K = p->ra[u] + p->rc[u];
+    Kv[u] = K;
+    o_Us[u] = off; off += al((size_t)M * K);
+    o_Vs[u] = off; off += al((size_t)M * K);
+    o_tA[u] = off; off += al(K);
+    o_tB[u] = off; off += al(K);
+    o_TA[u] = off; off += al(tp_elems(K));
+    o_TB[u] = off; off += al(tp_elems(K));
+    o_F[u] = off; off += al((size_t)p->ra[u] * p->rb[u]);
+    o_Tc[u] = off; off += al((size_t)K * K);
+    o_Tj[u] = off; off += (K > KL) ? al((size_t)K * K) : 0;
+    o_Zu[u] = off; off += al((size_t)K * K);
+    o_Zv[u] = off; off += al((size_t)K * K);
+    o_sg[u] = off; off += al(K);
+  }
+  std::vector<size_t> o_W(p->ns), o_X(p->ns);
+  for (int s = 0; s < p->ns; ++s) {
+    const int K = p->sK[s];
+    o_W[s] = off; off += al((size_t)K * K);
+    o_X[s] = off; off += al((size_t)M * K);
+  }
+  p->sigma_off = o_sg;
+  const size_t ws_elems = off;
+
+  // processing order: largest stacked rank first (longest jobs start first)
+  std::vector<int> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return Kv[a] > Kv[b]; });
+
+  // ---- device memory: [rank | status | workspace | descriptors] ----
+  Packer pk;
+  // descriptor contents reference device addresses, so the base must be known
+  // first: size the descriptors with a dry run, then allocate, then fill.
+  auto build = [&](char* wsbase, int* d_rank, int* d_status) {
+    T* W = reinterpret_cast<T*>(wsbase);
+    pk.buf.clear();
+    std::vector<StackProb<T>> stack(n);
+    std::vector<GemmProb<T>> gp;
+    std::vector<GemmTask> tF, tG, tT, tZ, tW, tX, tC;
+    std::vector<QrProb<T>> qr;
+    std::vector<CoreProb<T>> core_l, core_g;
+    std::vector<ApplyProb<T>> ap;
+    for (int idx = 0; idx < n; ++idx) {
+      const int u = order[idx];
+      const int K = Kv[u], ra = p->ra[u], rb = p->rb[u], rc = p->rc[u];
+      T* Us = W + o_Us[u];
+      T* Vs = W + o_Vs[u];
+      stack[idx] = {(const T*)p->CU[u], (const T*)p->CV[u], (const T*)p->AU[u], Us, Vs, rc, ra,
+                    p->ldc, p->lda};
+      // F = AV^T BV   (hcore_dgemm.c:175-178)
+      GemmProb<T> g{};
+      g.A = (const T*)p->AV[u]; g.B = (const T*)p->BV[u]; g.C = W + o_F[u];
+      g.m = ra; g.n = rb; g.k = M; g.lda = p->lda; g.ldb = p->ldb; g.ldc = ra;
+      g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_BETA_ZERO;
+      add_tasks(tF, (int)gp.size(), g.m, g.n); gp.push_back(g);
+      // G = BU F^T into Vs(:, rc:K)   (hcore_dgemm.c:199-205)
+      g = GemmProb<T>{};
+      g.A = (const T*)p->BU[u]; g.B = W + o_F[u]; g.C = Vs + (size_t)rc * M;
+      g.m = M; g.n = ra; g.k = rb; g.lda = p->ldb; g.ldb = ra; g.ldc = M;
+      g.alpha = T(1); g.beta = T(0); g.flags = GF_B_TRANS | GF_BETA_ZERO;
+      add_tasks(tG, (int)gp.size(), g.m, g.n); gp.push_back(g);
+      // T = R_A R_B^T   (hcore_dgemm.c:384-386)
+      g = GemmProb<T>{};
+      g.A = Us; g.B = Vs; g.C = W + o_Tc[u];
+      g.m = K; g.n = K; g.k = K; g.lda = M; g.ldb = M; g.ldc = K;
+      g.alpha = T(1); g.beta = T(0);
+      g.flags = GF_A_UPPER | GF_B_TRANS | GF_B_UPPER | GF_BETA_ZERO;
+      add_tasks(tT, (int)gp.size(), g.m, g.n); gp.push_back(g);
+      // Zv = T^T Zu = V_r S_r
+      g = GemmProb<T>{};
+      g.A = W + o_Tc[u]; g.B = W + o_Zu[u]; g.C = W + o_Zv[u];
+      g.m = K; g.n = K; g.k = K; g.lda = K; g.ldb = K; g.ldc = K;
+      g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_BETA_ZERO;
+      g.n_dev = d_rank + u; g.skip = d_status + u;
+      add_tasks(tZ, (int)gp.size(), g.m, g.n); gp.push_back(g);
+      // QR of both stacks
+      qr.push_back({Us, W + o_tA[u], W + o_TA[u], K});
+      qr.push_back({Vs, W + o_tB[u], W + o_TB[u], K});
+      CoreProb<T> c{W + o_Tc[u], (K > KL) ? W + o_Tj[u] : nullptr, W + o_Zu[u], W + o_sg[u],
+                    d_rank + u, d_status + u, K};
+      if (K <= KL) { core_l.push_back(c); p->kmax_l = std::max(p->kmax_l, K); }
+      else { core_g.push_back(c); p->kmax_g = std::max(p->kmax_g, K); }
+      T* CUo = (T*)(p->CUo[u] ? p->CUo[u] : p->CU[u]);
+      T* CVo = (T*)(p->CVo[u] ? p->CVo[u] : p->CV[u]);
+      ap.push_back({Us, W + o_TA[u], W + o_Zu[u], CUo, d_rank + u, d_status + u, K, p->ldco});
+      ap.push_back({Vs, W + o_TB[u], W + o_Zv[u], CVo, d_rank + u, d_status + u, K, p->ldco});
+    }
+    // syrk (hcore_dsyrk.c:103, 144, 165-181)
+    std::vector<GemmProb<T>> sp;
+    for (int s = 0; s < p->ns; ++s) {
+      const int K = p->sK[s];
+      GemmProb<T> g{};
+      g.A = (const T*)p->sAV[s]; g.B = (const T*)p->sAV[s]; g.C = W + o_W[s];
+      g.m = K; g.n = K; g.k = p->ldau; g.lda = p->ldau; g.ldb = p->ldau; g.ldc = K;
+      g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_BETA_ZERO;
+      add_tasks(tW, (int)sp.size(), g.m, g.n); sp.push_back(g);
+      g = GemmProb<T>{};
+      g.A = (const T*)p->sAU[s]; g.B = W + o_W[s]; g.C = W + o_X[s];
+      g.m = M; g.n = K; g.k = K; g.lda = p->ldau; g.ldb = K; g.ldc = M;
+      g.alpha = T(1); g.beta = T(0); g.flags = GF_BETA_ZERO;
+      add_tasks(tX, (int)sp.size(), g.m, g.n); sp.push_back(g);
+      g = GemmProb<T>{};
+      g.A = W + o_X[s]; g.B = (const T*)p->sAU[s]; g.C = (T*)p->sCD[s];
+      g.m = M; g.n = M; g.k = K; g.lda = M; g.ldb = p->ldau; g.ldc = p->ldcd;
+      g.alpha = T(p->salpha); g.beta = T(1); g.flags = GF_B_TRANS | GF_LOWER_ZERO_UPPER;
+      add_tasks(tC, (int)sp.size(), g.m, g.n); sp.push_back(g);
+    }
+    p->o_stack = pk.add(stack);
+    p->o_gp = pk.add(gp);
+    p->o_tF = pk.add(tF); p->n_tF = (int)tF.size();
+    p->o_tG = pk.add(tG); p->n_tG = (int)tG.size();
+    p->o_tT = pk.add(tT); p->n_tT = (int)tT.size();
+    p->o_tZ = pk.add(tZ); p->n_tZ = (int)tZ.size();
+    p->o_qr = pk.add(qr); p->n_qr = (int)qr.size();
+    p->o_core_l = pk.add(core_l); p->n_core_l = (int)core_l.size();
+    p->o_core_g = pk.add(core_g); p->n_core_g = (int)core_g.size();
+    p->o_apply = pk.add(ap); p->n_apply = (int)ap.size();
+    p->o_sp = pk.add(sp);
+    p->o_tW = pk.add(tW); p->n_tW = (int)tW.size();
+    p->o_tX = pk.add(tX); p->n_tX = (int)tX.size();
+    p->o_tC = pk.add(tC); p->n_tC = (int)tC.size();
+  };
+  build(nullptr, nullptr, nullptr);  // dry run: descriptor size
+  const size_t head = ((size_t)std::max(n, 1) * 2 * sizeof(int) + 255) & ~size_t(255);
+  const size_t wsb = (ws_elems * es + 255) & ~size_t(255);
+  const size_t total = head + wsb + pk.buf.size() + 256;
+  char* base = nullptr;
+  if (p->use_arena) {
+    if (g_ctx.arena_cap < total) {
+      if (g_ctx.arena) HCHECK(hipFree(g_ctx.arena));
+      g_ctx.arena = nullptr;
+      size_t cap = std::max(total, g_ctx.arena_cap * 2);
+      HCHECK(hipMalloc(&g_ctx.arena, cap));
+      g_ctx.arena_cap = cap;
+    }
+    base = g_ctx.arena;
+  } else {
+    HCHECK(hipMalloc(&base, total));
+    p->dmem = base;
+  }
+  p->dbytes = total;
+  p->ws_bytes = wsb;
+  p->d_rank = reinterpret_cast<int*>(base);
+  p->d_status = p->d_rank + std::max(n, 1);
+  p->ws = base + head;
+  p->desc = base + head + wsb;
+  p->kmax_l = p->kmax_g = 1;
+  build(p->ws, p->d_rank, p->d_status);
+  HCHECK(hipMemcpyAsync(p->desc, pk.buf.data(), pk.buf.size(), hipMemcpyHostToDevice,
+                        g_ctx.stream));
+  HCHECK(hipMemsetAsync(base, 0, head, g_ctx.stream));
+  HCHECK(hipStreamSynchronize(g_ctx.stream));
+  p->finalized = true;
+  return HCORE_HIP_OK;
+}
+
+template <typename T>
+static int plan_execute_t(hcore_hip_plan* p, hipStream_t s) {
+  const int M = p->M;
+  auto D = [&](size_t off) { return p->desc + off; };
+  const auto* gp = reinterpret_cast<const GemmProb<T>*>(D(p->o_gp));
+  if (p->ng > 0) {
+    launch_stack<T>(reinterpret_cast<const StackProb<T>*>(D(p->o_stack)), p->ng, M, T(p->alpha),
+                    T(p->beta), s);
+    launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(p->o_tF)), p->n_tF, s);
+    launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(p->o_tG)), p->n_tG, s);
+    launch_geqrf<T>(reinterpret_cast<const QrProb<T>*>(D(p->o_qr)), p->n_qr, M, s);
+    launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(p->o_tT)), p->n_tT, s);
+    launch_core<T>(reinterpret_cast<const CoreProb<T>*>(D(p->o_core_l)), p->n_core_l, true,
+                   p->kmax_l, p->rk, p->maxrk, T(p->acc), s);
+    launch_core<T>(reinterpret_cast<const CoreProb<T>*>(D(p->o_core_g)), p->n_core_g, false,
+                   p->kmax_g, p->rk, p->maxrk, T(p->acc), s);
+    launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(p->o_tZ)), p->n_tZ, s);
+    launch_apply<T>(reinterpret_cast<const ApplyProb<T>*>(D(p->o_apply)), p->n_apply, M, s);
+  }
+  if (p->ns > 0) {
+    const auto* sp = reinterpret_cast<const GemmProb<T>*>(D(p->o_sp));
+    launch_gemm<T>(sp, reinterpret_cast<const GemmTask*>(D(p->o_tW)), p->n_tW, s);
+    launch_gemm<T>(sp, reinterpret_cast<const GemmTask*>(D(p->o_tX)), p->n_tX, s);
+    launch_gemm<T>(sp, reinterpret_cast<const GemmTask*>(D(p->o_tC)), p->n_tC, s);
+  }
+  HCHECK(hipGetLastError());
+  return HCORE_HIP_OK;
+}
+
+static bool is_device_ptr(const void* ptr) {
+  if (!ptr) return false;
+  hipPointerAttribute_t a;
+  hipError_t e = hipPointerGetAttributes(&a, ptr);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged ||
+         a.type == hipMemoryTypeUnified;
+}
+
+// Report a fatal condition the way the reference does, then apply the policy.
+static void fail(int status, const char* msg) {
+  t_last_status = status;
+  fprintf(stderr, "%s\n", msg);
+  if (g_ctx.policy == HCORE_HIP_ON_ERROR_EXIT) exit(-1);
+}
+
+// ------------------------------------------------------------------------------
+// single-call recompression (HCORE_dgemm / HCORE_sgemm)
+// ------------------------------------------------------------------------------
+template <typename T>
+static void single_gemm(int M, T alpha, T* AU, T* AV, T* Ark, int LDA, T* BU, T* BV, T* Brk,
+                        int LDB, T beta, T* CU, T* CV, T* Crk, int LDC, int rk, int maxrk, T acc,
+                        flop_counter* flops, const char* name) {
+  t_last_status = HCORE_HIP_OK;
+  char msg[512];
+  // ranks may live on the host or the device
+  T rks[3];
+  T* rp[3] = {Ark, Brk, Crk};
+  for (int i = 0; i < 3; ++i) {
+    if (is_device_ptr(rp[i])) {
+      if (hipMemcpy(&rks[i], rp[i], sizeof(T), hipMemcpyDeviceToHost) != hipSuccess) {
+        fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: cannot read rank");
+        return;
+      }
+    } else {
+      rks[i] = *rp[i];
+    }
+  }
+  const int ra = (int)rks[0], rb = (int)rks[1], rc = (int)rks[2];
+  if (ra == 0 || rb == 0 || rc == 0) {
+    snprintf(msg, sizeof msg, "%s: _Ark=%d _Brk=%d _Crk=%d. These rank values should not be zero.",
+             name, ra, rb, rc);
+    return fail(HCORE_HIP_ERR_ZERO_RANK, msg);
+  }
+  if (ra < 0 || rb < 0 || rc < 0 || M <= 0) {
+    return fail(HCORE_HIP_ERR_ARG, "hcore_amd: negative rank or size");
+  }
+  const int K = ra + rc;
+  if (K > 2 * maxrk) {
+    snprintf(msg, sizeof msg, "%s: Sum of ranks (%d) is too big! _Ark:%d _Crk:%d maxrank:%d (x2: %d)",
+             name, K, ra, rc, maxrk, 2 * maxrk);
+    return fail(HCORE_HIP_ERR_SUM_RANKS, msg);
+  }
+  if (maxrk < K || K > M) {
+    snprintf(msg, sizeof msg, "%s: Increase maxrank. %d is not enough ld_newV:%d CU_ncols:%d",
+             name, maxrk, maxrk, K);
+    return fail(HCORE_HIP_ERR_K_GT_MAXRK, msg);
+  }
+  // argument checks above need no GPU; everything below runs on it
+  if (ctx_ensure() != HCORE_HIP_OK) {
+    fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: GPU initialisation failed");
+    return;
+  }
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  hipStream_t s = g_ctx.stream;
+  const bool dev = is_device_ptr(CU) && is_device_ptr(CV) && is_device_ptr(AU) &&
+                   is_device_ptr(AV) && is_device_ptr(BU) && is_device_ptr(BV);
+  hcore_hip_plan p;
+  p.M = M;
+  p.prec = sizeof(T) == 8 ? HCORE_HIP_FP64 : HCORE_HIP_FP32;
+  p.use_arena = true;
+  p.ng = 1;
+  p.ra = {ra}; p.rb = {rb}; p.rc = {rc};
+  p.alpha = alpha; p.beta = beta; p.rk = rk; p.maxrk = maxrk; p.acc = acc;
+  // staging buffers live in a separate block of the arena after the plan
+  T *dAU = AU, *dAV = AV, *dBU = BU, *dBV = BV, *dCU = CU, *dCV = CV;
+  T *oCU = nullptr, *oCV = nullptr;
+  void* stage = nullptr;
+  if (!dev) {
+    const size_t nA = (size_t)M * ra, nB = (size_t)M * rb, nC = (size_t)M * K;
+    const size_t bytes = sizeof(T) * (2 * nA + 2 * nB + 4 * nC) + 1024;
+    if (hipMalloc(&stage, bytes) != hipSuccess) {
+      return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: staging allocation failed");
+    }
+    T* q = (T*)stage;
+    dAU = q; q += nA; dAV = q; q += nA; dBU = q; q += nB; dBV = q; q += nB;
+    dCU = q; q += nC; dCV = q; q += nC; oCU = q; q += nC; oCV = q;
+    auto up = [&](T* d, const T* h, int ld, int cols) {
+      return hipMemcpy2DAsync(d, sizeof(T) * M, h, sizeof(T) * ld, sizeof(T) * M, cols,
+                              hipMemcpyHostToDevice, s);
+    };
+    if (up(dAU, AU, LDA, ra) || up(dAV, AV, LDA, ra) || up(dBU, BU, LDB, rb) ||
+        up(dBV, BV, LDB, rb) || up(dCU, CU, LDC, rc) || up(dCV, CV, LDC, rc)) {
+      hipFree(stage);
+      return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: host-to-device staging failed");
+    }
+    p.lda = M; p.ldb = M; p.ldc = M; p.ldco = M;
+    p.CUo = {oCU}; p.CVo = {oCV};
+  } else {
+    p.lda = LDA; p.ldb = LDB; p.ldc = LDC; p.ldco = LDC;
+    p.CUo = {nullptr}; p.CVo = {nullptr};
+  }
+  p.AU = {dAU}; p.AV = {dAV}; p.BU = {dBU}; p.BV = {dBV}; p.CU = {dCU}; p.CV = {dCV};
+  int st = plan_finalize_t<T>(&p);
+  if (st == HCORE_HIP_OK) st = plan_execute_t<T>(&p, s);
+  int r = 0, dst = 0;
+  if (st == HCORE_HIP_OK) {
+    if (hipMemcpyAsync(&r, p.d_rank, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(&dst, p.d_status, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      st = HCORE_HIP_ERR_RUNTIME;
+  }
+  if (st != HCORE_HIP_OK) {
+    if (stage) hipFree(stage);
+    return fail(st, "hcore_amd: GPU execution failed");
+  }
+  if (dst == ST_R_GT_MAXRK) {
+    if (stage) hipFree(stage);
+    snprintf(msg, sizeof msg, "%s: Rank after truncation is too big! finalrank:%d maxrank:%d", name,
+             r, maxrk);
+    return fail(HCORE_HIP_ERR_R_GT_MAXRK, msg);
+  }
+  if (dst != ST_OK) {
+    if (stage) hipFree(stage);
+    snprintf(msg, sizeof msg, "%s: ERROR in the core SVD (non-finite values), K=%d", name, K);
+    return fail(HCORE_HIP_ERR_NUMERIC, msg);
+  }
+  if (!dev) {
+    hipError_t e1 = hipMemcpy2DAsync(CU, sizeof(T) * LDC, oCU, sizeof(T) * M, sizeof(T) * M, r,
+                                     hipMemcpyDeviceToHost, s);
+    hipError_t e2 = hipMemcpy2DAsync(CV, sizeof(T) * LDC, oCV, sizeof(T) * M, sizeof(T) * M, r,
+                                     hipMemcpyDeviceToHost, s);
+    hipError_t e3 = hipStreamSynchronize(s);
+    hipFree(stage);
+    if (e1 || e2 || e3) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: device-to-host copy failed");
+  }
+  if (is_device_ptr(Crk)) {
+    T rv = (T)r;
+    (void)hipMemcpy(Crk, &rv, sizeof(T), hipMemcpyHostToDevice);
+  } else {
+    Crk[0] = (T)r;
+  }
+  if (flops) {
+    // the reference's increments (hcore_dgemm.c:126, 179, 208, 224, 434, 493, 595, 666)
+    unsigned long f = flop_counts('q', M, K, 0, 0) + flop_counts('m', ra, rb, M, 0) +
+                      flop_counts('m', M, ra, rb, 0) + flop_counts('q', M, K, 0, 0) +
+                      flop_counts('s', K, 0, 0, 0) + (unsigned long)r * (unsigned long)K +
+                      flop_counts('o', M, r, K, 1) + flop_counts('o', r, M, K, 2);
+    flops->update += f;
+  }
+}
+
+// Run a short list of dependent GEMMs on the context stream (dsyrk, dgemmcd).
+static int run_gemm_chain(const std::vector<GemmProb<double>>& gp) {
+  std::vector<std::vector<GemmTask>> tl(gp.size());
+  Packer pk;
+  size_t o_gp = pk.add(gp);
+  std::vector<size_t> o_t(gp.size());
+  for (size_t i = 0; i < gp.size(); ++i) {
+    add_tasks(tl[i], (int)i, gp[i].m, gp[i].n);
+    o_t[i] = pk.add(tl[i]);
+  }
+  char* d = nullptr;
+  HCHECK(hipMalloc(&d, pk.buf.size() + 256));
+  hipStream_t s = g_ctx.stream;
+  HCHECK(hipMemcpyAsync(d, pk.buf.data(), pk.buf.size(), hipMemcpyHostToDevice, s));
+  for (size_t i = 0; i < gp.size(); ++i)
+    launch_gemm<double>(reinterpret_cast<const GemmProb<double>*>(d + o_gp),
+                        reinterpret_cast<const GemmTask*>(d + o_t[i]), (int)tl[i].size(), s);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  hipFree(d);
+  if (e != hipSuccess) return hip_fail(e, "gemm chain");
+  return HCORE_HIP_OK;
+}
+
+// Device copies of host matrices (rows x cols, leading dimension ld) for the
+// dense-output routines; `dev` true means the caller's pointers are used.
+struct Staged {
+  std::vector<void*> bufs;
+  ~Staged() {
+    for (void* b : bufs) hipFree(b);
+  }
+  double* get(const double* h, int rows, int ld, int cols, bool dev, bool upload) {
+    if (dev) return const_cast<double*>(h);
+    double* d = nullptr;
+    if (hipMalloc(&d, sizeof(double) * (size_t)rows * std::max(cols, 1)) != hipSuccess)
+      return nullptr;
+    bufs.push_back(d);
+    if (upload && cols > 0 &&
+        hipMemcpy2D(d, sizeof(double) * rows, h, sizeof(double) * ld, sizeof(double) * rows, cols,
+                    hipMemcpyHostToDevice) != hipSuccess)
+      return nullptr;
+    return d;
+  }
+};
+
+}  // namespace hcore
+
+using namespace hcore;
+
+// ==============================================================================
+// C API
+// ==============================================================================
+extern "C" {
+
+void HCORE_dgemm(HCORE_enum transA, int transB, int M, int N, double alpha, double* AU,
+                 double* AV, double* Ark, int LDA, double* BU, double* BV, double* Brk, int LDB,
+                 double beta, double* CU, double* CV, double* Crk, int LDC, int rk, int maxrk,
+                 double acc, double* work, flop_counter* flops) {
+  (void)transA; (void)transB; (void)N; (void)work;
+  single_gemm<double>(M, alpha, AU, AV, Ark, LDA, BU, BV, Brk, LDB, beta, CU, CV, Crk, LDC, rk,
+                      maxrk, acc, flops, "HCORE_dgemm");
+}
+
+void HCORE_sgemm(HCORE_enum transA, HCORE_enum transB, int M, int N, float alpha, float* AU,
+                 float* AV, float* Ark, int LDA, float* BU, float* BV, float* Brk, int LDB,
+                 float beta, float* CU, float* CV, float* Crk, int LDC, int rk, int maxrk,
+                 float acc, float* work, flop_counter* flops) {
+  (void)transA; (void)transB; (void)N; (void)work;
+  single_gemm<float>(M, alpha, AU, AV, Ark, LDA, BU, BV, Brk, LDB, beta, CU, CV, Crk, LDC, rk,
+                     maxrk, acc, flops, "HCORE_sgemm");
+}
+
+void HCORE_dsyrk(HCORE_enum uplo, HCORE_enum trans, int M, int K, double alpha, const double* AU,
+                 int LDAU, const double* AV, int LDAV, double beta, double* CD, int LDCD,
+                 double* work, flop_counter* flops) {
+  (void)uplo; (void)trans; (void)LDAV; (void)beta; (void)work;
+  t_last_status = HCORE_HIP_OK;
+  if (ctx_ensure() != HCORE_HIP_OK) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: GPU init failed");
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  const int N = LDAU;  // rows of AV, as in the reference (hcore_dsyrk.c:57)
+  const bool dev = is_device_ptr(AU) && is_device_ptr(AV) && is_device_ptr(CD);
+  Staged st;
+  double* dAU = st.get(AU, LDAU, LDAU, K, dev, true);
+  double* dAV = st.get(AV, LDAU, LDAU, K, dev, true);
+  double* dCD = st.get(CD, LDCD, LDCD, M, dev, true);
+  double* dW = st.get(nullptr, std::max(K, 1), 0, K, false, false);
+  double* dX = st.get(nullptr, M, 0, K, false, false);
+  if (!dAU || !dAV || !dCD || !dW || !dX) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: staging failed");
+  std::vector<GemmProb<double>> gp(3);
+  gp[0] = GemmProb<double>{dAV, dAV, dW, K, K, N, LDAU, LDAU, K, 1.0, 0.0,
+                           GF_A_TRANS | GF_BETA_ZERO, nullptr, nullptr};
+  gp[1] = GemmProb<double>{dAU, dW, dX, M, K, K, LDAU, K, M, 1.0, 0.0, GF_BETA_ZERO, nullptr,
+                           nullptr};
+  gp[2] = GemmProb<double>{dX, dAU, dCD, M, M, K, M, LDAU, LDCD, alpha, 1.0,
+                           GF_B_TRANS | GF_LOWER_ZERO_UPPER, nullptr, nullptr};
+  if (run_gemm_chain(gp) != HCORE_HIP_OK) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: dsyrk failed");
+  if (!dev && hipMemcpy2D(CD, sizeof(double) * LDCD, dCD, sizeof(double) * LDCD,
+                          sizeof(double) * LDCD, M, hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: dsyrk copy-back failed");
+  if (flops)
+    flops->syrk += flop_counts('m', K, K, N, 0) + flop_counts('m', M, K, K, 0) +
+                   flop_counts('m', M, M, K, 0);
+}
+
+void HCORE_dgemmcd(HCORE_enum transA, HCORE_enum transB, int M, double alpha, double* Au,
+                   double* Av, int Arank, int LDA, double* Bu, double* Bv, int Brank, int LDB,
+                   double beta, double* CD, int LDC, int maxrank, double* work,
+                   flop_counter* flops) {
+  (void)transA; (void)transB; (void)alpha; (void)LDA; (void)LDB; (void)beta; (void)LDC;
+  (void)maxrank; (void)work; (void)flops;
+  t_last_status = HCORE_HIP_OK;
+  if (ctx_ensure() != HCORE_HIP_OK) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: GPU init failed");
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  const bool dev = is_device_ptr(Au) && is_device_ptr(Av) && is_device_ptr(Bu) &&
+                   is_device_ptr(Bv) && is_device_ptr(CD);
+  Staged st;
+  // every leading dimension is M (hcore_dgemmcd.c:66-100)
+  double* dAu = st.get(Au, M, M, Arank, dev, true);
+  double* dAv = st.get(Av, M, M, Arank, dev, true);
+  double* dBu = st.get(Bu, M, M, Brank, dev, true);
+  double* dBv = st.get(Bv, M, M, Brank, dev, true);
+  double* dCD = st.get(CD, M, M, M, dev, true);
+  double* dW = st.get(nullptr, std::max(Arank, 1), 0, Brank, false, false);
+  double* dX = st.get(nullptr, M, 0, std::max(Arank, Brank), false, false);
+  if (!dAu || !dAv || !dBu || !dBv || !dCD || !dW || !dX)
+    return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: staging failed");
+  std::vector<GemmProb<double>> gp(3);
+  gp[0] = GemmProb<double>{dAv, dBv, dW, Arank, Brank, M, M, M, Arank, 1.0, 0.0,
+                           GF_A_TRANS | GF_BETA_ZERO, nullptr, nullptr};
+  if (Arank > Brank) {  // X = Au W (M x Brank); CD -= X Bu^T
+    gp[1] = GemmProb<double>{dAu, dW, dX, M, Brank, Arank, M, Arank, M, 1.0, 0.0, GF_BETA_ZERO,
+                             nullptr, nullptr};
+    gp[2] = GemmProb<double>{dX, dBu, dCD, M, M, Brank, M, M, M, -1.0, 1.0, GF_B_TRANS, nullptr,
+                             nullptr};
+  } else {  // X = W Bu^T (Arank x M); CD -= Au X
+    gp[1] = GemmProb<double>{dW, dBu, dX, Arank, M, Brank, Arank, M, Arank, 1.0, 0.0,
+                             GF_B_TRANS | GF_BETA_ZERO, nullptr, nullptr};
+    gp[2] = GemmProb<double>{dAu, dX, dCD, M, M, Arank, M, Arank, M, -1.0, 1.0, 0, nullptr,
+                             nullptr};
+  }
+  if (run_gemm_chain(gp) != HCORE_HIP_OK) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: dgemmcd failed");
+  if (!dev && hipMemcpy(CD, dCD, sizeof(double) * (size_t)M * M, hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: dgemmcd copy-back failed");
+}
+
+// ---- context -----------------------------------------------------------------
+int HCORE_hip_init(int device) {
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  return ctx_init_locked(device);
+}
+
+int HCORE_hip_finalize(void) {
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  if (!g_ctx.ready) return HCORE_HIP_OK;
+  (void)hipStreamSynchronize(g_ctx.stream);
+  if (g_ctx.arena) (void)hipFree(g_ctx.arena);
+  g_ctx.arena = nullptr;
+  g_ctx.arena_cap = 0;
+  (void)hipStreamDestroy(g_ctx.stream);
+  g_ctx.stream = nullptr;
+  g_ctx.ready = false;
+  return HCORE_HIP_OK;
+}
+
+int HCORE_hip_set_error_policy(int policy) {
+  if (policy != HCORE_HIP_ON_ERROR_EXIT && policy != HCORE_HIP_ON_ERROR_RETURN)
+    return HCORE_HIP_ERR_ARG;
+  g_ctx.policy = policy;
+  return HCORE_HIP_OK;
+}
+
+int HCORE_hip_last_status(void) { return t_last_status; }
+
+// ---- plans -----------------------------------------------------------------------
+int HCORE_hip_plan_create(hcore_hip_plan** plan, int M, int precision) {
+  if (!plan || M <= 0) return HCORE_HIP_ERR_ARG;
+  if (precision != HCORE_HIP_FP64 && precision != HCORE_HIP_FP32) return HCORE_HIP_ERR_ARG;
+  int st = ctx_ensure();
+  if (st) return st;
+  hcore_hip_plan* p = new hcore_hip_plan();
+  p->M = M;
+  p->prec = precision;
+  *plan = p;
+  return HCORE_HIP_OK;
+}
+
+int HCORE_hip_plan_add_dgemm(hcore_hip_plan* p, int n, void* const* AU, void* const* AV,
+                             const int* Ark, int LDA, void* const* BU, void* const* BV,
+                             const int* Brk, int LDB, void* const* CU, void* const* CV,
+                             const int* Crk, int LDC, void* const* CUout, void* const* CVout,
+                             int LDCout, double alpha, double beta, int rk, int maxrk,
+                             double acc) {
+  if (!p || p->finalized || p->ng > 0 || n < 0) return HCORE_HIP_ERR_ARG;
+  for (int u = 0; u < n; ++u) {
+    const int K = Ark[u] + Crk[u];
+    if (Ark[u] <= 0 || Brk[u] <= 0 || Crk[u] <= 0) return HCORE_HIP_ERR_ZERO_RANK;
+    if (K > 2 * maxrk) return HCORE_HIP_ERR_SUM_RANKS;
+    if (K > maxrk || K > p->M) return HCORE_HIP_ERR_K_GT_MAXRK;
+  }
+  p->ng = n;
+  p->AU.assign(AU, AU + n); p->AV.assign(AV, AV + n);
+  p->BU.assign(BU, BU + n); p->BV.assign(BV, BV + n);
+  p->CU.assign(CU, CU + n); p->CV.assign(CV, CV + n);
+  p->CUo.assign(n, nullptr); p->CVo.assign(n, nullptr);
+  if (CUout) p->CUo.assign(CUout, CUout + n);
+  if (CVout) p->CVo.assign(CVout, CVout + n);
+  p->ra.assign(Ark, Ark + n); p->rb.assign(Brk, Brk + n); p->rc.assign(Crk, Crk + n);
+  p->lda = LDA; p->ldb = LDB; p->ldc = LDC; p->ldco = (CUout || CVout) ? LDCout : LDC;
+  p->alpha = alpha; p->beta = beta; p->rk = rk; p->maxrk = maxrk; p->acc = acc;
+  return HCORE_HIP_OK;
+}
+
+int HCORE_hip_plan_add_dsyrk(hcore_hip_plan* p, int n, void* const* AU, void* const* AV,
+                             const int* K, int LDAU, void* const* CD, int LDCD, double alpha) {
+  if (!p || p->finalized || p->ns > 0 || n < 0) return HCORE_HIP_ERR_ARG;
+  p->ns = n;
+  p->sAU.assign(AU, AU + n); p->sAV.assign(AV, AV + n); p->sCD.assign(CD, CD + n);
+  p->sK.assign(K, K + n);
+  p->ldau = LDAU; p->ldcd = LDCD; p->salpha = alpha;
+  return HCORE_HIP_OK;
+}
+
+int HCORE_hip_plan_finalize(hcore_hip_plan* p) {
+  if (!p || p->finalized) return HCORE_HIP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  return p->prec == HCORE_HIP_FP64 ? plan_finalize_t<double>(p) : plan_finalize_t<float>(p);
+}
+
+int HCORE_hip_plan_execute(hcore_hip_plan* p, void* stream) {
+  if (!p || !p->finalized) return HCORE_HIP_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  return p->prec == HCORE_HIP_FP64 ? plan_execute_t<double>(p, s) : plan_execute_t<float>(p, s);
+}
+
+int HCORE_hip_plan_execute_graph(hcore_hip_plan* p, void* stream) {
+  if (!p || !p->finalized) return HCORE_HIP_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  if (!p->gexec) {
+    hipStream_t cs = nullptr;
+    HCHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    HCHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    int st = p->prec == HCORE_HIP_FP64 ? plan_execute_t<double>(p, cs)
+                                       : plan_execute_t<float>(p, cs);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(cs, &g);
+    (void)hipStreamDestroy(cs);
+    if (st) return st;
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    p->graph = g;
+    HCHECK(hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0));
+  }
+  HCHECK(hipGraphLaunch(p->gexec, s));
+  return HCORE_HIP_OK;
+}
+
+const int* HCORE_hip_plan_ranks_device(const hcore_hip_plan* p) { return p ? p->d_rank : nullptr; }
+
+int HCORE_hip_plan_get_ranks(hcore_hip_plan* p, int* out) {
+  if (!p || !p->finalized) return HCORE_HIP_ERR_ARG;
+  HCHECK(hipDeviceSynchronize());
+  p->last_ranks.resize(p->ng);
+  if (p->ng) HCHECK(hipMemcpy(p->last_ranks.data(), p->d_rank, sizeof(int) * p->ng, hipMemcpyDeviceToHost));
+  if (out) std::copy(p->last_ranks.begin(), p->last_ranks.end(), out);
+  return HCORE_HIP_OK;
+}
+
+int HCORE_hip_plan_get_status(hcore_hip_plan* p, int* out) {
+  if (!p || !p->finalized || !out) return HCORE_HIP_ERR_ARG;
+  HCHECK(hipDeviceSynchronize());
+  if (p->ng) HCHECK(hipMemcpy(out, p->d_status, sizeof(int) * p->ng, hipMemcpyDeviceToHost));
+  return HCORE_HIP_OK;
+}
+
+int HCORE_hip_plan_get_sigma(hcore_hip_plan* p, int u, double* out, int* K_out) {
+  if (!p || !p->finalized || u < 0 || u >= p->ng) return HCORE_HIP_ERR_ARG;
+  const int K = p->ra[u] + p->rc[u];
+  if (K_out) *K_out = K;
+  if (!out) return HCORE_HIP_OK;
+  HCHECK(hipDeviceSynchronize());
+  if (p->prec == HCORE_HIP_FP64) {
+    HCHECK(hipMemcpy(out, reinterpret_cast<double*>(p->ws) + p->sigma_off[u], sizeof(double) * K,
+                     hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> f(K);
+    HCHECK(hipMemcpy(f.data(), reinterpret_cast<float*>(p->ws) + p->sigma_off[u], sizeof(float) * K,
+                     hipMemcpyDeviceToHost));
+    for (int i = 0; i < K; ++i) out[i] = f[i];
+  }
+  return HCORE_HIP_OK;
+}
+
+long long HCORE_hip_plan_workspace_bytes(const hcore_hip_plan* p) {
+  return p ? (long long)p->dbytes : 0;
+}
+
+int HCORE_hip_plan_flops(hcore_hip_plan* p, unsigned long* update, unsigned long* syrk) {
+  if (!p || !p->finalized) return HCORE_HIP_ERR_ARG;
+  if ((int)p->last_ranks.size() != p->ng) return HCORE_HIP_ERR_ARG;
+  unsigned long fu = 0, fs = 0;
+  const unsigned long M = p->M;
+  for (int u = 0; u < p->ng; ++u) {
+    const unsigned long ra = p->ra[u], rb = p->rb[u], K = p->ra[u] + p->rc[u];
+    const unsigned long r = p->last_ranks[u];
+    fu += flop_counts('q', M, K, 0, 0) * 2 + flop_counts('m', ra, rb, M, 0) +
+          flop_counts('m', M, ra, rb, 0) + flop_counts('s', K, 0, 0, 0) + r * K +
+          flop_counts('o', M, r, K, 1) + flop_counts('o', r, M, K, 2);
+  }
+  for (int s = 0; s < p->ns; ++s) {
+    const unsigned long K = p->sK[s], N = p->ldau;
+    fs += flop_counts('m', K, K, N, 0) + flop_counts('m', M, K, K, 0) + flop_counts('m', M, M, K, 0);
+  }
+  if (update) *update = fu;
+  if (syrk) *syrk = fs;
+  return HCORE_HIP_OK;
+}
+
+void HCORE_hip_plan_destroy(hcore_hip_plan* p) {
+  if (!p) return;
+  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->dmem) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(p->dmem);
+  }
+  delete p;
+}
+
+}  // extern "C"

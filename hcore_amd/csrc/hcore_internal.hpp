@@ -1,0 +1,119 @@
+// hcore_internal.hpp — device work descriptors and kernel launchers of the
+// MI355X-native HCORE.  Internal to the library (not installed).
+//
+// Every kernel is batched: it takes a device array of per-item descriptors
+// and processes one item per workgroup, so a single tile (the HCORE_* C API)
+// and a whole TLR Cholesky trailing step use the same code.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace hcore {
+
+// ---- status codes written per update by the device --------------------------
+enum : int {
+  ST_OK = 0,
+  ST_R_GT_MAXRK = 4,  // truncated rank above maxrk (reference hcore_dgemm.c:465-468)
+  ST_NONFINITE = 5,   // NaN/Inf reached the core SVD (the reference's LAPACK info!=0 class)
+};
+
+// ---- generic batched GEMM on 64x64 output tiles ------------------------------
+// C(m x n) = alpha * op(A) * op(B) + beta * C, column-major.
+enum : int {
+  GF_A_TRANS = 1,           // op(A) = A^T
+  GF_B_TRANS = 2,           // op(B) = B^T
+  GF_A_UPPER = 4,           // stored A(p,q) read as 0 when p > q
+  GF_B_UPPER = 8,           // stored B(p,q) read as 0 when p > q
+  GF_LOWER_ZERO_UPPER = 16, // write i >= j only; strict upper triangle of C set to 0
+  GF_BETA_ZERO = 32,        // C is not read
+};
+
+template <typename T>
+struct GemmProb {
+  const T* A;
+  const T* B;
+  T* C;
+  int m, n, k;
+  int lda, ldb, ldc;
+  T alpha, beta;
+  int flags;
+  const int* n_dev;   // if set, n is clamped to *n_dev (device-side rank)
+  const int* skip;    // if set and *skip != 0 the problem is skipped
+};
+
+struct GemmTask {
+  int prob;
+  int ti, tj;  // 64x64 tile coordinates
+};
+
+// ---- stacked factors: Us = [beta*CU | alpha*AU], Vs[:, :rc] = CV --------------
+template <typename T>
+struct StackProb {
+  const T* CU;
+  const T* CV;
+  const T* AU;
+  T* Us;
+  T* Vs;
+  int rc, ra, ldc, lda;
+};
+
+// ---- Householder QR of one nb x K stack (in place, ld = nb) ------------------
+template <typename T>
+struct QrProb {
+  T* A;    // nb x K: R on and above the diagonal, reflectors below
+  T* tau;  // K
+  T* Tp;   // ceil(K/PB) blocks of PB x PB compact-WY factors
+  int K;
+};
+
+// ---- core SVD: one-sided Jacobi on T = R_A R_B^T -----------------------------
+template <typename T>
+struct CoreProb {
+  const T* Tc;  // K x K, ld K (kept intact)
+  T* Tj;        // K x K scratch used when the matrix does not fit in LDS
+  T* Zu;        // K x K, ld K: first r columns = left singular vectors
+  T* sigma;     // K singular values, descending
+  int* rank;    // new rank r
+  int* status;  // ST_*
+  int K;
+};
+
+// ---- reassembly: X = Q [Z; 0] written into the caller's factor ---------------
+template <typename T>
+struct ApplyProb {
+  const T* A;    // factored stack (reflectors below the diagonal)
+  const T* Tp;   // compact-WY factors of the stack's panels
+  const T* Z;    // K x r, ld K
+  T* X;          // nb x r output, leading dimension ldx
+  const int* rank;
+  const int* status;
+  int K, ldx;
+};
+
+constexpr int kThreads = 256;
+constexpr int kPB = 8;  // Householder panel width
+
+// Largest K whose K x K FP64 core fits in LDS next to the Jacobi bookkeeping.
+int core_lds_max_k(size_t elem_size);
+
+// ---- launchers (hcore_kernels.hip) -------------------------------------------
+template <typename T>
+void launch_gemm(const GemmProb<T>* probs, const GemmTask* tasks, int ntasks, hipStream_t s);
+template <typename T>
+void launch_stack(const StackProb<T>* probs, int n, int nb, T alpha, T beta, hipStream_t s);
+template <typename T>
+void launch_geqrf(const QrProb<T>* probs, int n, int nb, hipStream_t s);
+template <typename T>
+void launch_core(const CoreProb<T>* probs, int n, bool in_lds, int kmax, int rk, int maxrk,
+                 T acc, hipStream_t s);
+template <typename T>
+void launch_apply(const ApplyProb<T>* probs, int n, int nb, hipStream_t s);
+
+template <typename T>
+void prepare_kernels();
+
+// Size in elements of the compact-WY storage for a K-column stack.
+inline int tp_elems(int K) { return ((K + kPB - 1) / kPB) * kPB * kPB; }
+
+}  // namespace hcore

@@ -1,0 +1,797 @@
+// hcore_kernels.hip — CDNA4 (gfx950) kernels of the TLR recompression path.
+//
+// One workgroup (256 threads = 4 wavefronts of 64) per work item.  The hot
+// path of HCORE_dgemm (reference src/compute/hcore_dgemm.c:703-1012) becomes
+//   k_stack   clone/scale of the factors into stacked panels   (hcore_dgemm.c:760-799, 106-114)
+//   k_gemm    skinny products F = AV^T BV, G = BU F^T, the core
+//             T = R_A R_B^T and the dense updates of dsyrk/dgemmcd
+//   k_geqrf   blocked Householder QR of the nb x K stacks      (hcore_dgemm.c:123, 216)
+//   k_core    one-sided Jacobi SVD of T + rank truncation       (hcore_dgemm.c:403-489)
+//   k_apply   Q_A [U_r;0] and Q_B [V_r S_r;0] by compact-WY     (hcore_dgemm.c:543, 655)
+//
+// Reductions over the nb rows of a tile use wavefront shuffles; panels of
+// Householder vectors are staged in LDS; the core matrix of the Jacobi SVD
+// lives in LDS when it fits (gfx950: 160 KiB per workgroup).
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <type_traits>
+
+#include "hcore_internal.hpp"
+
+namespace hcore {
+
+// ------------------------------------------------------------------------------
+// wavefront helpers
+// ------------------------------------------------------------------------------
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum of a group of G consecutive lanes (G a power of two, groups aligned).
+template <typename T>
+__device__ __forceinline__ T group_sum(T v, int g) {
+  for (int off = g >> 1; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Reduce N values (N a power of two <= 64) across the 64 lanes: each exchange
+// step hands half of the remaining values to the partner lane, so the cost is
+// about N + 6 shuffles instead of 6N.  On return v[0] holds the wave total of
+// value index reduce_scatter_index<N>(lane).  The halving is written as a
+// compile-time recursion so every array index is a constant (no scratch).
+template <int HALF, int OFF, int N, typename T>
+__device__ __forceinline__ void rs_level(T (&v)[N], int lane) {
+  if constexpr (HALF >= 1) {
+    const bool upper = (lane & OFF) != 0;
+#pragma unroll
+    for (int i = 0; i < HALF; ++i) {
+      const T send = upper ? v[i] : v[i + HALF];
+      const T keep = upper ? v[i + HALF] : v[i];
+      v[i] = keep + __shfl_xor(send, OFF, 64);
+    }
+    rs_level<HALF / 2, OFF / 2>(v, lane);
+  } else {
+#pragma unroll
+    for (int off = OFF; off >= 1; off >>= 1) v[0] += __shfl_xor(v[0], off, 64);
+  }
+}
+template <int N, typename T>
+__device__ __forceinline__ void reduce_scatter(T (&v)[N], int lane) {
+  static_assert((N & (N - 1)) == 0 && N <= 64, "N must be a power of two <= 64");
+  rs_level<N / 2, 32>(v, lane);
+}
+template <int N>
+__device__ __forceinline__ int reduce_scatter_index(int lane) {
+  constexpr int LOG = __builtin_ctz(N);
+  return (lane >> (6 - LOG)) & (N - 1);
+}
+
+template <typename T>
+__device__ __forceinline__ T eps_of();
+template <>
+__device__ __forceinline__ double eps_of<double>() { return DBL_EPSILON * 0.5; }
+template <>
+__device__ __forceinline__ float eps_of<float>() { return FLT_EPSILON * 0.5f; }
+
+// ------------------------------------------------------------------------------
+// batched GEMM on 64x64 tiles, 4x4 outputs per thread, k staged through LDS
+// ------------------------------------------------------------------------------
+constexpr int GT = 64;  // output tile edge
+constexpr int GK = 16;  // k chunk
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_gemm(const GemmProb<T>* __restrict__ probs,
+                                                   const GemmTask* __restrict__ tasks) {
+  __shared__ T As[GK][GT];
+  __shared__ T Bs[GK][GT];
+  const GemmTask tk = tasks[blockIdx.x];
+  const GemmProb<T> P = probs[tk.prob];
+  if (P.skip && *P.skip) return;
+  int n = P.n;
+  if (P.n_dev) n = min(n, *P.n_dev);
+  const int row0 = tk.ti * GT, col0 = tk.tj * GT;
+  if (row0 >= P.m || col0 >= n) return;
+  const int t = threadIdx.x;
+  const int tx = t & 15, ty = t >> 4;
+  const bool lzu = (P.flags & GF_LOWER_ZERO_UPPER) != 0;
+  if (lzu && row0 + GT - 1 < col0) {  // tile strictly above the diagonal: zero fill
+    for (int e = t; e < GT * GT; e += kThreads) {
+      const int i = row0 + (e & 63), j = col0 + (e >> 6);
+      if (i < P.m && j < n) P.C[i + (size_t)j * P.ldc] = T(0);
+    }
+    return;
+  }
+  const bool ta = P.flags & GF_A_TRANS, tb = P.flags & GF_B_TRANS;
+  const bool ua = P.flags & GF_A_UPPER, ub = P.flags & GF_B_UPPER;
+  T acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
+
+  for (int k0 = 0; k0 < P.k; k0 += GK) {
+    // A tile: As[kk][i] = op(A)(row0+i, k0+kk)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int i, kk;
+      if (!ta) { i = t & 63; kk = (t >> 6) + 4 * q; }
+      else     { kk = t & 15; i = (t >> 4) + 16 * q; }
+      const int gi = row0 + i, gk = k0 + kk;
+      T v = T(0);
+      if (gi < P.m && gk < P.k) {
+        const int p = ta ? gk : gi, qq = ta ? gi : gk;  // stored (row, col)
+        if (!(ua && p > qq)) v = P.A[p + (size_t)qq * P.lda];
+      }
+      As[kk][i] = v;
+    }
+    // B tile: Bs[kk][j] = op(B)(k0+kk, col0+j)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int j, kk;
+      if (tb) { j = t & 63; kk = (t >> 6) + 4 * q; }
+      else    { kk = t & 15; j = (t >> 4) + 16 * q; }
+      const int gj = col0 + j, gk = k0 + kk;
+      T v = T(0);
+      if (gj < n && gk < P.k) {
+        const int p = tb ? gj : gk, qq = tb ? gk : gj;
+        if (!(ub && p > qq)) v = P.B[p + (size_t)qq * P.ldb];
+      }
+      Bs[kk][j] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GK; ++kk) {
+      T av[4], bv[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) av[a] = As[kk][tx + 16 * a];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) bv[b] = Bs[kk][ty + 16 * b];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = fma(av[a], bv[b], acc[a][b]);
+    }
+    __syncthreads();
+  }
+  const bool bz = (P.flags & GF_BETA_ZERO) != 0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int i = row0 + tx + 16 * a, j = col0 + ty + 16 * b;
+      if (i < P.m && j < n) {
+        T* c = P.C + i + (size_t)j * P.ldc;
+        if (lzu && i < j) { *c = T(0); continue; }
+        T v = P.alpha * acc[a][b];
+        if (!bz) v = fma(P.beta, *c, v);
+        *c = v;
+      }
+    }
+}
+
+template <typename T>
+void launch_gemm(const GemmProb<T>* probs, const GemmTask* tasks, int ntasks, hipStream_t s) {
+  if (ntasks <= 0) return;
+  hipLaunchKernelGGL(k_gemm<T>, dim3(ntasks), dim3(kThreads), 0, s, probs, tasks);
+}
+
+// ------------------------------------------------------------------------------
+// stacked factors (clone + scale): Us = [beta*CU | alpha*AU], Vs(:, :rc) = CV
+// ------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_stack(const StackProb<T>* __restrict__ probs, int nb,
+                                                    T alpha, T beta) {
+  const StackProb<T> P = probs[blockIdx.x];
+  const int ncol = 2 * P.rc + P.ra;
+  for (int col = 0; col < ncol; ++col) {
+    for (int row = threadIdx.x; row < nb; row += kThreads) {
+      if (col < P.rc) {
+        P.Us[row + (size_t)col * nb] = beta * P.CU[row + (size_t)col * P.ldc];
+      } else if (col < P.rc + P.ra) {
+        const int c = col - P.rc;
+        P.Us[row + (size_t)col * nb] = alpha * P.AU[row + (size_t)c * P.lda];
+      } else {
+        const int c = col - P.rc - P.ra;
+        P.Vs[row + (size_t)c * nb] = P.CV[row + (size_t)c * P.ldc];
+      }
+    }
+  }
+}
+
+template <typename T>
+void launch_stack(const StackProb<T>* probs, int n, int nb, T alpha, T beta, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_stack<T>, dim3(n), dim3(kThreads), 0, s, probs, nb, alpha, beta);
+}
+
+// ------------------------------------------------------------------------------
+// Householder QR (LAPACK convention: H = I - tau v v^T, v(j) = 1, R(j,j) = beta)
+// ------------------------------------------------------------------------------
+// Panel factorization by one wavefront on the LDS panel Yp (PB columns of
+// NBP = 64*NBW rows).  Lane l owns rows l + 64 i.
+template <typename T, int NBW, int PB>
+__device__ void panel_factor(T* Yp, T* tau_s, T* beta_s, int j0, int pb, int lane) {
+  constexpr int NBP = 64 * NBW;
+  for (int c = 0; c < pb; ++c) {
+    const int j = j0 + c;
+    T* col = Yp + c * NBP;
+    T ss = T(0);
+#pragma unroll
+    for (int i = 0; i < NBW; ++i) {
+      const int row = lane + 64 * i;
+      const T x = col[row];
+      if (row > j) ss = fma(x, x, ss);
+    }
+    ss = wave_sum(ss);
+    const T alpha = col[j];
+    T beta = alpha, tau = T(0);
+    if (ss != T(0)) {
+      const T nrm = sqrt(fma(alpha, alpha, ss));
+      beta = alpha >= T(0) ? -nrm : nrm;
+      tau = (beta - alpha) / beta;
+      const T scal = T(1) / (alpha - beta);
+#pragma unroll
+      for (int i = 0; i < NBW; ++i) {
+        const int row = lane + 64 * i;
+        if (row > j) col[row] *= scal;
+      }
+      // apply H to the remaining panel columns
+      for (int k = c + 1; k < pb; ++k) {
+        T* ck = Yp + k * NBP;
+        T w = T(0);
+#pragma unroll
+        for (int i = 0; i < NBW; ++i) {
+          const int row = lane + 64 * i;
+          if (row > j) w = fma(col[row], ck[row], w);
+          else if (row == j) w += ck[row];
+        }
+        w = wave_sum(w) * tau;
+#pragma unroll
+        for (int i = 0; i < NBW; ++i) {
+          const int row = lane + 64 * i;
+          if (row > j) ck[row] = fma(-col[row], w, ck[row]);
+          else if (row == j) ck[row] -= w;
+        }
+      }
+    }
+    if (lane == 0) {
+      tau_s[c] = tau;
+      beta_s[c] = beta;
+    }
+    wave_sync();
+  }
+}
+
+template <typename T, int NBW, int PB, int CB>
+__global__ __launch_bounds__(kThreads) void k_geqrf(const QrProb<T>* __restrict__ probs, int nb) {
+  constexpr int NBP = 64 * NBW;
+  constexpr int NW = PB * CB;  // W block per chunk
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* Yp = reinterpret_cast<T*>(smem_raw);  // PB x NBP
+  T* Tt = Yp + PB * NBP;                   // PB x PB, column-major
+  T* Sg = Tt + PB * PB;                    // PB x PB
+  T* tau_s = Sg + PB * PB;                 // PB
+  T* beta_s = tau_s + PB;                  // PB
+  T* scr = beta_s + PB;                    // 4 waves x 2 x NW
+
+  const QrProb<T> P = probs[blockIdx.x];
+  const int K = P.K;
+  T* __restrict__ A = P.A;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  T* wscr0 = scr + wid * 2 * NW;
+  T* wscr1 = wscr0 + NW;
+  const T* Yl = Yp + lane;  // this lane's rows: Yl[c * NBP + 64 * i]
+
+  for (int j0 = 0; j0 < K; j0 += PB) {
+    const int pb = min(PB, K - j0);
+    #pragma unroll 1
+    for (int e = tid; e < PB * NBP; e += kThreads) {
+      const int c = e / NBP, row = e - c * NBP;
+      T v = T(0);
+      if (c < pb && row < nb) v = A[row + (size_t)(j0 + c) * nb];
+      Yp[e] = v;
+    }
+    if (tid < PB) {
+      tau_s[tid] = T(0);
+      beta_s[tid] = T(0);
+    }
+    __syncthreads();
+    if (wid == 0) panel_factor<T, NBW, PB>(Yp, tau_s, beta_s, j0, pb, lane);
+    __syncthreads();
+    // write the factored panel back: R above/on the diagonal, reflectors below
+    #pragma unroll 1
+    for (int e = tid; e < pb * NBP; e += kThreads) {
+      const int c = e / NBP, row = e - c * NBP;
+      if (row >= j0 && row < nb) {
+        const T v = (row == j0 + c) ? beta_s[c] : Yp[e];
+        A[row + (size_t)(j0 + c) * nb] = v;
+      }
+    }
+    if (tid < pb) P.tau[j0 + tid] = tau_s[tid];
+    __syncthreads();
+    // turn the LDS panel into the unit lower-trapezoidal reflector block
+    #pragma unroll 1
+    for (int e = tid; e < PB * NBP; e += kThreads) {
+      const int c = e / NBP, row = e - c * NBP;
+      if (c >= pb || row < j0 + c) Yp[e] = T(0);
+      else if (row == j0 + c) Yp[e] = T(1);
+    }
+    __syncthreads();
+    // Gram S(a,b) = v_a^T v_b, a < b, spread over the 4 wavefronts
+    {
+#pragma unroll 1
+      for (int p = wid; p < PB * (PB - 1) / 2; p += 4) {
+          int b = 1, a = p;
+          while (a >= b) { a -= b; ++b; }
+          T s = T(0);
+#pragma unroll
+          for (int i = 0; i < NBW; ++i) {
+            s = fma(Yl[a * NBP + 64 * i], Yl[b * NBP + 64 * i], s);
+          }
+          s = wave_sum(s);
+          if (lane == 0) Sg[a + b * PB] = s;
+        }
+    }
+    __syncthreads();
+    // compact-WY factor (forward, column-wise): T(:c,c) = -tau_c T(:c,:c) S(:c,c)
+    if (wid == 0) {
+#pragma unroll 1
+      for (int c = 0; c < PB; ++c) {
+        if (lane < PB) {
+          T val = T(0);
+          if (lane < c) {
+            T s = T(0);
+            for (int c2 = lane; c2 < c; ++c2) s = fma(Tt[lane + c2 * PB], Sg[c2 + c * PB], s);
+            val = -tau_s[c] * s;
+          } else if (lane == c) {
+            val = tau_s[c];
+          }
+          Tt[lane + c * PB] = val;
+        }
+        wave_sync();
+      }
+    }
+    __syncthreads();
+    if (tid < PB * PB) P.Tp[(size_t)(j0 / PB) * PB * PB + tid] = Tt[tid];
+
+    // trailing update A(:, j0+PB:) <- (I - Y T Y^T)^T A, CB columns per wavefront
+    const int ncols = K - (j0 + PB);
+    if (ncols > 0) {
+      const int nch = (ncols + CB - 1) / CB;
+      for (int q = wid; q < nch; q += 4) {
+        const int k0 = j0 + PB + q * CB;
+        const int cb = min(CB, K - k0);
+        T X[NBW][CB];
+#pragma unroll
+        for (int i = 0; i < NBW; ++i) {
+          const int row = lane + 64 * i;
+#pragma unroll
+          for (int kk = 0; kk < CB; ++kk)
+            X[i][kk] = (row >= j0 && row < nb && kk < cb) ? A[row + (size_t)(k0 + kk) * nb] : T(0);
+        }
+        T W[NW];
+#pragma unroll
+        for (int c = 0; c < PB; ++c) {
+          T y[NBW];
+#pragma unroll
+          for (int i = 0; i < NBW; ++i) y[i] = Yl[c * NBP + 64 * i];
+#pragma unroll
+          for (int kk = 0; kk < CB; ++kk) {
+            T acc = T(0);
+#pragma unroll
+            for (int i = 0; i < NBW; ++i) acc = fma(y[i], X[i][kk], acc);
+            W[c * CB + kk] = acc;
+          }
+          __builtin_amdgcn_sched_barrier(0);
+        }
+        reduce_scatter<NW>(W, lane);
+        wscr0[reduce_scatter_index<NW>(lane)] = W[0];
+        wave_sync();
+        if (lane < NW) {
+          const int c = lane / CB, kk = lane - c * CB;
+          T s = T(0);
+          for (int c2 = 0; c2 <= c; ++c2) s = fma(Tt[c2 + c * PB], wscr0[c2 * CB + kk], s);
+          wscr1[lane] = s;
+        }
+        wave_sync();
+#pragma unroll
+        for (int c = 0; c < PB; ++c) {
+          T y[NBW];
+#pragma unroll
+          for (int i = 0; i < NBW; ++i) y[i] = Yl[c * NBP + 64 * i];
+#pragma unroll
+          for (int kk = 0; kk < CB; ++kk) {
+            const T w2 = wscr1[c * CB + kk];
+#pragma unroll
+            for (int i = 0; i < NBW; ++i) X[i][kk] = fma(-y[i], w2, X[i][kk]);
+          }
+          __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int i = 0; i < NBW; ++i) {
+          const int row = lane + 64 * i;
+          if (row >= j0 && row < nb) {
+#pragma unroll
+            for (int kk = 0; kk < CB; ++kk)
+              if (kk < cb) A[row + (size_t)(k0 + kk) * nb] = X[i][kk];
+          }
+        }
+        wave_sync();
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int NBW>
+constexpr int qr_cb() { return NBW >= 32 ? 1 : (NBW >= 16 ? 2 : 4); }
+
+template <typename T, int NBW>
+static size_t qr_smem() {
+  constexpr int CB = qr_cb<NBW>();
+  return sizeof(T) * ((size_t)kPB * 64 * NBW + 2 * kPB * kPB + 2 * kPB + 4 * 2 * kPB * CB);
+}
+
+template <typename T, int NBW>
+static void launch_geqrf_nbw(const QrProb<T>* probs, int n, int nb, hipStream_t s) {
+  constexpr int CB = qr_cb<NBW>();
+  const size_t sm = qr_smem<T, NBW>();
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_geqrf<T, NBW, kPB, CB>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    attr = true;
+  }
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_geqrf<T, NBW, kPB, CB>), dim3(n), dim3(kThreads), sm, s, probs, nb);
+}
+
+template <typename T>
+void launch_geqrf(const QrProb<T>* probs, int n, int nb, hipStream_t s) {
+  if (nb <= 128) launch_geqrf_nbw<T, 2>(probs, n, nb, s);
+  else if (nb <= 256) launch_geqrf_nbw<T, 4>(probs, n, nb, s);
+  else if (nb <= 512) launch_geqrf_nbw<T, 8>(probs, n, nb, s);
+  else if (nb <= 1024) launch_geqrf_nbw<T, 16>(probs, n, nb, s);
+  else launch_geqrf_nbw<T, 32>(probs, n, nb, s);
+}
+
+// ------------------------------------------------------------------------------
+// core SVD: one-sided (Hestenes) Jacobi on the columns of T, round-robin
+// ordering, then sort, truncation (hcore_dgemm.c:444-464) and U_r = G_r / sigma
+// ------------------------------------------------------------------------------
+constexpr int kMaxSweeps = 60;
+
+template <typename T, bool IN_LDS>
+__global__ __launch_bounds__(kThreads) void k_core(const CoreProb<T>* __restrict__ probs, int kmax,
+                                                   int rk, int maxrk, T acc) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sig = reinterpret_cast<T*>(smem_raw);  // kmax
+  int* perm = reinterpret_cast<int*>(sig + kmax);
+  int* flags = perm + kmax;  // [0] rotated, [1] r, [2] nonfinite
+  T* Gl = reinterpret_cast<T*>(smem_raw + (((sizeof(T) + sizeof(int)) * kmax + 16 + 15) / 16) * 16);
+
+  const CoreProb<T> P = probs[blockIdx.x];
+  const int K = P.K;
+  const int tid = threadIdx.x;
+  T* G = IN_LDS ? Gl : P.Tj;
+  #pragma unroll 1
+  for (int e = tid; e < K * K; e += kThreads) G[e] = P.Tc[e];
+  __syncthreads();
+
+  const int n = K + (K & 1);
+  const int npairs = n / 2;
+  int g = kThreads / max(npairs, 1);
+  g = g >= 64 ? 64 : (g >= 32 ? 32 : (g >= 16 ? 16 : (g >= 8 ? 8 : (g >= 4 ? 4 : (g >= 2 ? 2 : 1)))));
+  const int ngroups = kThreads / g;
+  const int grp = tid / g, gl = tid - grp * g;
+  const T tol = sqrt(T(K)) * eps_of<T>();
+
+  if (K > 1) {
+    for (int sweep = 0; sweep < kMaxSweeps; ++sweep) {
+      if (tid == 0) flags[0] = 0;
+      __syncthreads();
+      for (int round = 0; round < n - 1; ++round) {
+        for (int pid = grp; pid < npairs; pid += ngroups) {
+          const int a = pid == 0 ? 0 : ((pid - 1 + round) % (n - 1)) + 1;
+          const int b = ((n - 2 - pid + round) % (n - 1)) + 1;
+          if (a >= K || b >= K) continue;  // virtual column of an odd K (uniform in the group)
+          T* ca = G + (size_t)a * K;
+          T* cb = G + (size_t)b * K;
+          T app = T(0), aqq = T(0), apq = T(0);
+          for (int row = gl; row < K; row += g) {
+            const T x = ca[row], y = cb[row];
+            app = fma(x, x, app);
+            aqq = fma(y, y, aqq);
+            apq = fma(x, y, apq);
+          }
+          app = group_sum(app, g);
+          aqq = group_sum(aqq, g);
+          apq = group_sum(apq, g);
+          if (apq != T(0) && fabs(apq) > tol * sqrt(app * aqq)) {
+            const T zeta = (aqq - app) / (T(2) * apq);
+            T t;
+            if (fabs(zeta) > T(1e30)) t = T(0.5) / zeta;
+            else t = (zeta >= T(0) ? T(1) : T(-1)) / (fabs(zeta) + sqrt(fma(zeta, zeta, T(1))));
+            const T cs = T(1) / sqrt(fma(t, t, T(1)));
+            const T sn = cs * t;
+            for (int row = gl; row < K; row += g) {
+              const T x = ca[row], y = cb[row];
+              ca[row] = cs * x - sn * y;
+              cb[row] = sn * x + cs * y;
+            }
+            if (gl == 0) flags[0] = 1;
+          }
+        }
+        __syncthreads();
+      }
+      const int rotated = flags[0];
+      __syncthreads();
+      if (!rotated) break;
+    }
+  }
+
+  // singular values = column norms
+  if (tid == 0) {
+    flags[1] = K;
+    flags[2] = 0;
+  }
+  for (int j = tid; j < K; j += kThreads) {
+    T s = T(0);
+    const T* cj = G + (size_t)j * K;
+    for (int row = 0; row < K; ++row) s = fma(cj[row], cj[row], s);
+    s = sqrt(s);
+    sig[j] = s;
+    if (!(s == s) || s > T(1e300)) flags[2] = 1;
+  }
+  __syncthreads();
+  if (flags[2]) {  // NaN/Inf: the reference's gesvd would fail (info != 0)
+    if (tid == 0) {
+      *P.status = ST_NONFINITE;
+      *P.rank = 0;
+    }
+    return;
+  }
+  // descending order, ties by index
+  for (int j = tid; j < K; j += kThreads) {
+    const T sj = sig[j];
+    int pos = 0;
+    for (int i = 0; i < K; ++i) {
+      const T si = sig[i];
+      pos += (si > sj) || (si == sj && i < j);
+    }
+    perm[pos] = j;
+  }
+  __syncthreads();
+  // truncation: fixed rank, else first i >= 1 with sigma_i < acc (r >= 1)
+  int r;
+  if (rk != 0) {
+    r = rk < K ? rk : K;
+  } else {
+    for (int i = tid + 1; i < K; i += kThreads)
+      if (sig[perm[i]] < acc) atomicMin(&flags[1], i);
+    __syncthreads();
+    r = flags[1];
+  }
+  for (int i = tid; i < K; i += kThreads) P.sigma[i] = sig[perm[i]];
+  if (tid == 0) {
+    *P.rank = r;
+    *P.status = r > maxrk ? ST_R_GT_MAXRK : ST_OK;
+  }
+  #pragma unroll 1
+  for (int e = tid; e < r * K; e += kThreads) {
+    const int i = e / K, row = e - i * K;
+    const int j = perm[i];
+    const T s = sig[j];
+    P.Zu[row + (size_t)i * K] = s > T(0) ? G[row + (size_t)j * K] / s : T(0);
+  }
+}
+
+int core_lds_max_k(size_t elem) {
+  // G (K*K) + sigma/perm (K each) + flags within 160 KiB
+  const size_t lds = 160 * 1024 - 256;
+  int k = 1;
+  while (true) {
+    const size_t kk = k + 1;
+    const size_t need = (((elem + sizeof(int)) * kk + 16 + 15) / 16) * 16 + elem * kk * kk;
+    if (need > lds) break;
+    ++k;
+  }
+  return k;
+}
+
+template <typename T>
+void launch_core(const CoreProb<T>* probs, int n, bool in_lds, int kmax, int rk, int maxrk, T acc,
+                 hipStream_t s) {
+  const size_t head = (((sizeof(T) + sizeof(int)) * (size_t)kmax + 16 + 15) / 16) * 16;
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_core<T, true>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr = true;
+  }
+  if (n <= 0) return;
+  if (in_lds) {
+    const size_t sm = head + sizeof(T) * (size_t)kmax * kmax;
+    hipLaunchKernelGGL((k_core<T, true>), dim3(n), dim3(kThreads), sm, s, probs, kmax, rk, maxrk,
+                       acc);
+  } else {
+    hipLaunchKernelGGL((k_core<T, false>), dim3(n), dim3(kThreads), head, s, probs, kmax, rk,
+                       maxrk, acc);
+  }
+}
+
+// ------------------------------------------------------------------------------
+// reassembly X = Q [Z; 0]: panels applied last to first, X <- (I - Y T Y^T) X
+// ------------------------------------------------------------------------------
+template <typename T, int NBW, int PB, int CB>
+__global__ __launch_bounds__(kThreads) void k_apply(const ApplyProb<T>* __restrict__ probs, int nb) {
+  constexpr int NBP = 64 * NBW;
+  constexpr int NW = PB * CB;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* Yp = reinterpret_cast<T*>(smem_raw);  // PB x NBP
+  T* Tt = Yp + PB * NBP;                   // PB x PB
+  T* scr = Tt + PB * PB;                   // 4 x 2 x NW
+
+  const ApplyProb<T> P = probs[blockIdx.x];
+  if (*P.status != ST_OK) return;
+  const int r = *P.rank;
+  const int K = P.K;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  T* __restrict__ X = P.X;
+  const size_t ldx = P.ldx;
+  T* wscr0 = scr + wid * 2 * NW;
+  T* wscr1 = wscr0 + NW;
+  const T* Yl = Yp + lane;  // this lane's rows: Yl[c * NBP + 64 * i]
+
+  #pragma unroll 1
+
+  for (int e = tid; e < r * nb; e += kThreads) {
+    const int col = e / nb, row = e - col * nb;
+    X[row + col * ldx] = row < K ? P.Z[row + (size_t)col * K] : T(0);
+  }
+  __syncthreads();
+
+  const int npan = (K + PB - 1) / PB;
+  const int nch = (r + CB - 1) / CB;
+  for (int p = npan - 1; p >= 0; --p) {
+    const int j0 = p * PB;
+    const int pb = min(PB, K - j0);
+    #pragma unroll 1
+    for (int e = tid; e < PB * NBP; e += kThreads) {
+      const int c = e / NBP, row = e - c * NBP;
+      T v = T(0);
+      if (c < pb && row < nb) {
+        if (row == j0 + c) v = T(1);
+        else if (row > j0 + c) v = P.A[row + (size_t)(j0 + c) * nb];
+      }
+      Yp[e] = v;
+    }
+    if (tid < PB * PB) Tt[tid] = P.Tp[(size_t)p * PB * PB + tid];
+    __syncthreads();
+    for (int q = wid; q < nch; q += 4) {
+      const int k0 = q * CB;
+      const int cb = min(CB, r - k0);
+      T Xr[NBW][CB];
+#pragma unroll
+      for (int i = 0; i < NBW; ++i) {
+        const int row = lane + 64 * i;
+#pragma unroll
+        for (int kk = 0; kk < CB; ++kk)
+          Xr[i][kk] = (row >= j0 && row < nb && kk < cb) ? X[row + (k0 + kk) * ldx] : T(0);
+      }
+      T W[NW];
+#pragma unroll
+      for (int c = 0; c < PB; ++c) {
+        T y[NBW];
+#pragma unroll
+        for (int i = 0; i < NBW; ++i) y[i] = Yl[c * NBP + 64 * i];
+#pragma unroll
+        for (int kk = 0; kk < CB; ++kk) {
+          T acc = T(0);
+#pragma unroll
+          for (int i = 0; i < NBW; ++i) acc = fma(y[i], Xr[i][kk], acc);
+          W[c * CB + kk] = acc;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      reduce_scatter<NW>(W, lane);
+      wscr0[reduce_scatter_index<NW>(lane)] = W[0];
+      wave_sync();
+      if (lane < NW) {  // W2 = T W (T upper triangular)
+        const int c = lane / CB, kk = lane - c * CB;
+        T s = T(0);
+        for (int c2 = c; c2 < PB; ++c2) s = fma(Tt[c + c2 * PB], wscr0[c2 * CB + kk], s);
+        wscr1[lane] = s;
+      }
+      wave_sync();
+#pragma unroll
+      for (int c = 0; c < PB; ++c) {
+        T y[NBW];
+#pragma unroll
+        for (int i = 0; i < NBW; ++i) y[i] = Yl[c * NBP + 64 * i];
+#pragma unroll
+        for (int kk = 0; kk < CB; ++kk) {
+          const T w2 = wscr1[c * CB + kk];
+#pragma unroll
+          for (int i = 0; i < NBW; ++i) Xr[i][kk] = fma(-y[i], w2, Xr[i][kk]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+#pragma unroll
+      for (int i = 0; i < NBW; ++i) {
+        const int row = lane + 64 * i;
+        if (row >= j0 && row < nb) {
+#pragma unroll
+          for (int kk = 0; kk < CB; ++kk)
+            if (kk < cb) X[row + (k0 + kk) * ldx] = Xr[i][kk];
+        }
+      }
+      wave_sync();
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, int NBW>
+static void launch_apply_nbw(const ApplyProb<T>* probs, int n, int nb, hipStream_t s) {
+  constexpr int CB = qr_cb<NBW>();
+  const size_t sm = sizeof(T) * ((size_t)kPB * 64 * NBW + kPB * kPB + 4 * 2 * kPB * CB);
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply<T, NBW, kPB, CB>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    attr = true;
+  }
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_apply<T, NBW, kPB, CB>), dim3(n), dim3(kThreads), sm, s, probs, nb);
+}
+
+template <typename T>
+void launch_apply(const ApplyProb<T>* probs, int n, int nb, hipStream_t s) {
+  if (nb <= 128) launch_apply_nbw<T, 2>(probs, n, nb, s);
+  else if (nb <= 256) launch_apply_nbw<T, 4>(probs, n, nb, s);
+  else if (nb <= 512) launch_apply_nbw<T, 8>(probs, n, nb, s);
+  else if (nb <= 1024) launch_apply_nbw<T, 16>(probs, n, nb, s);
+  else launch_apply_nbw<T, 32>(probs, n, nb, s);
+}
+
+// Set the large-LDS attributes of every kernel variant up front, so that no
+// attribute call happens while a stream is being captured into a graph.
+template <typename T>
+void prepare_kernels() {
+  launch_geqrf<T>(nullptr, 0, 128, nullptr);
+  launch_geqrf<T>(nullptr, 0, 256, nullptr);
+  launch_geqrf<T>(nullptr, 0, 512, nullptr);
+  launch_geqrf<T>(nullptr, 0, 1024, nullptr);
+  launch_geqrf<T>(nullptr, 0, 2048, nullptr);
+  launch_apply<T>(nullptr, 0, 128, nullptr);
+  launch_apply<T>(nullptr, 0, 256, nullptr);
+  launch_apply<T>(nullptr, 0, 512, nullptr);
+  launch_apply<T>(nullptr, 0, 1024, nullptr);
+  launch_apply<T>(nullptr, 0, 2048, nullptr);
+  launch_core<T>(nullptr, 0, true, 1, 0, 1, T(0), nullptr);
+}
+
+// ---- explicit instantiations ---------------------------------------------------
+#define HCORE_INSTANTIATE(T)                                                                  \
+  template void launch_gemm<T>(const GemmProb<T>*, const GemmTask*, int, hipStream_t);        \
+  template void launch_stack<T>(const StackProb<T>*, int, int, T, T, hipStream_t);            \
+  template void launch_geqrf<T>(const QrProb<T>*, int, int, hipStream_t);                     \
+  template void launch_core<T>(const CoreProb<T>*, int, bool, int, int, int, T, hipStream_t); \
+  template void launch_apply<T>(const ApplyProb<T>*, int, int, hipStream_t);                 \
+  template void prepare_kernels<T>();
+HCORE_INSTANTIATE(double)
+HCORE_INSTANTIATE(float)
+
+}  // namespace hcore

@@ -1,0 +1,103 @@
+/*
+ * hcore_d.h — FP64 tile low-rank kernels of the MI355X-native HCORE.
+ *
+ * Drop-in for the reference's src/include/hcore_d.h: the three FP64 entry
+ * points of the recompression hot path keep their exact prototypes so that a
+ * runtime built against the reference (e.g. HiCMA codelets) relinks without
+ * source changes.  The arithmetic runs on the GPU (HIP, gfx950); pointers may
+ * be host or device memory (host buffers are staged through the device
+ * workspace pool).
+ *
+ * Tile storage: an nb x nb tile T is kept as column-major factors TU (nb x r)
+ * and TV (nb x r) with T = TU * TV^T.  Ranks are passed as doubles (the value
+ * is cast to int), as in the reference.
+ */
+#ifndef HCORE_AMD_HCORE_D_H
+#define HCORE_AMD_HCORE_D_H
+#include "flop_util_structs.h"
+#include "hcore.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/*
+ * C <- beta*C + alpha*A*B^T for low-rank A = AU*AV^T, B = BU*BV^T and
+ * C = CU*CV^T, recompressed to the smallest rank whose dropped singular
+ * values are below acc (absolute), or to rank rk when rk != 0.
+ *
+ * Replaces reference src/compute/hcore_dgemm.c:703-1012 (prototype
+ * src/include/hcore_d.h:25-46).  transA, transB and N are ignored (the op is
+ * always beta*C + alpha*A*B^T).  LDA must equal M (the AU block is read as a
+ * contiguous M x rank array, reference hcore_dgemm.c:106).  On return CU and
+ * CV (leading dimension LDC) hold the first r columns of the new factors, CU
+ * with orthonormal columns and the singular values folded into CV, and
+ * Crk[0] = r.  The caller guarantees room for r columns.  Fatal conditions
+ * (zero input rank, rc+ra > 2*maxrk, rc+ra > maxrk, r > maxrk) print a
+ * message to stderr and call exit(-1), leaving CU/CV untouched.
+ * `work` is accepted for ABI compatibility; the device workspace comes from
+ * the library's pool.  flops->update is incremented by the reference's own
+ * closed-form count.
+ */
+void HCORE_dgemm(HCORE_enum transA, int transB,
+                 int M, int N,
+                 double alpha,
+                 double *AU, double *AV, double *Ark, int LDA,
+                 double *BU, double *BV, double *Brk, int LDB,
+                 double beta,
+                 double *CU, double *CV, double *Crk, int LDC,
+                 int rk, int maxrk, double acc,
+                 double *work, flop_counter *flops);
+
+/*
+ * Dense-output update CD <- CD - (Au*Av^T) * (Bu*Bv^T)^T.
+ * Replaces reference src/compute/hcore_dgemmcd.c:36-106 (prototype
+ * src/include/hcore_d.h:68-86).  As in the reference, alpha = -1 and
+ * beta = 1 are fixed, transA/transB are ignored, every leading dimension is
+ * taken as M, and flops is not updated.
+ */
+void HCORE_dgemmcd(HCORE_enum transA, HCORE_enum transB,
+                   int M,
+                   double alpha,
+                   double *Au, double *Av, int Arank, int LDA,
+                   double *Bu, double *Bv, int Brank, int LDB,
+                   double beta,
+                   double *CD, int LDC,
+                   int maxrank,
+                   double *work, flop_counter *flops);
+
+/*
+ * Diagonal update CD <- alpha * AU*(AV^T*AV)*AU^T + CD with the strict upper
+ * triangle of CD then set to zero.
+ * Replaces reference src/compute/hcore_dsyrk.c:47-189 (prototype
+ * src/include/hcore_d.h:107-116).  As in the reference, the row count of AV
+ * is taken as LDAU, uplo/trans/beta/LDAV are ignored and flops->syrk is
+ * incremented.
+ */
+void HCORE_dsyrk(HCORE_enum uplo, HCORE_enum trans,
+                 int M, int K,
+                 double alpha,
+                 const double *AU, int LDAU,
+                 const double *AV, int LDAV,
+                 double beta,
+                 double *CD, int LDCD,
+                 double *work, flop_counter *flops);
+
+/* Data symbols the reference exports or imports (reference
+ * src/compute/hcore_dgemm.c:37-43, hcore_dsyrk.c:21-22,
+ * hcore_dgemmcd.c:24-26).  `use_scratch` is defined weakly (0) by the
+ * library so applications that define it still link.  None of them changes
+ * results on the GPU path. */
+extern int use_scratch;
+extern int use_trmm;
+extern int gemm_print_index;
+extern int gemm_print_mat;
+extern int hc_nelm_limit;
+extern int syrk_print_index;
+extern int gemmcd_print_index;
+extern int gemmcd_print_mat;
+extern int gemmcd_use_scratch;
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HCORE_AMD_HCORE_D_H */

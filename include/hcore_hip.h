@@ -1,0 +1,103 @@
+/*
+ * hcore_hip.h — GPU-side extensions of the MI355X-native HCORE (no
+ * counterpart in the reference, whose routines are single-tile host calls).
+ *
+ * 1. Library context: device selection, error policy, last status.
+ * 2. Batched plans: the hundreds to thousands of independent tile updates of
+ *    one TLR Cholesky trailing step (HCORE_dgemm on every C_ij, HCORE_dsyrk on
+ *    every diagonal tile) recorded once, with device-resident factors, and
+ *    executed as a fixed sequence of batched kernels on one HIP stream — so
+ *    the whole step can be captured into a hipGraph and replayed.
+ *
+ * All pointers inside a plan are DEVICE pointers; the pointer arrays and the
+ * rank arrays passed to the plan builders are HOST arrays (the runtime keeps
+ * tile ranks on the host, as HiCMA does).  Streams are hipStream_t passed as
+ * void*.  Every function returns 0 on success or a HCORE_HIP_* status.
+ */
+#ifndef HCORE_AMD_HCORE_HIP_H
+#define HCORE_AMD_HCORE_HIP_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (the reference's exit(-1) sites, reference hcore_dgemm.c) */
+#define HCORE_HIP_OK 0
+#define HCORE_HIP_ERR_ZERO_RANK 1    /* an input rank is 0 (hcore_dgemm.c:743-746) */
+#define HCORE_HIP_ERR_SUM_RANKS 2    /* rc+ra > 2*maxrk (hcore_dgemm.c:91-95) */
+#define HCORE_HIP_ERR_K_GT_MAXRK 3   /* rc+ra > maxrk or > M (hcore_dgemm.c:924-927) */
+#define HCORE_HIP_ERR_R_GT_MAXRK 4   /* truncated rank > maxrk (hcore_dgemm.c:465-468) */
+#define HCORE_HIP_ERR_NUMERIC 5      /* NaN/Inf in the core SVD (LAPACK info != 0) */
+#define HCORE_HIP_ERR_RUNTIME 6      /* HIP runtime failure */
+#define HCORE_HIP_ERR_ARG 7          /* invalid argument */
+
+/* precision forms of the recompression */
+#define HCORE_HIP_FP64 0   /* everything in FP64 (the plain HCORE_dgemm) */
+#define HCORE_HIP_FP32 1   /* everything in FP32 (HCORE_sgemm) */
+
+/* error policy of the drop-in HCORE_* entry points */
+#define HCORE_HIP_ON_ERROR_EXIT 0    /* reference behaviour: message + exit(-1) */
+#define HCORE_HIP_ON_ERROR_RETURN 1  /* message, record status, return */
+
+int HCORE_hip_init(int device);
+int HCORE_hip_finalize(void);
+int HCORE_hip_set_error_policy(int policy);
+/* status of the last HCORE_* call made by this thread */
+int HCORE_hip_last_status(void);
+
+typedef struct hcore_hip_plan hcore_hip_plan;
+
+/* Create an empty plan for tiles of M rows in the given precision form. */
+int HCORE_hip_plan_create(hcore_hip_plan **plan, int M, int precision);
+
+/*
+ * Record n independent HCORE_dgemm updates C_u <- beta*C_u + alpha*A_u*B_u^T.
+ * AU..CV: host arrays of n device pointers (FP64, or FP32 for HCORE_HIP_FP32)
+ * to column-major factors with leading dimensions LDA, LDB, LDC.  CUout/CVout
+ * (may be NULL: in place) receive the new factors with leading dimension
+ * LDCout.  Ark/Brk/Crk: host int ranks.  rk, maxrk, acc as in HCORE_dgemm.
+ * May be called once per plan.
+ */
+int HCORE_hip_plan_add_dgemm(hcore_hip_plan *plan, int n,
+                             void *const *AU, void *const *AV, const int *Ark, int LDA,
+                             void *const *BU, void *const *BV, const int *Brk, int LDB,
+                             void *const *CU, void *const *CV, const int *Crk, int LDC,
+                             void *const *CUout, void *const *CVout, int LDCout,
+                             double alpha, double beta, int rk, int maxrk, double acc);
+
+/*
+ * Record n HCORE_dsyrk diagonal updates CD_u <- alpha*AU_u (AV_u^T AV_u) AU_u^T
+ * + CD_u, strict upper triangle of CD_u set to 0.  Host arrays of device
+ * pointers; K: host int ranks.  May be called once per plan.
+ */
+int HCORE_hip_plan_add_dsyrk(hcore_hip_plan *plan, int n,
+                             void *const *AU, void *const *AV, const int *K, int LDAU,
+                             void *const *CD, int LDCD, double alpha);
+
+/* Upload descriptors and carve the device workspace.  Called once. */
+int HCORE_hip_plan_finalize(hcore_hip_plan *plan);
+
+/* Enqueue the whole batch on `stream` (hipStream_t; NULL = default stream). */
+int HCORE_hip_plan_execute(hcore_hip_plan *plan, void *stream);
+
+/* Same, through a hipGraph captured on the first call and replayed after. */
+int HCORE_hip_plan_execute_graph(hcore_hip_plan *plan, void *stream);
+
+/* Device array of the n new ranks (int), valid after execution. */
+const int *HCORE_hip_plan_ranks_device(const hcore_hip_plan *plan);
+/* Copy new ranks / per-update status (HCORE_HIP_*) to host; synchronizes. */
+int HCORE_hip_plan_get_ranks(hcore_hip_plan *plan, int *ranks_host);
+int HCORE_hip_plan_get_status(hcore_hip_plan *plan, int *status_host);
+/* Copy the K singular values of update u (descending, FP64) to host. */
+int HCORE_hip_plan_get_sigma(hcore_hip_plan *plan, int u, double *sigma_host, int *K_out);
+/* Bytes of device workspace held by the plan. */
+long long HCORE_hip_plan_workspace_bytes(const hcore_hip_plan *plan);
+/* Flop count of the recorded work by the reference's closed forms, using
+ * the ranks of the last execution (requires HCORE_hip_plan_get_ranks). */
+int HCORE_hip_plan_flops(hcore_hip_plan *plan, unsigned long *update, unsigned long *syrk);
+
+void HCORE_hip_plan_destroy(hcore_hip_plan *plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HCORE_AMD_HCORE_HIP_H */

@@ -1,0 +1,231 @@
+"""Pins of the CPU oracle (oracle/hcore_oracle.c) — no GPU.
+
+1. The reference's own tests, reproduced on the reference's own inputs:
+   testing_dgemm.c:250-392 (dlatms, iseed {0,0,0,1}; ||.||_inf < 10 acc),
+   testing_sgemm.c (FP32 twin), testing_dsyrk.c:104-181 and
+   testing_dgemmcd.c:105-205 (glibc rand(), seed 1; ||.||_inf < 1000 eps).
+2. What the mathematics fixes for the recompression: the core singular values
+   are those of the dense update beta*C + alpha*A*B^T (Q_A, Q_B orthonormal),
+   the rank follows the truncation rule on them, the truncated product is the
+   Eckart-Young optimum, CU is orthonormal and CV carries sigma.
+3. Planted mistakes in the oracle that these pins must catch.
+"""
+import numpy as np
+import pytest
+
+from conftest import glibc_rand_stream, rel_fro, trunc_rank
+
+
+def _dense(U, V):
+    return U @ V.T
+
+
+# ---------------------------------------------------------------- reference tests
+def test_reference_testing_dgemm(orc):
+    """testing_dgemm.c:250-392 with the oracle in place of HCORE_dgemm."""
+    M, smr, acc = 10, 6, 1e-4
+    iseed = np.array([0, 0, 0, 1], dtype=np.int32)
+    A, AU, AV, ra = orc.generate_compress(M, M, M, smr, M, acc, iseed)
+    B, BU, BV, rb = orc.generate_compress(M, M, M, smr, M, acc, iseed)
+    Cd, CU, CV, rc = orc.generate_compress(M, M, M, smr, M, acc, iseed)
+    ref = Cd + A @ B.T
+    st, CUn, CVn, r, sig, fl = orc.dgemm(1.0, AU[:, :ra], AV[:M, :ra], BU[:, :rb], BV[:M, :rb],
+                                         1.0, CU[:, :rc], CV[:M, :rc], rk=0, maxrk=M, acc=acc)
+    assert st == orc.OK
+    rnorm = np.abs(ref - CUn @ CVn.T).sum(axis=1).max()
+    assert rnorm < 10 * acc
+    assert 1 <= r <= ra + rc
+
+
+def test_reference_testing_sgemm(orc):
+    """testing_sgemm.c: FP32 generator (slatms) and storage_maxrank 10."""
+    M, smr, acc = 10, 10, np.float32(1e-4)
+    iseed = np.array([0, 0, 0, 1], dtype=np.int32)
+    f32 = np.float32
+    A, AU, AV, ra = orc.generate_compress(M, M, M, smr, M, acc, iseed, dtype=f32)
+    B, BU, BV, rb = orc.generate_compress(M, M, M, smr, M, acc, iseed, dtype=f32)
+    Cd, CU, CV, rc = orc.generate_compress(M, M, M, smr, M, acc, iseed, dtype=f32)
+    ref = (Cd + A @ B.T).astype(np.float64)
+    st, CUn, CVn, r, sig, fl = orc.dgemm(1.0, AU[:, :ra], AV[:M, :ra], BU[:, :rb], BV[:M, :rb],
+                                         1.0, CU[:, :rc], CV[:M, :rc], rk=0, maxrk=M, acc=acc,
+                                         dtype=f32)
+    assert st == orc.OK
+    rnorm = np.abs(ref - CUn.astype(np.float64) @ CVn.astype(np.float64).T).sum(axis=1).max()
+    assert rnorm < 10 * acc
+
+
+def reference_dsyrk_inputs(nrows=10, rank=7):
+    rnd = glibc_rand_stream()
+    C = np.zeros((nrows, nrows), order="F")
+    for i in range(nrows):
+        for j in range(i + 1):
+            C[i, j] = rnd()
+    AU = np.zeros((nrows, rank), order="F")
+    AV = np.zeros((nrows, rank), order="F")
+    for i in range(nrows):
+        for j in range(rank):
+            AU[i, j] = rnd()
+            AV[i, j] = rnd()
+    return C, AU, AV
+
+
+def reference_dgemmcd_inputs(nrows=10, ra=7, rb=6):
+    rnd = glibc_rand_stream()
+    C = np.zeros((nrows, nrows), order="F")
+    for i in range(nrows):
+        for j in range(i + 1):
+            C[i, j] = rnd()
+    AU = np.zeros((nrows, ra), order="F")
+    AV = np.zeros((nrows, ra), order="F")
+    BU = np.zeros((nrows, rb), order="F")
+    BV = np.zeros((nrows, rb), order="F")
+    for i in range(nrows):
+        for j in range(ra):
+            AU[i, j] = rnd()
+            AV[i, j] = rnd()
+        for j in range(rb):
+            BU[i, j] = rnd()
+            BV[i, j] = rnd()
+    return C, AU, AV, BU, BV
+
+
+EPS = np.finfo(np.float64).eps
+
+
+def test_reference_testing_dsyrk(orc):
+    """testing_dsyrk.c:104-181: lower triangle vs C + A A^T, < 1000 eps."""
+    C, AU, AV = reference_dsyrk_inputs()
+    A = AU @ AV.T
+    ref = C + np.tril(A @ A.T)
+    out, fl = orc.dsyrk(1.0, AU, AV, C)
+    assert np.abs(np.tril(ref - out)).sum(axis=1).max() < 1000 * EPS
+    assert np.all(np.triu(out, 1) == 0.0)  # hcore_dsyrk.c:174-181
+    assert fl == 2 * 7 * 7 * 10 + 2 * 10 * 7 * 7 + 2 * 10 * 10 * 7
+
+
+def test_reference_testing_dgemmcd(orc):
+    """testing_dgemmcd.c:105-205: C - A B^T, < 1000 eps (both associations)."""
+    for ra, rb in ((7, 6), (6, 7)):
+        C, AU, AV, BU, BV = reference_dgemmcd_inputs(10, ra, rb)
+        ref = C - (AU @ AV.T) @ (BU @ BV.T).T
+        out = orc.dgemmcd(AU, AV, BU, BV, C)
+        assert np.abs(ref - out).sum(axis=1).max() < 1000 * EPS
+
+
+# ------------------------------------------------------------------ math pins
+def _random_case(seed, M, ra, rb, rc, decay=8.0):
+    from hcore_amd.tiles import lowrank_tile
+    AU, AV = lowrank_tile(M, ra, decay, ("orc", seed, "A"))
+    BU, BV = lowrank_tile(M, rb, decay, ("orc", seed, "B"))
+    CU, CV = lowrank_tile(M, rc, decay, ("orc", seed, "C"))
+    return AU, AV, BU, BV, CU, CV
+
+
+def check_recompression(AU, AV, BU, BV, CU, CV, alpha, beta, CUn, CVn, r, sig, acc, rk=0,
+                        sigma_tol=1e-12):
+    """The mathematical pins of one recompression result."""
+    E = beta * _dense(CU, CV) + alpha * _dense(AU, AV) @ _dense(BU, BV).T
+    s_np = np.linalg.svd(E, compute_uv=False)
+    K = AU.shape[1] + CU.shape[1]
+    s0 = s_np[0]
+    # core singular values = singular values of the dense update (first K)
+    assert np.max(np.abs(np.sort(sig)[::-1] - s_np[:K])) <= sigma_tol * s0
+    # rank = truncation rule on the (independent) numpy singular values
+    assert r == trunc_rank(s_np[:K], acc, rk)
+    # Eckart-Young: the error equals the dropped tail
+    tail = np.sqrt(np.sum(s_np[r:] ** 2))
+    err = np.linalg.norm(CUn @ CVn.T - E)
+    assert err <= tail * (1 + 1e-6) + 1e-12 * s0
+    # output convention: CU orthonormal, CV columns carry sigma
+    assert np.max(np.abs(CUn.T @ CUn - np.eye(r))) < 1e-10
+    assert np.max(np.abs(np.linalg.norm(CVn, axis=0) - s_np[:r])) < 1e-10 * s0
+
+
+CASES = [  # (seed, M, ra, rb, rc, alpha, beta, rk, acc)
+    (1, 64, 8, 8, 8, -1.0, 1.0, 0, 1e-8),
+    (2, 100, 12, 20, 7, 1.0, 1.0, 0, 1e-6),
+    (3, 128, 30, 25, 40, -1.0, 1.0, 0, 1e-8),
+    (4, 200, 16, 16, 16, 0.5, 2.0, 0, 1e-8),
+    (5, 96, 10, 5, 12, -1.0, 1.0, 9, 1e-8),
+    (6, 500, 32, 32, 32, -1.0, 1.0, 0, 1e-8),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[str(c[0]) for c in CASES])
+def test_oracle_math_pins(orc, case):
+    seed, M, ra, rb, rc, alpha, beta, rk, acc = case
+    AU, AV, BU, BV, CU, CV = _random_case(seed, M, ra, rb, rc)
+    st, CUn, CVn, r, sig, fl = orc.dgemm(alpha, AU, AV, BU, BV, beta, CU, CV, rk=rk, maxrk=M,
+                                         acc=acc)
+    assert st == orc.OK
+    check_recompression(AU, AV, BU, BV, CU, CV, alpha, beta, CUn, CVn, r, sig, acc, rk)
+
+
+def test_oracle_brute_force_tiny(orc):
+    """nb = 6: the update is compared with a dense truncated SVD directly."""
+    rng = np.random.default_rng(7)
+    M = 6
+    AU, AV, BU, BV = (rng.standard_normal((M, 2)) for _ in range(4))
+    CU, CV = rng.standard_normal((M, 3)), rng.standard_normal((M, 3))
+    E = CU @ CV.T - (AU @ AV.T) @ (BU @ BV.T).T
+    U, s, Vt = np.linalg.svd(E)
+    st, CUn, CVn, r, sig, fl = orc.dgemm(-1.0, AU, AV, BU, BV, 1.0, CU, CV, rk=0, maxrk=M,
+                                         acc=1e-9)
+    assert st == orc.OK and r == 5
+    assert np.allclose(CUn @ CVn.T, (U[:, :r] * s[:r]) @ Vt[:r], atol=1e-12)
+
+
+def test_oracle_error_paths(orc):
+    AU, AV, BU, BV, CU, CV = _random_case(9, 32, 4, 4, 4)
+    st = orc.dgemm(-1.0, AU, AV, BU, BV, 1.0, CU[:, :0], CV[:, :0], maxrk=32)[0]
+    assert st == orc.ERR_ZERO_RANK
+    st = orc.dgemm(-1.0, AU, AV, BU, BV, 1.0, CU, CV, maxrk=3)[0]
+    assert st == orc.ERR_SUM_RANKS
+    st = orc.dgemm(-1.0, AU, AV, BU, BV, 1.0, CU, CV, maxrk=6)[0]
+    assert st == orc.ERR_K_GT_MAXRK
+
+
+def test_oracle_flop_count_worked_value(orc):
+    """The 500x500 rank-32 update is quoted at about 23.5 MFLOP at r = 32."""
+    M, ra, rb, rc, r = 500, 32, 32, 32, 32
+    K = ra + rc
+    f = (2 * orc.flop_counts('q', M, K) + orc.flop_counts('m', ra, rb, M) +
+         orc.flop_counts('m', M, ra, rb) + orc.flop_counts('s', K) + r * K +
+         orc.flop_counts('o', M, r, K, 0 + 1) + orc.flop_counts('o', r, M, K, 2))
+    assert abs(f / 1e6 - 23.5) < 0.1
+    assert orc.flop_counts('m', 3, 4, 5) == 120
+    assert orc.flop_counts('s', 10) == 22000
+
+
+# --------------------------------------------------------------- planted mistakes
+def _pins_fail(orc, mutate, case=(3, 128, 24, 24, 40, -1.0, 1.0, 0, 1e-8)):
+    seed, M, ra, rb, rc, alpha, beta, rk, acc = case
+    AU, AV, BU, BV, CU, CV = _random_case(seed, M, ra, rb, rc)
+    st, CUn, CVn, r, sig, fl = orc.dgemm(alpha, AU, AV, BU, BV, beta, CU, CV, rk=rk, maxrk=M,
+                                         acc=acc, mutate=mutate)
+    try:
+        check_recompression(AU, AV, BU, BV, CU, CV, alpha, beta, CUn, CVn, r, sig, acc, rk)
+    except AssertionError:
+        return True
+    return False
+
+
+@pytest.mark.parametrize("mut", ["MUTATE_ALPHA_SIGN", "MUTATE_NO_G_TRANSPOSE",
+                                 "MUTATE_TRUNC_OFF_BY_ONE", "MUTATE_SIGMA_TO_U"])
+def test_planted_mistakes_are_caught(orc, mut):
+    assert not _pins_fail(orc, orc.MUTATE_NONE)
+    assert _pins_fail(orc, getattr(orc, mut))
+
+
+def test_oracle_dsyrk_dgemmcd_closed_forms(orc):
+    rng = np.random.default_rng(3)
+    M, K = 70, 9
+    AU, AV = rng.standard_normal((M, K)), rng.standard_normal((M, K))
+    CD = rng.standard_normal((M, M))
+    out, _ = orc.dsyrk(-1.0, AU, AV, CD)
+    ref = CD - AU @ (AV.T @ AV) @ AU.T
+    assert rel_fro(np.tril(out), np.tril(ref)) < 1e-13
+    assert np.all(np.triu(out, 1) == 0)
+    Bu, Bv = rng.standard_normal((M, 5)), rng.standard_normal((M, 5))
+    out = orc.dgemmcd(AU, AV, Bu, Bv, CD)
+    assert rel_fro(out, CD - (AU @ AV.T) @ (Bu @ Bv.T).T) < 1e-13
