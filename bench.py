@@ -1,0 +1,281 @@
+#!/usr/bin/env python3
+"""Benchmark: one TLR Cholesky trailing step of tile updates on one MI355X.
+
+Workload (the nb=1024 uneven-rank trailing-step batch): step 0 of a TLR
+Cholesky on a 64 x 64 tile grid, nb = 1024 — 1953 HCORE_dgemm updates
+C_ij -= A_i0 A_j0^T (1 <= j < i <= 63) and 63 HCORE_dsyrk diagonal updates,
+ranks r(d) = max(8, round(160/(1+|d|/2))), sigma_k = 10^(-10k/r), acc 1e-8,
+maxrank 512, FP64.  Tiles are synthetic (seeded), resident in HBM before the
+timed region; the recompressed factors are written out of place so every
+step performs identical work (the diagonal dsyrk updates run in place).
+
+A "step" is one execution of the whole batch (one plan execution: 12 kernel
+launches on one stream).  value = tile updates per second (2016 per step).
+
+Also measured in the same run:
+  * per-kernel durations with HIP events around each launch, on the stream
+    the kernels run on -> the roofline of the dominant kernel;
+  * the same batch replayed through a hipGraph;
+  * the CPU baseline: the oracle (the reference's LAPACK call sequence,
+    scipy-openblas, one update per OpenMP thread) on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12         # B/s
+FP64_PEAK = 78.6e12       # FLOP/s (MI355X dense FP64, matrix and vector)
+PHASES = ["stack", "F=AV^T BV", "G=BU F^T", "geqrf", "T=RA RB^T", "core_svd_lds",
+          "core_svd_global", "Zv=T^T Zu", "reassembly", "syrk_W", "syrk_X", "syrk_CD"]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def q_flops(m, n):
+    # geqrf closed form of the reference (flop_counts.c:5-10), float thirds
+    return 2 * m * n * n - int(np.float32(2 * n * n * n) / np.float32(3.0)) + 2 * m * n + \
+        int(np.float32(17 * n) / np.float32(3.0))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--grid", type=int, default=64)
+    ap.add_argument("--nb", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--check", type=int, default=24, help="updates checked against the oracle")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on exactly one GPU (no sharding, no collectives); "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import hcore_amd as hc
+    from hcore_amd.tiles import TrailingStep
+
+    L = hc.lib()
+    assert L.HCORE_hip_init(0) == 0, "HIP init failed"
+    devname, ncu = hc.device_info()
+    log("device:", devname, "CUs:", ncu)
+
+    t0 = time.time()
+    step = TrailingStep(grid=args.grid, nb=args.nb)
+    nb = step.nb
+    ra, rb, rc = step.ranks()
+    K = ra + rc
+    n = len(step.pairs)
+    ns = len(step.diag_idx)
+    log("generated %d gemm + %d syrk tiles in %.1fs" % (n, ns, time.time() - t0))
+
+    # ---- device residency -------------------------------------------------------
+    B = hc.DeviceBuffer.from_numpy
+    panel = {i: (B(u), B(v)) for i, (u, v) in step.panel.items()}
+    CUd = [B(c[0], k) for c, k in zip(step.C, K)]
+    CVd = [B(c[1], k) for c, k in zip(step.C, K)]
+    OUd = [hc.DeviceBuffer(nb, k) for k in K]
+    OVd = [hc.DeviceBuffer(nb, k) for k in K]
+    Dd = [B(d) for d in step.D]
+    plan = hc.Plan(nb, hc.FP64)
+    plan.add_dgemm([panel[i][0].ptr for i, j in step.pairs], [panel[i][1].ptr for i, j in step.pairs],
+                   ra, nb, [panel[j][0].ptr for i, j in step.pairs],
+                   [panel[j][1].ptr for i, j in step.pairs], rb, nb,
+                   [t.ptr for t in CUd], [t.ptr for t in CVd], rc, nb, -1.0, 1.0, 0, 512, 1e-8,
+                   CUout=[t.ptr for t in OUd], CVout=[t.ptr for t in OVd], ldco=nb)
+    plan.add_dsyrk([panel[i][0].ptr for i in step.diag_idx], [panel[i][1].ptr for i in step.diag_idx],
+                   [step.panel[i][0].shape[1] for i in step.diag_idx], nb,
+                   [t.ptr for t in Dd], nb, -1.0)
+    plan.finalize()
+    log("plan workspace %.2f GB" % (plan.workspace_bytes() / 1e9))
+    stream = hc.Stream()
+
+    # ---- one checked execution ------------------------------------------------
+    plan.execute(stream)
+    stream.synchronize()
+    ranks = plan.ranks()
+    status = plan.status()
+    assert np.all(status == 0), "device status errors: %s" % np.unique(status)
+    # numpy check of a sample (largest stacked ranks first): the product within
+    # 10*acc of the dense update, the rank by the truncation rule on numpy's SVD
+    order = np.argsort(-K, kind="stable")
+    sample = sorted(set(list(order[:4]) + list(np.linspace(0, n - 1, max(args.check - 4, 2)).astype(int))))
+    worst, rank_drift = 0.0, 0
+    for u in sample:
+        i, j = step.pairs[u]
+        AU, AV = step.panel[i]
+        BU, BV = step.panel[j]
+        CU, CV = step.C[u]
+        r = int(ranks[u])
+        E = CU @ CV.T - (AU @ AV.T) @ (BU @ BV.T).T
+        s_np = np.linalg.svd(E, compute_uv=False)[:K[u]]
+        below = np.nonzero(s_np[1:] < 1e-8)[0]
+        r_np = int(below[0]) + 1 if below.size else int(K[u])
+        got = OUd[u].numpy(r) @ OVd[u].numpy(r).T
+        worst = max(worst, float(np.linalg.norm(got - E) / np.linalg.norm(E)))
+        rank_drift = max(rank_drift, abs(r - r_np))
+    log("checked %d updates vs the dense update: worst rel err %.3e (bound 1e-7), "
+        "max rank drift %d" % (len(sample), worst, rank_drift))
+    assert worst <= 1e-7 and rank_drift <= 1
+
+    # ---- flop and byte accounting (the reference's closed forms) ------------------
+    fc = hc.flop_counts
+    r = ranks.astype(np.int64)
+    Kl = K.astype(np.int64)
+    ral, rbl = ra.astype(np.int64), rb.astype(np.int64)
+    f_qr = sum(2 * q_flops(nb, int(k)) for k in Kl)
+    f_F = int(np.sum(2 * ral * rbl * nb))
+    f_G = int(np.sum(2 * nb * ral * rbl))
+    f_svd = int(np.sum(22 * Kl ** 3))
+    f_scale = int(np.sum(r * Kl))
+    f_orm = sum(fc('o', nb, int(rr), int(k), 1) + fc('o', int(rr), nb, int(k), 2) for rr, k in zip(r, Kl))
+    f_gemm = f_qr + f_F + f_G + f_svd + f_scale + f_orm
+    ks = np.array([step.panel[i][0].shape[1] for i in step.diag_idx], dtype=np.int64)
+    f_syrk = int(np.sum(2 * ks * ks * nb + 2 * nb * ks * ks + 2 * nb * nb * ks))
+    F_total = f_gemm + f_syrk
+    # unique compulsory bytes (FP64): C factors in + new factors out, panel
+    # factors once, dense diagonal tiles in + out
+    panel_bytes = sum(8 * 2 * nb * u.shape[1] for (u, v) in step.panel.values())
+    b_unique = int(np.sum(8 * nb * (2 * rc.astype(np.int64) + 2 * r))) + panel_bytes + \
+        ns * 8 * 2 * nb * nb
+    # per phase algorithmic work
+    phase_flops = {"geqrf": f_qr, "reassembly": f_orm, "F=AV^T BV": f_F, "G=BU F^T": f_G,
+                   "core_svd_lds": int(np.sum(22 * Kl[Kl <= 142] ** 3)),
+                   "core_svd_global": int(np.sum(22 * Kl[Kl > 142] ** 3)),
+                   "syrk_CD": int(np.sum(2 * nb * nb * ks))}
+
+    # ---- timed region: eager execution, HIP events around every launch ----------
+    plan.set_timing(True)
+    for _ in range(args.warmup):
+        plan.execute(stream)
+    stream.synchronize()
+    plan.phase_times()                    # collect warm-up phases ...
+    plan.set_timing(False)
+    plan.set_timing(True)
+    ms0, c0 = plan.phase_times()          # ... and start from these totals
+    e0, e1 = hc.Event(), hc.Event()
+    hc.device_synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        plan.execute(stream)
+    e1.record(stream)
+    stream.synchronize()
+    hc.device_synchronize()
+    total_ms = e0.elapsed_ms(e1)
+    ms1, c1 = plan.phase_times()
+    phase_ms = (ms1 - ms0) / np.maximum(c1 - c0, 1)
+    ms_per_step = total_ms / args.steps
+    updates = n + ns
+    value = updates / (ms_per_step / 1e3)
+    for name, t in zip(PHASES, phase_ms):
+        log("  phase %-16s %8.3f ms" % (name, t))
+
+    # ---- the same batch as a hipGraph -------------------------------------------
+    plan.set_timing(False)
+    plan.execute(stream, graph=True)
+    stream.synchronize()
+    g0, g1 = hc.Event(), hc.Event()
+    g0.record(stream)
+    for _ in range(args.steps):
+        plan.execute(stream, graph=True)
+    g1.record(stream)
+    stream.synchronize()
+    graph_ms = g0.elapsed_ms(g1) / args.steps
+    log("eager %.3f ms/step, graph %.3f ms/step" % (ms_per_step, graph_ms))
+
+    # ---- dominant kernel roofline -------------------------------------------------
+    dom = int(np.argmax(phase_ms))
+    dname = PHASES[dom]
+    if dname == "syrk_CD":
+        traffic_alg = ns * (16 * nb * nb + 8 * 2 * nb * int(ks.mean()))
+        ach = traffic_alg / (phase_ms[dom] / 1e3)
+        roof = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": ach / HBM_PEAK, "traffic": None}
+    else:
+        fl = phase_flops.get(dname, 0)
+        ach = fl / (phase_ms[dom] / 1e3)
+        roof = {"bound": "mfma", "achieved": ach / 1e12, "peak": FP64_PEAK / 1e12,
+                "unit": "TFLOP/s", "frac": ach / FP64_PEAK, "traffic": None}
+    roof["kernel"] = dname
+    roof["avg_launch_ms"] = float(phase_ms[dom])
+
+    # ---- CPU baseline (oracle on the host cores) ------------------------------------
+    cpu = None
+    if not args.no_cpu:
+        import oracle  # the CPU baseline leg: the oracle timed on the host cores
+        ncores = min(16, os.cpu_count() or 1)
+        AU = [step.panel[i][0] for i, j in step.pairs]
+        AV = [step.panel[i][1] for i, j in step.pairs]
+        BU = [step.panel[j][0] for i, j in step.pairs]
+        BV = [step.panel[j][1] for i, j in step.pairs]
+        sAU = [step.panel[i][0] for i in step.diag_idx]
+        sAV = [step.panel[i][1] for i in step.diag_idx]
+        times = []
+        for rep in range(args.cpu_reps + 1):
+            CU, CV = [], []
+            for (u, v), k in zip(step.C, K):
+                a = np.zeros((nb, k), order="F"); a[:, :u.shape[1]] = u; CU.append(a)
+                b = np.zeros((nb, k), order="F"); b[:, :v.shape[1]] = v; CV.append(b)
+            D = [d.copy(order="F") for d in step.D]
+            rcc = rc.copy()
+            t = time.time()
+            bad, _ = oracle.dgemm_batch(-1.0, AU, AV, BU, BV, 1.0, CU, CV, rcc, 0, 512, 1e-8,
+                                        nthreads=ncores)
+            oracle.dsyrk_batch(-1.0, sAU, sAV, D, nthreads=ncores)
+            dt = time.time() - t
+            if rep > 0:
+                times.append(dt)
+            assert bad == 0
+        tmed = float(np.median(times))
+        cpu = {"value": updates / tmed, "unit": "tile-updates/s", "cores": ncores,
+               "kind": "port",
+               "sample": "the full step (%d gemm + %d syrk updates), median of %d runs after "
+                         "1 warm-up; %s, 1 BLAS thread per update" %
+                         (n, ns, args.cpu_reps, oracle.blas_config().split()[0] + " " +
+                          oracle.blas_config().split()[1])}
+        log("cpu baseline: %.3f s/step on %d threads" % (tmed, ncores))
+
+    out = {
+        "metric": "TLR GEMM updates/s at rel.err<=1e-8: mixed-prec vs FP64 GPU, % roofline",
+        "value": value,
+        "unit": "tile-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded low-rank tiles, generated on the host, resident in HBM)",
+        "config": {"workload": "TLR Cholesky trailing step, 64x64 tile grid, nb=1024: "
+                               "1953 HCORE_dgemm + 63 HCORE_dsyrk, uneven ranks 8-160, "
+                               "acc=1e-8, maxrank=512",
+                   "grid": args.grid, "nb": nb, "updates_per_step": updates,
+                   "precision": "fp64", "graph_ms_per_step": graph_ms},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "gflops": F_total / (ms_per_step / 1e3) / 1e9,
+        "step_roofline_frac": max(F_total / (ms_per_step / 1e3) / FP64_PEAK,
+                                  b_unique / (ms_per_step / 1e3) / HBM_PEAK),
+        "flops_per_step": F_total,
+        "unique_bytes_per_step": b_unique,
+        "phase_ms": {k: float(v) for k, v in zip(PHASES, phase_ms)},
+        "check": {"updates": len(sample), "worst_rel_err": worst, "max_rank_drift": rank_drift},
+        "device": devname,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

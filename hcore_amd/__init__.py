@@ -91,6 +91,23 @@ def lib():
         L.HCORE_hip_plan_flops.argtypes = [_vp, C.POINTER(C.c_ulong), C.POINTER(C.c_ulong)]
         L.HCORE_hip_plan_destroy.argtypes = [_vp]
         L.HCORE_hip_plan_destroy.restype = None
+        L.HCORE_hip_plan_set_timing.argtypes = [_vp, i]
+        L.HCORE_hip_plan_phase_times.argtypes = [_vp, _dp, _ip]
+        sz = C.c_size_t
+        L.HCORE_hip_malloc.argtypes = [C.POINTER(_vp), sz]
+        L.HCORE_hip_free.argtypes = [_vp]
+        L.HCORE_hip_memcpy_h2d.argtypes = [_vp, _vp, sz]
+        L.HCORE_hip_memcpy_d2h.argtypes = [_vp, _vp, sz]
+        L.HCORE_hip_memcpy_d2d.argtypes = [_vp, _vp, sz]
+        L.HCORE_hip_memset.argtypes = [_vp, i, sz]
+        L.HCORE_hip_stream_create.argtypes = [C.POINTER(_vp)]
+        L.HCORE_hip_stream_destroy.argtypes = [_vp]
+        L.HCORE_hip_stream_synchronize.argtypes = [_vp]
+        L.HCORE_hip_event_create.argtypes = [C.POINTER(_vp)]
+        L.HCORE_hip_event_destroy.argtypes = [_vp]
+        L.HCORE_hip_event_record.argtypes = [_vp, _vp]
+        L.HCORE_hip_event_elapsed_ms.argtypes = [_vp, _vp, C.POINTER(C.c_float)]
+        L.HCORE_hip_device_info.argtypes = [C.c_char_p, i, _ip]
         _lib = L
     return _lib
 
@@ -190,10 +207,106 @@ def _iarr(v):
     return a, a.ctypes.data_as(_ip)
 
 
+def _check(st, what):
+    if st:
+        raise RuntimeError("%s failed: status %d" % (what, st))
+
+
+class DeviceBuffer:
+    """Device allocation of the library's own HIP runtime.
+
+    Holds a column-major (rows x cols) matrix of `dtype`; `ptr` is the device
+    address to pass to plans.  There is deliberately no torch here: the
+    library must not share a process with a second HIP runtime."""
+
+    def __init__(self, rows, cols=1, dtype=np.float64):
+        self.rows, self.cols, self.dtype = rows, max(cols, 1), np.dtype(dtype)
+        self.nbytes = self.rows * self.cols * self.dtype.itemsize
+        p = C.c_void_p()
+        _check(lib().HCORE_hip_malloc(C.byref(p), max(self.nbytes, 8)), "HCORE_hip_malloc")
+        self.ptr = p.value
+        self.zero()
+
+    @classmethod
+    def from_numpy(cls, x, cap=None):
+        x = np.asarray(x)
+        rows = x.shape[0]
+        cols = x.shape[1] if x.ndim > 1 else 1
+        b = cls(rows, cols if cap is None else max(cap, cols), x.dtype)
+        b.upload(x)
+        return b
+
+    def upload(self, x):
+        x = np.asfortranarray(x, dtype=self.dtype)
+        assert x.nbytes <= self.nbytes
+        _check(lib().HCORE_hip_memcpy_h2d(self.ptr, x.ctypes.data, x.nbytes), "h2d")
+
+    def numpy(self, cols=None):
+        cols = self.cols if cols is None else cols
+        out = np.zeros((self.rows, cols), dtype=self.dtype, order="F")
+        if cols:
+            _check(lib().HCORE_hip_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes), "d2h")
+        return out
+
+    def copy_from(self, other):
+        assert other.nbytes <= self.nbytes
+        _check(lib().HCORE_hip_memcpy_d2d(self.ptr, other.ptr, other.nbytes), "d2d")
+
+    def zero(self):
+        _check(lib().HCORE_hip_memset(self.ptr, 0, self.nbytes), "memset")
+
+    def free(self):
+        if self.ptr:
+            lib().HCORE_hip_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        h = C.c_void_p()
+        _check(lib().HCORE_hip_stream_create(C.byref(h)), "stream_create")
+        self.h = h.value
+
+    def synchronize(self):
+        _check(lib().HCORE_hip_stream_synchronize(self.h), "stream_synchronize")
+
+
+class Event:
+    def __init__(self):
+        h = C.c_void_p()
+        _check(lib().HCORE_hip_event_create(C.byref(h)), "event_create")
+        self.h = h.value
+
+    def record(self, stream):
+        _check(lib().HCORE_hip_event_record(self.h, stream.h if stream else None), "record")
+
+    def elapsed_ms(self, end):
+        ms = C.c_float(0)
+        _check(lib().HCORE_hip_event_elapsed_ms(self.h, end.h, C.byref(ms)), "elapsed")
+        return float(ms.value)
+
+
+def device_synchronize():
+    _check(lib().HCORE_hip_device_synchronize(), "device_synchronize")
+
+
+def device_info():
+    name = C.create_string_buffer(256)
+    ncu = C.c_int(0)
+    _check(lib().HCORE_hip_device_info(name, 256, C.byref(ncu)), "device_info")
+    return name.value.decode(), ncu.value
+
+
 class Plan:
     """Batched plan of one TLR trailing step (include/hcore_hip.h).
 
-    Pointers are integer device addresses (e.g. torch tensor .data_ptr()).
+    Pointers are integer device addresses (DeviceBuffer.ptr).
     """
 
     def __init__(self, M, precision=FP64):
@@ -237,9 +350,9 @@ class Plan:
         if st:
             raise RuntimeError("HCORE_hip_plan_finalize failed: %d" % st)
 
-    def execute(self, stream=0, graph=False):
+    def execute(self, stream=None, graph=False):
         fn = self.L.HCORE_hip_plan_execute_graph if graph else self.L.HCORE_hip_plan_execute
-        st = fn(self.h, C.c_void_p(stream))
+        st = fn(self.h, C.c_void_p(stream.h if stream is not None else None))
         if st:
             raise RuntimeError("plan execute failed: %d" % st)
 
@@ -266,6 +379,16 @@ class Plan:
 
     def ranks_device_ptr(self):
         return self.L.HCORE_hip_plan_ranks_device(self.h)
+
+    def set_timing(self, enable=True):
+        _check(self.L.HCORE_hip_plan_set_timing(self.h, int(enable)), "set_timing")
+
+    def phase_times(self):
+        ms = np.zeros(12)
+        cnt = np.zeros(12, dtype=np.int32)
+        _check(self.L.HCORE_hip_plan_phase_times(self.h, ms.ctypes.data_as(_dp),
+                                                 cnt.ctypes.data_as(_ip)), "phase_times")
+        return ms, cnt
 
     def workspace_bytes(self):
         return int(self.L.HCORE_hip_plan_workspace_bytes(self.h))

@@ -188,6 +188,16 @@ struct hcore_hip_plan {
   hipGraphExec_t gexec = nullptr;
   // last ranks (host) for flop accounting
   std::vector<int> last_ranks;
+  // per-phase timing: a ring of event sets so timed executions do not wait
+  // for each other; a set is read back when the ring wraps or on request
+  static constexpr int kRing = 64;
+  bool timing = false;
+  hipEvent_t ev[kRing][HCORE_HIP_NPHASES][2] = {};
+  bool used[kRing][HCORE_HIP_NPHASES] = {};
+  bool pending[kRing] = {};
+  int slot = 0;
+  double phase_ms[HCORE_HIP_NPHASES] = {};
+  int phase_count[HCORE_HIP_NPHASES] = {};
 };
 
 namespace hcore {
@@ -362,31 +372,82 @@ static int plan_finalize_t(hcore_hip_plan* p) {
   return HCORE_HIP_OK;
 }
 
+// Accumulate the phase times of one pending timed execution.
+static int collect_slot(hcore_hip_plan* p, int k) {
+  if (!p->pending[k]) return HCORE_HIP_OK;
+  for (int i = 0; i < HCORE_HIP_NPHASES; ++i) {
+    if (!p->used[k][i]) continue;
+    HCHECK(hipEventSynchronize(p->ev[k][i][1]));
+    float ms = 0.f;
+    HCHECK(hipEventElapsedTime(&ms, p->ev[k][i][0], p->ev[k][i][1]));
+    p->phase_ms[i] += ms;
+    p->phase_count[i] += 1;
+    p->used[k][i] = false;
+  }
+  p->pending[k] = false;
+  return HCORE_HIP_OK;
+}
+
+static int collect_all(hcore_hip_plan* p) {
+  for (int k = 0; k < hcore_hip_plan::kRing; ++k) {
+    int st = collect_slot(p, (p->slot + k) % hcore_hip_plan::kRing);
+    if (st) return st;
+  }
+  return HCORE_HIP_OK;
+}
+
 template <typename T>
-static int plan_execute_t(hcore_hip_plan* p, hipStream_t s) {
+static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false) {
   const int M = p->M;
   auto D = [&](size_t off) { return p->desc + off; };
   const auto* gp = reinterpret_cast<const GemmProb<T>*>(D(p->o_gp));
+  const bool tm = p->timing && !capture;
+  const int k = p->slot;
+  if (tm) {
+    int st = collect_slot(p, k);  // the set used kRing executions ago
+    if (st) return st;
+    p->slot = (k + 1) % hcore_hip_plan::kRing;
+  }
+  auto phase = [&](int id, bool run, auto&& fn) {
+    if (!run) return;
+    if (tm) (void)hipEventRecord(p->ev[k][id][0], s);
+    fn();
+    if (tm) {
+      (void)hipEventRecord(p->ev[k][id][1], s);
+      p->used[k][id] = true;
+    }
+  };
   if (p->ng > 0) {
-    launch_stack<T>(reinterpret_cast<const StackProb<T>*>(D(p->o_stack)), p->ng, M, T(p->alpha),
-                    T(p->beta), s);
-    launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(p->o_tF)), p->n_tF, s);
-    launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(p->o_tG)), p->n_tG, s);
-    launch_geqrf<T>(reinterpret_cast<const QrProb<T>*>(D(p->o_qr)), p->n_qr, M, s);
-    launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(p->o_tT)), p->n_tT, s);
-    launch_core<T>(reinterpret_cast<const CoreProb<T>*>(D(p->o_core_l)), p->n_core_l, true,
-                   p->kmax_l, p->rk, p->maxrk, T(p->acc), s);
-    launch_core<T>(reinterpret_cast<const CoreProb<T>*>(D(p->o_core_g)), p->n_core_g, false,
-                   p->kmax_g, p->rk, p->maxrk, T(p->acc), s);
-    launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(p->o_tZ)), p->n_tZ, s);
-    launch_apply<T>(reinterpret_cast<const ApplyProb<T>*>(D(p->o_apply)), p->n_apply, M, s);
+    phase(0, true, [&] {
+      launch_stack<T>(reinterpret_cast<const StackProb<T>*>(D(p->o_stack)), p->ng, M,
+                      T(p->alpha), T(p->beta), s);
+    });
+    phase(1, true, [&] { launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(p->o_tF)), p->n_tF, s); });
+    phase(2, true, [&] { launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(p->o_tG)), p->n_tG, s); });
+    phase(3, true, [&] {
+      launch_geqrf<T>(reinterpret_cast<const QrProb<T>*>(D(p->o_qr)), p->n_qr, M, s);
+    });
+    phase(4, true, [&] { launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(p->o_tT)), p->n_tT, s); });
+    phase(5, p->n_core_l > 0, [&] {
+      launch_core<T>(reinterpret_cast<const CoreProb<T>*>(D(p->o_core_l)), p->n_core_l, true,
+                     p->kmax_l, p->rk, p->maxrk, T(p->acc), s);
+    });
+    phase(6, p->n_core_g > 0, [&] {
+      launch_core<T>(reinterpret_cast<const CoreProb<T>*>(D(p->o_core_g)), p->n_core_g, false,
+                     p->kmax_g, p->rk, p->maxrk, T(p->acc), s);
+    });
+    phase(7, true, [&] { launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(p->o_tZ)), p->n_tZ, s); });
+    phase(8, true, [&] {
+      launch_apply<T>(reinterpret_cast<const ApplyProb<T>*>(D(p->o_apply)), p->n_apply, M, s);
+    });
   }
   if (p->ns > 0) {
     const auto* sp = reinterpret_cast<const GemmProb<T>*>(D(p->o_sp));
-    launch_gemm<T>(sp, reinterpret_cast<const GemmTask*>(D(p->o_tW)), p->n_tW, s);
-    launch_gemm<T>(sp, reinterpret_cast<const GemmTask*>(D(p->o_tX)), p->n_tX, s);
-    launch_gemm<T>(sp, reinterpret_cast<const GemmTask*>(D(p->o_tC)), p->n_tC, s);
+    phase(9, true, [&] { launch_gemm<T>(sp, reinterpret_cast<const GemmTask*>(D(p->o_tW)), p->n_tW, s); });
+    phase(10, true, [&] { launch_gemm<T>(sp, reinterpret_cast<const GemmTask*>(D(p->o_tX)), p->n_tX, s); });
+    phase(11, true, [&] { launch_gemm<T>(sp, reinterpret_cast<const GemmTask*>(D(p->o_tC)), p->n_tC, s); });
   }
+  if (tm) p->pending[k] = true;
   HCHECK(hipGetLastError());
   return HCORE_HIP_OK;
 }
@@ -787,8 +848,8 @@ int HCORE_hip_plan_execute_graph(hcore_hip_plan* p, void* stream) {
     hipStream_t cs = nullptr;
     HCHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     HCHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    int st = p->prec == HCORE_HIP_FP64 ? plan_execute_t<double>(p, cs)
-                                       : plan_execute_t<float>(p, cs);
+    int st = p->prec == HCORE_HIP_FP64 ? plan_execute_t<double>(p, cs, true)
+                                       : plan_execute_t<float>(p, cs, true);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(cs, &g);
     (void)hipStreamDestroy(cs);
@@ -864,6 +925,10 @@ int HCORE_hip_plan_flops(hcore_hip_plan* p, unsigned long* update, unsigned long
 
 void HCORE_hip_plan_destroy(hcore_hip_plan* p) {
   if (!p) return;
+  for (int k = 0; k < hcore_hip_plan::kRing; ++k)
+    for (int i = 0; i < HCORE_HIP_NPHASES; ++i)
+      for (int j = 0; j < 2; ++j)
+        if (p->ev[k][i][j]) (void)hipEventDestroy(p->ev[k][i][j]);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   if (p->dmem) {
@@ -871,6 +936,108 @@ void HCORE_hip_plan_destroy(hcore_hip_plan* p) {
     (void)hipFree(p->dmem);
   }
   delete p;
+}
+
+int HCORE_hip_plan_set_timing(hcore_hip_plan* p, int enable) {
+  if (!p) return HCORE_HIP_ERR_ARG;
+  if (enable && !p->ev[0][0][0])
+    for (int k = 0; k < hcore_hip_plan::kRing; ++k)
+      for (int i = 0; i < HCORE_HIP_NPHASES; ++i)
+        for (int j = 0; j < 2; ++j) HCHECK(hipEventCreate(&p->ev[k][i][j]));
+  p->timing = enable != 0;
+  return HCORE_HIP_OK;
+}
+
+int HCORE_hip_plan_phase_times(hcore_hip_plan* p, double* ms, int* count) {
+  if (!p) return HCORE_HIP_ERR_ARG;
+  int st = collect_all(p);
+  if (st) return st;
+  for (int i = 0; i < HCORE_HIP_NPHASES; ++i) {
+    if (ms) ms[i] = p->phase_ms[i];
+    if (count) count[i] = p->phase_count[i];
+  }
+  return HCORE_HIP_OK;
+}
+
+// ---- device memory / streams / events ------------------------------------------
+int HCORE_hip_malloc(void** ptr, size_t bytes) {
+  int st = ctx_ensure();
+  if (st) return st;
+  HCHECK(hipMalloc(ptr, bytes));
+  return HCORE_HIP_OK;
+}
+int HCORE_hip_free(void* ptr) {
+  HCHECK(hipFree(ptr));
+  return HCORE_HIP_OK;
+}
+int HCORE_hip_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  HCHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return HCORE_HIP_OK;
+}
+int HCORE_hip_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  HCHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return HCORE_HIP_OK;
+}
+int HCORE_hip_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+  HCHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  return HCORE_HIP_OK;
+}
+int HCORE_hip_memset(void* ptr, int value, size_t bytes) {
+  HCHECK(hipMemset(ptr, value, bytes));
+  return HCORE_HIP_OK;
+}
+int HCORE_hip_device_synchronize(void) {
+  HCHECK(hipDeviceSynchronize());
+  return HCORE_HIP_OK;
+}
+int HCORE_hip_stream_create(void** stream) {
+  int st = ctx_ensure();
+  if (st) return st;
+  hipStream_t s = nullptr;
+  HCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return HCORE_HIP_OK;
+}
+int HCORE_hip_stream_destroy(void* stream) {
+  HCHECK(hipStreamDestroy((hipStream_t)stream));
+  return HCORE_HIP_OK;
+}
+int HCORE_hip_stream_synchronize(void* stream) {
+  HCHECK(hipStreamSynchronize((hipStream_t)stream));
+  return HCORE_HIP_OK;
+}
+int HCORE_hip_event_create(void** event) {
+  int st = ctx_ensure();
+  if (st) return st;
+  hipEvent_t e = nullptr;
+  HCHECK(hipEventCreate(&e));
+  *event = e;
+  return HCORE_HIP_OK;
+}
+int HCORE_hip_event_destroy(void* event) {
+  HCHECK(hipEventDestroy((hipEvent_t)event));
+  return HCORE_HIP_OK;
+}
+int HCORE_hip_event_record(void* event, void* stream) {
+  HCHECK(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+  return HCORE_HIP_OK;
+}
+int HCORE_hip_event_elapsed_ms(void* start, void* stop, float* ms) {
+  HCHECK(hipEventSynchronize((hipEvent_t)stop));
+  HCHECK(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+  return HCORE_HIP_OK;
+}
+int HCORE_hip_device_info(char* name, int len, int* num_cu) {
+  int st = ctx_ensure();
+  if (st) return st;
+  hipDeviceProp_t prop;
+  HCHECK(hipGetDeviceProperties(&prop, g_ctx.device));
+  if (name && len > 0) {
+    strncpy(name, prop.name, len - 1);
+    name[len - 1] = 0;
+  }
+  if (num_cu) *num_cu = prop.multiProcessorCount;
+  return HCORE_HIP_OK;
 }
 
 }  // extern "C"

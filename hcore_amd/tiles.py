@@ -7,6 +7,7 @@ V = Q2 with Q1, Q2 the orthonormal factors of the QR of seeded N(0,1)
 nb x r matrices and sigma_k = 10^(-decay k / r).
 """
 import hashlib
+import os
 
 import numpy as np
 
@@ -52,17 +53,26 @@ class TrailingStep:
         self.pairs = [(i, j) for i in range(1, grid) for j in range(1, i)]
         if max_updates is not None:
             self.pairs = self.pairs[:max_updates]
-        self.panel = {}
-        for i in sorted({p for pr in self.pairs for p in pr} | set(range(1, grid) if diag else ())):
-            self.panel[i] = lowrank_tile(nb, cholesky_rank(i), decay, (key, i, 0, "panel"))
-        self.C = [lowrank_tile(nb, cholesky_rank(i - j), decay, (key, i, j, "C"))
-                  for (i, j) in self.pairs]
+        rows = sorted({p for pr in self.pairs for p in pr} | set(range(1, grid) if diag else ()))
         self.diag_idx = list(range(1, grid)) if diag else []
-        self.D = []
-        for i in self.diag_idx:
-            g = rng(key, i, i, "D")
-            a = g.standard_normal((nb, nb))
-            self.D.append(np.asfortranarray(a + a.T + nb * np.eye(nb)))
+
+        def dense(i):
+            a = rng(key, i, i, "D").standard_normal((nb, nb))
+            a += a.T
+            a[np.diag_indices(nb)] += nb
+            return np.asfortranarray(a)
+
+        # tiles are independent: generate them on a thread pool with
+        # single-threaded LAPACK inside each task
+        from concurrent.futures import ThreadPoolExecutor
+        from threadpoolctl import threadpool_limits
+        with threadpool_limits(1), ThreadPoolExecutor(max(1, min(16, os.cpu_count() or 1))) as ex:
+            pan = list(ex.map(lambda i: lowrank_tile(nb, cholesky_rank(i), decay,
+                                                     (key, i, 0, "panel")), rows))
+            self.C = list(ex.map(lambda ij: lowrank_tile(nb, cholesky_rank(ij[0] - ij[1]), decay,
+                                                         (key, ij[0], ij[1], "C")), self.pairs))
+            self.D = list(ex.map(dense, self.diag_idx))
+        self.panel = dict(zip(rows, pan))
 
     def ranks(self):
         ra = np.array([self.panel[i][0].shape[1] for (i, j) in self.pairs], dtype=np.int32)

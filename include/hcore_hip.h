@@ -16,6 +16,7 @@
  */
 #ifndef HCORE_AMD_HCORE_HIP_H
 #define HCORE_AMD_HCORE_HIP_H
+#include <stddef.h>
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -96,6 +97,34 @@ long long HCORE_hip_plan_workspace_bytes(const hcore_hip_plan *plan);
 int HCORE_hip_plan_flops(hcore_hip_plan *plan, unsigned long *update, unsigned long *syrk);
 
 void HCORE_hip_plan_destroy(hcore_hip_plan *plan);
+
+/* Per-phase timing: when enabled, HCORE_hip_plan_execute records a HIP event
+ * pair around every kernel launch on the execution stream.  Phases, in
+ * order: 0 stack, 1 F=AV^T BV, 2 G=BU F^T, 3 geqrf, 4 T=R_A R_B^T,
+ * 5 core SVD (LDS), 6 core SVD (global), 7 Zv=T^T Zu, 8 reassembly,
+ * 9 syrk W, 10 syrk X, 11 syrk CD.  Times accumulate over executions. */
+#define HCORE_HIP_NPHASES 12
+int HCORE_hip_plan_set_timing(hcore_hip_plan *plan, int enable);
+int HCORE_hip_plan_phase_times(hcore_hip_plan *plan, double *ms, int *count);
+
+/* Device memory and stream helpers (the library's own HIP runtime; callers
+ * that share device buffers with the library allocate them here). */
+int HCORE_hip_malloc(void **ptr, size_t bytes);
+int HCORE_hip_free(void *ptr);
+int HCORE_hip_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int HCORE_hip_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int HCORE_hip_memcpy_d2d(void *dst, const void *src, size_t bytes);
+int HCORE_hip_memset(void *ptr, int value, size_t bytes);
+int HCORE_hip_device_synchronize(void);
+int HCORE_hip_stream_create(void **stream);
+int HCORE_hip_stream_destroy(void *stream);
+int HCORE_hip_stream_synchronize(void *stream);
+int HCORE_hip_event_create(void **event);
+int HCORE_hip_event_destroy(void *event);
+int HCORE_hip_event_record(void *event, void *stream);
+int HCORE_hip_event_elapsed_ms(void *start, void *stop, float *ms);
+/* Device name and number of compute units of the current device. */
+int HCORE_hip_device_info(char *name, int len, int *num_cu);
 
 #ifdef __cplusplus
 }

@@ -89,81 +89,71 @@ def test_dgemm_vs_oracle(hc, orc, case):
     _say("   rank gpu %d oracle %d  relerr gpu %.3e oracle %.3e" %
          (r, ro, err, rel_fro(oU @ oV.T, E)))
     assert abs(r - ro) <= 1
-    assert err <= 10 * acc
+    if rk == 0:
+        assert err <= 10 * acc
+    else:  # fixed rank: the error is the dropped tail, the same as the oracle's
+        assert err <= rel_fro(oU @ oV.T, E) * (1 + 1e-6) + 10 * acc
     assert np.max(np.abs(CUn.T @ CUn - np.eye(r))) < 1e-10
     if r == ro:
         assert fl.update == ofl  # same closed-form accounting as the reference
         assert rel_fro(CUn @ CVn.T, oU @ oV.T) <= 10 * acc
 
 
-def colmajor(x, cap=None):
-    """Device buffer holding x (M x r) column-major with room for cap columns.
-    Returned as a (cap, M) row-major torch tensor; column j of x is row j."""
-    import torch
-    M, r = x.shape
-    cap = r if cap is None else cap
-    t = torch.zeros((max(cap, 1), M), dtype=torch.float64, device="cuda")
-    if r:
-        t[:r] = torch.from_numpy(np.ascontiguousarray(x.T)).cuda()
-    return t
+def colmajor(hc, x, cap=None):
+    """Device copy of x (M x r, column-major) with room for cap columns."""
+    return hc.DeviceBuffer.from_numpy(np.asfortranarray(x), cap)
 
 
 def test_plan_sigma_and_rank_match_oracle(hc, orc):
     """One update through the plan API: core singular values equal to the
     oracle's within 1e-12 sigma_0, same rank, all math pins."""
-    import torch
     for (seed, M, ra, rb, rc) in [(31, 500, 32, 32, 32), (32, 256, 40, 20, 60),
                                   (33, 1024, 60, 50, 100)]:
         _say("case: plan sigma", seed, M, ra, rb, rc)
         AU, AV, BU, BV, CU, CV = _random_case(seed, M, ra, rb, rc)
         K = ra + rc
-        dA = [colmajor(x) for x in (AU, AV, BU, BV)]
-        cu, cv = colmajor(CU, K), colmajor(CV, K)
+        dA = [colmajor(hc, x) for x in (AU, AV, BU, BV)]
+        cu, cv = colmajor(hc, CU, K), colmajor(hc, CV, K)
         p = hc.Plan(M)
-        p.add_dgemm([dA[0].data_ptr()], [dA[1].data_ptr()], [ra], M,
-                    [dA[2].data_ptr()], [dA[3].data_ptr()], [rb], M,
-                    [cu.data_ptr()], [cv.data_ptr()], [rc], M, -1.0, 1.0, 0, M, 1e-8)
+        p.add_dgemm([dA[0].ptr], [dA[1].ptr], [ra], M, [dA[2].ptr], [dA[3].ptr], [rb], M,
+                    [cu.ptr], [cv.ptr], [rc], M, -1.0, 1.0, 0, M, 1e-8)
         p.finalize()
         p.execute()
-        torch.cuda.synchronize()
+        hc.device_synchronize()
         r = int(p.ranks()[0])
         assert p.status()[0] == 0
         sig = p.sigma(0)
         st, oU, oV, ro, osig, _ = orc.dgemm(-1.0, AU, AV, BU, BV, 1.0, CU, CV, maxrk=M, acc=1e-8)
         assert np.max(np.abs(sig - osig)) <= 1e-12 * osig[0]
         assert r == ro
-        CUn = cu[:r].cpu().numpy().T
-        CVn = cv[:r].cpu().numpy().T
+        CUn, CVn = cu.numpy(r), cv.numpy(r)
         check_recompression(AU, AV, BU, BV, CU, CV, -1.0, 1.0, CUn, CVn, r, sig, 1e-8)
         p.destroy()
 
 
 def _trailing_plan(hc, step, out_of_place):
     """Device buffers + plan for a TrailingStep (gemm updates and dsyrk)."""
-    import torch
     nb = step.nb
     ra, rb, rc = step.ranks()
     K = ra + rc
-    panel = {i: (colmajor(u), colmajor(v)) for i, (u, v) in step.panel.items()}
-    CUd = [colmajor(c[0], k) for c, k in zip(step.C, K)]
-    CVd = [colmajor(c[1], k) for c, k in zip(step.C, K)]
-    outU = [torch.zeros_like(t) for t in CUd] if out_of_place else None
-    outV = [torch.zeros_like(t) for t in CVd] if out_of_place else None
+    panel = {i: (colmajor(hc, u), colmajor(hc, v)) for i, (u, v) in step.panel.items()}
+    CUd = [colmajor(hc, c[0], k) for c, k in zip(step.C, K)]
+    CVd = [colmajor(hc, c[1], k) for c, k in zip(step.C, K)]
+    outU = [hc.DeviceBuffer(nb, k) for k in K] if out_of_place else None
+    outV = [hc.DeviceBuffer(nb, k) for k in K] if out_of_place else None
     p = hc.Plan(nb)
-    p.add_dgemm([panel[i][0].data_ptr() for i, j in step.pairs],
-                [panel[i][1].data_ptr() for i, j in step.pairs], ra, nb,
-                [panel[j][0].data_ptr() for i, j in step.pairs],
-                [panel[j][1].data_ptr() for i, j in step.pairs], rb, nb,
-                [t.data_ptr() for t in CUd], [t.data_ptr() for t in CVd], rc, nb,
-                -1.0, 1.0, 0, 512, 1e-8,
-                CUout=[t.data_ptr() for t in outU] if outU else None,
-                CVout=[t.data_ptr() for t in outV] if outV else None, ldco=nb)
-    Dd = [torch.from_numpy(np.ascontiguousarray(D.T)).cuda() for D in step.D]
+    p.add_dgemm([panel[i][0].ptr for i, j in step.pairs], [panel[i][1].ptr for i, j in step.pairs],
+                ra, nb, [panel[j][0].ptr for i, j in step.pairs],
+                [panel[j][1].ptr for i, j in step.pairs], rb, nb,
+                [t.ptr for t in CUd], [t.ptr for t in CVd], rc, nb, -1.0, 1.0, 0, 512, 1e-8,
+                CUout=[t.ptr for t in outU] if outU else None,
+                CVout=[t.ptr for t in outV] if outV else None, ldco=nb)
+    Dd = [colmajor(hc, D) for D in step.D]
     if step.D:
-        p.add_dsyrk([panel[i][0].data_ptr() for i in step.diag_idx],
-                    [panel[i][1].data_ptr() for i in step.diag_idx],
+        p.add_dsyrk([panel[i][0].ptr for i in step.diag_idx],
+                    [panel[i][1].ptr for i in step.diag_idx],
                     [step.panel[i][0].shape[1] for i in step.diag_idx], nb,
-                    [t.data_ptr() for t in Dd], nb, -1.0)
+                    [t.ptr for t in Dd], nb, -1.0)
     p.finalize()
     keep = (panel, CUd, CVd, outU, outV, Dd)
     return p, keep
@@ -172,18 +162,16 @@ def _trailing_plan(hc, step, out_of_place):
 def test_trailing_step_batch_vs_oracle(hc, orc):
     """A small TLR Cholesky trailing step (8x8 grid, nb=256) in one plan:
     every update within 10*acc of the dense update and the oracle's rank;
-    every diagonal tile equal to the oracle's dsyrk."""
-    import torch
+    every diagonal tile equal to the oracle's dsyrk to 1e-12 relative."""
     from hcore_amd.tiles import TrailingStep
     step = TrailingStep(grid=8, nb=256, key="gpu-test")
     _say("case: trailing step grid 8 nb 256, updates", len(step.pairs))
     p, keep = _trailing_plan(hc, step, out_of_place=False)
     panel, CUd, CVd, _, _, Dd = keep
     p.execute()
-    torch.cuda.synchronize()
+    hc.device_synchronize()
     ranks = p.ranks()
     assert np.all(p.status() == 0)
-    ra, rb, rc = step.ranks()
     for u, (i, j) in enumerate(step.pairs):
         AU, AV = step.panel[i]
         BU, BV = step.panel[j]
@@ -192,42 +180,40 @@ def test_trailing_step_batch_vs_oracle(hc, orc):
         st, oU, oV, ro, osig, _ = orc.dgemm(-1.0, AU, AV, BU, BV, 1.0, CU, CV, maxrk=512,
                                             acc=1e-8)
         E = CU @ CV.T - (AU @ AV.T) @ (BU @ BV.T).T
-        got = CUd[u][:r].cpu().numpy().T @ CVd[u][:r].cpu().numpy()
+        got = CUd[u].numpy(r) @ CVd[u].numpy(r).T
         assert abs(r - ro) <= 1, (u, r, ro)
         assert rel_fro(got, E) <= 10 * 1e-8, u
     for s, i in enumerate(step.diag_idx):
         AU, AV = step.panel[i]
         ref, _ = orc.dsyrk(-1.0, AU, AV, step.D[s])
-        got = Dd[s].cpu().numpy().T
+        got = Dd[s].numpy()
         assert np.max(np.abs(got - ref)) <= 1e-12 * np.abs(ref).max()
     p.destroy()
 
 
 def test_graph_replays_match_eager(hc):
     """The plan captured in a hipGraph and replayed several times (out of
-    place, eager work and synchronisation between replays): every replay
+    place; host copies and synchronisation between replays): every replay
     equals the eager result bit for bit."""
-    import torch
     from hcore_amd.tiles import TrailingStep
-    step = TrailingStep(grid=6, nb=128, key="graph-test", diag=False)
-    _say("case: graph replay grid 6 nb 128")
+    step = TrailingStep(grid=6, nb=256, key="graph-test", diag=False)
+    _say("case: graph replay grid 6 nb 256")
     p, keep = _trailing_plan(hc, step, out_of_place=True)
     _, _, _, outU, outV, _ = keep
     p.execute()
-    torch.cuda.synchronize()
-    eager = [t.clone() for t in outU] + [t.clone() for t in outV]
+    hc.device_synchronize()
+    eager = [t.numpy() for t in outU + outV]
     r0 = p.ranks().copy()
     for rep in range(3):
         for t in outU + outV:
-            t.zero_()
-        torch.cuda.synchronize()
+            t.zero()
+        hc.device_synchronize()
         p.execute(graph=True)
-        torch.cuda.synchronize()
-        _ = torch.ones(1000, device="cuda").sum().item()  # unrelated eager work
-        got = [t.clone() for t in outU] + [t.clone() for t in outV]
+        hc.device_synchronize()
+        got = [t.numpy() for t in outU + outV]
         assert np.array_equal(p.ranks(), r0)
         for a, b in zip(eager, got):
-            assert torch.equal(a, b), rep
+            assert np.array_equal(a, b), rep
     p.destroy()
 
 
