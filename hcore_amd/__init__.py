@@ -86,6 +86,7 @@ def lib():
         L.HCORE_hip_plan_get_ranks.argtypes = [_vp, _ip]
         L.HCORE_hip_plan_get_status.argtypes = [_vp, _ip]
         L.HCORE_hip_plan_get_sigma.argtypes = [_vp, i, _dp, _ip]
+        L.HCORE_hip_plan_get_sweeps.argtypes = [_vp, _ip]
         L.HCORE_hip_plan_workspace_bytes.argtypes = [_vp]
         L.HCORE_hip_plan_workspace_bytes.restype = C.c_longlong
         L.HCORE_hip_plan_flops.argtypes = [_vp, C.POINTER(C.c_ulong), C.POINTER(C.c_ulong)]
@@ -368,6 +369,11 @@ class Plan:
         st = self.L.HCORE_hip_plan_get_status(self.h, out.ctypes.data_as(_ip))
         if st:
             raise RuntimeError("get_status failed: %d" % st)
+        return out[:self.n]
+
+    def sweeps(self):
+        out = np.zeros(max(self.n, 1), dtype=np.int32)
+        _check(self.L.HCORE_hip_plan_get_sweeps(self.h, out.ctypes.data_as(_ip)), "get_sweeps")
         return out[:self.n]
 
     def sigma(self, u):

@@ -178,11 +178,21 @@ struct hcore_hip_plan {
   std::vector<size_t> sigma_off;  // per update, element offset into ws
   char* ws = nullptr;
   // descriptor offsets and counts
-  size_t o_stack = 0, o_gp = 0, o_tF = 0, o_tG = 0, o_tT = 0, o_tZ = 0, o_qr = 0, o_core_l = 0,
-         o_core_g = 0, o_apply = 0, o_sp = 0, o_tW = 0, o_tX = 0, o_tC = 0;
-  int n_tF = 0, n_tG = 0, n_tT = 0, n_tZ = 0, n_qr = 0, n_core_l = 0, n_core_g = 0, n_apply = 0,
-      n_tW = 0, n_tX = 0, n_tC = 0;
-  int kmax_l = 1, kmax_g = 1;
+  size_t o_stack = 0, o_gp = 0, o_tF = 0, o_tG = 0, o_tT = 0, o_tZ = 0, o_qr = 0, o_apply = 0,
+         o_sp = 0, o_tW = 0, o_tX = 0, o_tC = 0;
+  int n_tF = 0, n_tG = 0, n_tT = 0, n_tZ = 0, n_qr = 0, n_apply = 0, n_tW = 0, n_tX = 0,
+      n_tC = 0;
+  struct CoreLaunch {
+    size_t off;
+    int count, variant, kmax;
+  };
+  std::vector<CoreLaunch> core;  // one launch per K bucket
+  int* d_sweeps = nullptr;
+  // fork/join of the concurrent core buckets
+  static constexpr int kMaxAux = 8;
+  hipStream_t aux[kMaxAux] = {};
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_join[kMaxAux] = {};
   // graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -245,8 +255,9 @@ static int plan_finalize_t(hcore_hip_plan* p) {
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return Kv[a] > Kv[b]; });
 
-  // ---- device memory: [rank | status | workspace | descriptors] ----
+  // ---- device memory: [rank | status | sweeps | workspace | descriptors] ----
   Packer pk;
+  int* d_sweeps = nullptr;
   // descriptor contents reference device addresses, so the base must be known
   // first: size the descriptors with a dry run, then allocate, then fill.
   auto build = [&](char* wsbase, int* d_rank, int* d_status) {
@@ -256,7 +267,10 @@ static int plan_finalize_t(hcore_hip_plan* p) {
     std::vector<GemmProb<T>> gp;
     std::vector<GemmTask> tF, tG, tT, tZ, tW, tX, tC;
     std::vector<QrProb<T>> qr;
-    std::vector<CoreProb<T>> core_l, core_g;
+    // K buckets of the core SVD, one kernel variant each
+    constexpr int NV = CORE_K512 + 1;
+    std::vector<std::vector<CoreProb<T>>> core_b(NV);
+    std::vector<int> core_kmax(NV, 1);
     std::vector<ApplyProb<T>> ap;
     for (int idx = 0; idx < n; ++idx) {
       const int u = order[idx];
@@ -295,9 +309,10 @@ static int plan_finalize_t(hcore_hip_plan* p) {
       qr.push_back({Us, W + o_tA[u], W + o_TA[u], K});
       qr.push_back({Vs, W + o_tB[u], W + o_TB[u], K});
       CoreProb<T> c{W + o_Tc[u], (K > KL) ? W + o_Tj[u] : nullptr, W + o_Zu[u], W + o_sg[u],
-                    d_rank + u, d_status + u, K};
-      if (K <= KL) { core_l.push_back(c); p->kmax_l = std::max(p->kmax_l, K); }
-      else { core_g.push_back(c); p->kmax_g = std::max(p->kmax_g, K); }
+                    d_rank + u, d_status + u, d_sweeps + u, K};
+      const int b = core_variant_for(K, KL);
+      core_b[b].push_back(c);
+      core_kmax[b] = std::max(core_kmax[b], K);
       T* CUo = (T*)(p->CUo[u] ? p->CUo[u] : p->CU[u]);
       T* CVo = (T*)(p->CVo[u] ? p->CVo[u] : p->CV[u]);
       ap.push_back({Us, W + o_TA[u], W + o_Zu[u], CUo, d_rank + u, d_status + u, K, p->ldco});
@@ -330,8 +345,11 @@ static int plan_finalize_t(hcore_hip_plan* p) {
     p->o_tT = pk.add(tT); p->n_tT = (int)tT.size();
     p->o_tZ = pk.add(tZ); p->n_tZ = (int)tZ.size();
     p->o_qr = pk.add(qr); p->n_qr = (int)qr.size();
-    p->o_core_l = pk.add(core_l); p->n_core_l = (int)core_l.size();
-    p->o_core_g = pk.add(core_g); p->n_core_g = (int)core_g.size();
+    p->core.clear();
+    for (int b = NV - 1; b >= 0; --b) {  // largest K first: the longest jobs start first
+      if (core_b[b].empty()) continue;
+      p->core.push_back({pk.add(core_b[b]), (int)core_b[b].size(), b, core_kmax[b]});
+    }
     p->o_apply = pk.add(ap); p->n_apply = (int)ap.size();
     p->o_sp = pk.add(sp);
     p->o_tW = pk.add(tW); p->n_tW = (int)tW.size();
@@ -339,7 +357,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
     p->o_tC = pk.add(tC); p->n_tC = (int)tC.size();
   };
   build(nullptr, nullptr, nullptr);  // dry run: descriptor size
-  const size_t head = ((size_t)std::max(n, 1) * 2 * sizeof(int) + 255) & ~size_t(255);
+  const size_t head = ((size_t)std::max(n, 1) * 3 * sizeof(int) + 255) & ~size_t(255);
   const size_t wsb = (ws_elems * es + 255) & ~size_t(255);
   const size_t total = head + wsb + pk.buf.size() + 256;
   char* base = nullptr;
@@ -360,14 +378,22 @@ static int plan_finalize_t(hcore_hip_plan* p) {
   p->ws_bytes = wsb;
   p->d_rank = reinterpret_cast<int*>(base);
   p->d_status = p->d_rank + std::max(n, 1);
+  p->d_sweeps = p->d_status + std::max(n, 1);
+  d_sweeps = p->d_sweeps;
   p->ws = base + head;
   p->desc = base + head + wsb;
-  p->kmax_l = p->kmax_g = 1;
   build(p->ws, p->d_rank, p->d_status);
   HCHECK(hipMemcpyAsync(p->desc, pk.buf.data(), pk.buf.size(), hipMemcpyHostToDevice,
                         g_ctx.stream));
   HCHECK(hipMemsetAsync(base, 0, head, g_ctx.stream));
   HCHECK(hipStreamSynchronize(g_ctx.stream));
+  if (p->core.size() > 1 && !p->ev_fork) {
+    HCHECK(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
+    for (int i = 1; i < (int)p->core.size() && i < hcore_hip_plan::kMaxAux; ++i) {
+      HCHECK(hipStreamCreateWithFlags(&p->aux[i], hipStreamNonBlocking));
+      HCHECK(hipEventCreateWithFlags(&p->ev_join[i], hipEventDisableTiming));
+    }
+  }
   p->finalized = true;
   return HCORE_HIP_OK;
 }
@@ -428,13 +454,19 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
       launch_geqrf<T>(reinterpret_cast<const QrProb<T>*>(D(p->o_qr)), p->n_qr, M, s);
     });
     phase(4, true, [&] { launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(p->o_tT)), p->n_tT, s); });
-    phase(5, p->n_core_l > 0, [&] {
-      launch_core<T>(reinterpret_cast<const CoreProb<T>*>(D(p->o_core_l)), p->n_core_l, true,
-                     p->kmax_l, p->rk, p->maxrk, T(p->acc), s);
-    });
-    phase(6, p->n_core_g > 0, [&] {
-      launch_core<T>(reinterpret_cast<const CoreProb<T>*>(D(p->o_core_g)), p->n_core_g, false,
-                     p->kmax_g, p->rk, p->maxrk, T(p->acc), s);
+    // the K buckets run concurrently: fork onto auxiliary streams, join back
+    phase(5, !p->core.empty(), [&] {
+      const int nbk = (int)p->core.size();
+      if (nbk > 1) (void)hipEventRecord(p->ev_fork, s);
+      for (int i = 0; i < nbk; ++i) {
+        const auto& c = p->core[i];
+        hipStream_t sb = i == 0 ? s : p->aux[i];
+        if (i > 0) (void)hipStreamWaitEvent(sb, p->ev_fork, 0);
+        launch_core<T>(reinterpret_cast<const CoreProb<T>*>(D(c.off)), c.count, c.variant,
+                       c.kmax, p->rk, p->maxrk, T(p->acc), sb);
+        if (i > 0) (void)hipEventRecord(p->ev_join[i], sb);
+      }
+      for (int i = 1; i < nbk; ++i) (void)hipStreamWaitEvent(s, p->ev_join[i], 0);
     });
     phase(7, true, [&] { launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(p->o_tZ)), p->n_tZ, s); });
     phase(8, true, [&] {
@@ -520,6 +552,11 @@ static void single_gemm(int M, T alpha, T* AU, T* AV, T* Ark, int LDA, T* BU, T*
   }
   std::lock_guard<std::mutex> lk(g_ctx.mu);
   hipStream_t s = g_ctx.stream;
+  if (K > kCoreMaxK) {
+    snprintf(msg, sizeof msg, "%s: stacked rank %d exceeds the supported maximum %d", name, K,
+             kCoreMaxK);
+    return fail(HCORE_HIP_ERR_ARG, msg);
+  }
   const bool dev = is_device_ptr(CU) && is_device_ptr(CV) && is_device_ptr(AU) &&
                    is_device_ptr(AV) && is_device_ptr(BU) && is_device_ptr(BV);
   hcore_hip_plan p;
@@ -805,6 +842,7 @@ int HCORE_hip_plan_add_dgemm(hcore_hip_plan* p, int n, void* const* AU, void* co
     if (Ark[u] <= 0 || Brk[u] <= 0 || Crk[u] <= 0) return HCORE_HIP_ERR_ZERO_RANK;
     if (K > 2 * maxrk) return HCORE_HIP_ERR_SUM_RANKS;
     if (K > maxrk || K > p->M) return HCORE_HIP_ERR_K_GT_MAXRK;
+    if (K > kCoreMaxK) return HCORE_HIP_ERR_ARG;  // stacked rank beyond the core SVD kernels
   }
   p->ng = n;
   p->AU.assign(AU, AU + n); p->AV.assign(AV, AV + n);
@@ -880,6 +918,13 @@ int HCORE_hip_plan_get_status(hcore_hip_plan* p, int* out) {
   return HCORE_HIP_OK;
 }
 
+int HCORE_hip_plan_get_sweeps(hcore_hip_plan* p, int* out) {
+  if (!p || !p->finalized || !out) return HCORE_HIP_ERR_ARG;
+  HCHECK(hipDeviceSynchronize());
+  if (p->ng) HCHECK(hipMemcpy(out, p->d_sweeps, sizeof(int) * p->ng, hipMemcpyDeviceToHost));
+  return HCORE_HIP_OK;
+}
+
 int HCORE_hip_plan_get_sigma(hcore_hip_plan* p, int u, double* out, int* K_out) {
   if (!p || !p->finalized || u < 0 || u >= p->ng) return HCORE_HIP_ERR_ARG;
   const int K = p->ra[u] + p->rc[u];
@@ -929,6 +974,11 @@ void HCORE_hip_plan_destroy(hcore_hip_plan* p) {
     for (int i = 0; i < HCORE_HIP_NPHASES; ++i)
       for (int j = 0; j < 2; ++j)
         if (p->ev[k][i][j]) (void)hipEventDestroy(p->ev[k][i][j]);
+  for (int i = 0; i < hcore_hip_plan::kMaxAux; ++i) {
+    if (p->aux[i]) (void)hipStreamDestroy(p->aux[i]);
+    if (p->ev_join[i]) (void)hipEventDestroy(p->ev_join[i]);
+  }
+  if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   if (p->dmem) {
@@ -1033,7 +1083,7 @@ int HCORE_hip_device_info(char* name, int len, int* num_cu) {
   hipDeviceProp_t prop;
   HCHECK(hipGetDeviceProperties(&prop, g_ctx.device));
   if (name && len > 0) {
-    strncpy(name, prop.name, len - 1);
+    strncpy(name, prop.name[0] ? prop.name : prop.gcnArchName, len - 1);
     name[len - 1] = 0;
   }
   if (num_cu) *num_cu = prop.multiProcessorCount;

@@ -76,6 +76,7 @@ struct CoreProb {
   T* sigma;     // K singular values, descending
   int* rank;    // new rank r
   int* status;  // ST_*
+  int* sweeps;  // Jacobi sweeps used (diagnostic, may be null)
   int K;
 };
 
@@ -104,8 +105,14 @@ template <typename T>
 void launch_stack(const StackProb<T>* probs, int n, int nb, T alpha, T beta, hipStream_t s);
 template <typename T>
 void launch_geqrf(const QrProb<T>* probs, int n, int nb, hipStream_t s);
+// core SVD variants by stacked rank K: one wavefront for K <= 64, one
+// workgroup with the core in LDS up to the LDS limit, 1024 threads with the
+// core in global memory beyond it
+enum : int { CORE_K16, CORE_K32, CORE_K64, CORE_K128, CORE_KLDS, CORE_K256, CORE_K512 };
+int core_variant_for(int K, int lds_max_k);
+constexpr int kCoreMaxK = 512;
 template <typename T>
-void launch_core(const CoreProb<T>* probs, int n, bool in_lds, int kmax, int rk, int maxrk,
+void launch_core(const CoreProb<T>* probs, int n, int variant, int kmax, int rk, int maxrk,
                  T acc, hipStream_t s);
 template <typename T>
 void launch_apply(const ApplyProb<T>* probs, int n, int nb, hipStream_t s);

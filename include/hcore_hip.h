@@ -88,6 +88,8 @@ const int *HCORE_hip_plan_ranks_device(const hcore_hip_plan *plan);
 /* Copy new ranks / per-update status (HCORE_HIP_*) to host; synchronizes. */
 int HCORE_hip_plan_get_ranks(hcore_hip_plan *plan, int *ranks_host);
 int HCORE_hip_plan_get_status(hcore_hip_plan *plan, int *status_host);
+/* Copy the number of Jacobi sweeps each update's core SVD used (diagnostic). */
+int HCORE_hip_plan_get_sweeps(hcore_hip_plan *plan, int *sweeps_host);
 /* Copy the K singular values of update u (descending, FP64) to host. */
 int HCORE_hip_plan_get_sigma(hcore_hip_plan *plan, int u, double *sigma_host, int *K_out);
 /* Bytes of device workspace held by the plan. */
@@ -101,8 +103,9 @@ void HCORE_hip_plan_destroy(hcore_hip_plan *plan);
 /* Per-phase timing: when enabled, HCORE_hip_plan_execute records a HIP event
  * pair around every kernel launch on the execution stream.  Phases, in
  * order: 0 stack, 1 F=AV^T BV, 2 G=BU F^T, 3 geqrf, 4 T=R_A R_B^T,
- * 5 core SVD (LDS), 6 core SVD (global), 7 Zv=T^T Zu, 8 reassembly,
- * 9 syrk W, 10 syrk X, 11 syrk CD.  Times accumulate over executions. */
+ * 5 core SVD (all K buckets, run concurrently), 6 reserved, 7 Zv=T^T Zu,
+ * 8 reassembly, 9 syrk W, 10 syrk X, 11 syrk CD.  Times accumulate over
+ * executions. */
 #define HCORE_HIP_NPHASES 12
 int HCORE_hip_plan_set_timing(hcore_hip_plan *plan, int enable);
 int HCORE_hip_plan_phase_times(hcore_hip_plan *plan, double *ms, int *count);
