@@ -107,8 +107,8 @@ def main():
     status = plan.status()
     assert np.all(status == 0), "device status errors: %s" % np.unique(status)
     sw = plan.sweeps()
-    log("Jacobi sweeps: mean %.1f max %d (K>142: %s)" % (sw.mean(), sw.max(),
-                                                        sorted(sw[K > 142].tolist())))
+    lane_of = plan.lanes()
+    log("Jacobi sweeps: mean %.1f max %d" % (sw.mean(), sw.max()))
     # numpy check of a sample (largest stacked ranks first): the product within
     # 10*acc of the dense update, the rank by the truncation rule on numpy's SVD
     order = np.argsort(-K, kind="stable")
@@ -151,10 +151,6 @@ def main():
     panel_bytes = sum(8 * 2 * nb * u.shape[1] for (u, v) in step.panel.values())
     b_unique = int(np.sum(8 * nb * (2 * rc.astype(np.int64) + 2 * r))) + panel_bytes + \
         ns * 8 * 2 * nb * nb
-    # per phase algorithmic work
-    phase_flops = {"geqrf": f_qr, "reassembly": f_orm, "F=AV^T BV": f_F, "G=BU F^T": f_G,
-                   "core_svd": int(np.sum(22 * Kl ** 3)),
-                   "syrk_CD": int(np.sum(2 * nb * nb * ks))}
 
     # ---- timed region: eager execution, HIP events around every launch ----------
     plan.set_timing(True)
@@ -175,15 +171,29 @@ def main():
     hc.device_synchronize()
     total_ms = e0.elapsed_ms(e1)
     ms1, c1 = plan.phase_times()
-    phase_ms = (ms1 - ms0) / np.maximum(c1 - c0, 1)
+    phase_ms = (ms1 - ms0) / np.maximum(c1 - c0, 1)   # (lane, phase) average per launch
     ms_per_step = total_ms / args.steps
     updates = n + ns
     value = updates / (ms_per_step / 1e3)
-    for name, t in zip(PHASES, phase_ms):
-        log("  phase %-16s %8.3f ms" % (name, t))
+    lane_names = ["lane0", "lane1", "lane2", "syrk"]
+    for l in range(3):
+        m = lane_of == l
+        if m.any():
+            lane_names[l] = "K%d-%d" % (K[m].min(), K[m].max())
+    for l in range(4):
+        row = " ".join("%s %.3f" % (PHASES[i], phase_ms[l, i]) for i in range(12) if c1[l, i] > c0[l, i])
+        if row:
+            log("  lane %-10s %s" % (lane_names[l], row))
 
-    # ---- the same batch as a hipGraph -------------------------------------------
+    # ---- eager without per-launch events, and the same batch as a hipGraph -----
     plan.set_timing(False)
+    q0, q1 = hc.Event(), hc.Event()
+    q0.record(stream)
+    for _ in range(args.steps):
+        plan.execute(stream)
+    q1.record(stream)
+    stream.synchronize()
+    eager_plain_ms = q0.elapsed_ms(q1) / args.steps
     plan.execute(stream, graph=True)
     stream.synchronize()
     g0, g1 = hc.Event(), hc.Event()
@@ -193,23 +203,35 @@ def main():
     g1.record(stream)
     stream.synchronize()
     graph_ms = g0.elapsed_ms(g1) / args.steps
-    log("eager %.3f ms/step, graph %.3f ms/step" % (ms_per_step, graph_ms))
+    log("eager+events %.3f ms/step, eager %.3f ms/step, graph %.3f ms/step" %
+        (ms_per_step, eager_plain_ms, graph_ms))
 
-    # ---- dominant kernel roofline -------------------------------------------------
-    dom = int(np.argmax(phase_ms))
-    dname = PHASES[dom]
+    # ---- dominant kernel roofline ------------------------------------------------
+    # the (lane, kernel) with the largest average launch duration; its
+    # algorithmic work is that of the updates in the lane
+    lane_of = plan.lanes()
+    dl, dp = np.unravel_index(int(np.argmax(phase_ms)), phase_ms.shape)
+    dname = PHASES[dp]
+    sel = lane_of == dl
+    Ks, rs = Kl[sel], r[sel]
     if dname == "syrk_CD":
         traffic_alg = ns * (16 * nb * nb + 8 * 2 * nb * int(ks.mean()))
-        ach = traffic_alg / (phase_ms[dom] / 1e3)
+        ach = traffic_alg / (phase_ms[dl, dp] / 1e3)
         roof = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": ach / HBM_PEAK, "traffic": None}
     else:
-        fl = phase_flops.get(dname, 0)
-        ach = fl / (phase_ms[dom] / 1e3)
+        fl = {"geqrf": sum(2 * q_flops(nb, int(k)) for k in Ks),
+              "core_svd": int(np.sum(22 * Ks ** 3)),
+              "reassembly": sum(fc('o', nb, int(a), int(k), 1) + fc('o', int(a), nb, int(k), 2)
+                                for a, k in zip(rs, Ks)),
+              "F=AV^T BV": int(np.sum(2 * ral[sel] * rbl[sel] * nb)),
+              "G=BU F^T": int(np.sum(2 * nb * ral[sel] * rbl[sel]))}.get(dname, 0)
+        ach = fl / (phase_ms[dl, dp] / 1e3)
         roof = {"bound": "mfma", "achieved": ach / 1e12, "peak": FP64_PEAK / 1e12,
                 "unit": "TFLOP/s", "frac": ach / FP64_PEAK, "traffic": None}
     roof["kernel"] = dname
-    roof["avg_launch_ms"] = float(phase_ms[dom])
+    roof["lane"] = lane_names[dl]
+    roof["avg_launch_ms"] = float(phase_ms[dl, dp])
 
     # ---- CPU baseline (oracle on the host cores) ------------------------------------
     cpu = None
@@ -272,7 +294,8 @@ def main():
                                   b_unique / (ms_per_step / 1e3) / HBM_PEAK),
         "flops_per_step": F_total,
         "unique_bytes_per_step": b_unique,
-        "phase_ms": {k: float(v) for k, v in zip(PHASES, phase_ms)},
+        "phase_ms": {lane_names[l]: {PHASES[i]: float(phase_ms[l, i]) for i in range(12)
+                                     if c1[l, i] > c0[l, i]} for l in range(4)},
         "check": {"updates": len(sample), "worst_rel_err": worst, "max_rank_drift": rank_drift},
         "device": devname,
     }

@@ -87,6 +87,7 @@ def lib():
         L.HCORE_hip_plan_get_status.argtypes = [_vp, _ip]
         L.HCORE_hip_plan_get_sigma.argtypes = [_vp, i, _dp, _ip]
         L.HCORE_hip_plan_get_sweeps.argtypes = [_vp, _ip]
+        L.HCORE_hip_plan_get_lanes.argtypes = [_vp, _ip]
         L.HCORE_hip_plan_workspace_bytes.argtypes = [_vp]
         L.HCORE_hip_plan_workspace_bytes.restype = C.c_longlong
         L.HCORE_hip_plan_flops.argtypes = [_vp, C.POINTER(C.c_ulong), C.POINTER(C.c_ulong)]
@@ -371,6 +372,11 @@ class Plan:
             raise RuntimeError("get_status failed: %d" % st)
         return out[:self.n]
 
+    def lanes(self):
+        out = np.zeros(max(self.n, 1), dtype=np.int32)
+        _check(self.L.HCORE_hip_plan_get_lanes(self.h, out.ctypes.data_as(_ip)), "get_lanes")
+        return out[:self.n]
+
     def sweeps(self):
         out = np.zeros(max(self.n, 1), dtype=np.int32)
         _check(self.L.HCORE_hip_plan_get_sweeps(self.h, out.ctypes.data_as(_ip)), "get_sweeps")
@@ -390,11 +396,12 @@ class Plan:
         _check(self.L.HCORE_hip_plan_set_timing(self.h, int(enable)), "set_timing")
 
     def phase_times(self):
-        ms = np.zeros(12)
-        cnt = np.zeros(12, dtype=np.int32)
+        """(ms, count) arrays of shape (lanes=4, phases=12), accumulated."""
+        ms = np.zeros(4 * 12)
+        cnt = np.zeros(4 * 12, dtype=np.int32)
         _check(self.L.HCORE_hip_plan_phase_times(self.h, ms.ctypes.data_as(_dp),
                                                  cnt.ctypes.data_as(_ip)), "phase_times")
-        return ms, cnt
+        return ms.reshape(4, 12), cnt.reshape(4, 12)
 
     def workspace_bytes(self):
         return int(self.L.HCORE_hip_plan_workspace_bytes(self.h))

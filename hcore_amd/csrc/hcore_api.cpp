@@ -177,22 +177,27 @@ struct hcore_hip_plan {
   int* d_status = nullptr;
   std::vector<size_t> sigma_off;  // per update, element offset into ws
   char* ws = nullptr;
-  // descriptor offsets and counts
-  size_t o_stack = 0, o_gp = 0, o_tF = 0, o_tG = 0, o_tT = 0, o_tZ = 0, o_qr = 0, o_apply = 0,
-         o_sp = 0, o_tW = 0, o_tX = 0, o_tC = 0;
-  int n_tF = 0, n_tG = 0, n_tT = 0, n_tZ = 0, n_qr = 0, n_apply = 0, n_tW = 0, n_tX = 0,
-      n_tC = 0;
+  // the batch is split into lanes by stacked rank; each lane runs its whole
+  // pipeline on its own stream so the long large-K chains overlap the rest
   struct CoreLaunch {
     size_t off;
     int count, variant, kmax;
   };
-  std::vector<CoreLaunch> core;  // one launch per K bucket
+  struct Lane {
+    int n = 0;
+    size_t o_stack = 0, o_tF = 0, o_tG = 0, o_tT = 0, o_tZ = 0, o_qr = 0, o_apply = 0;
+    int n_tF = 0, n_tG = 0, n_tT = 0, n_tZ = 0, n_qr = 0, n_apply = 0;
+    std::vector<CoreLaunch> core;  // one launch per K bucket
+    hipStream_t stream = nullptr;  // lanes > 0 (lane 0 uses the caller's stream)
+    hipEvent_t ev_done = nullptr;
+  };
+  std::vector<Lane> lanes;
+  std::vector<int> lane_of;  // lane index of each update (HCORE_hip_plan_get_lanes)
+  size_t o_gp = 0, o_sp = 0, o_tW = 0, o_tX = 0, o_tC = 0;
+  int n_tW = 0, n_tX = 0, n_tC = 0;
+  hipStream_t syrk_stream = nullptr;
+  hipEvent_t ev_start = nullptr, ev_syrk = nullptr;
   int* d_sweeps = nullptr;
-  // fork/join of the concurrent core buckets
-  static constexpr int kMaxAux = 8;
-  hipStream_t aux[kMaxAux] = {};
-  hipEvent_t ev_fork = nullptr;
-  hipEvent_t ev_join[kMaxAux] = {};
   // graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -200,14 +205,15 @@ struct hcore_hip_plan {
   std::vector<int> last_ranks;
   // per-phase timing: a ring of event sets so timed executions do not wait
   // for each other; a set is read back when the ring wraps or on request
-  static constexpr int kRing = 64;
+  static constexpr int kRing = 16;
+  static constexpr int kMaxLanes = 4;  // 3 gemm lanes + the syrk stream
   bool timing = false;
-  hipEvent_t ev[kRing][HCORE_HIP_NPHASES][2] = {};
-  bool used[kRing][HCORE_HIP_NPHASES] = {};
+  hipEvent_t ev[kRing][kMaxLanes][HCORE_HIP_NPHASES][2] = {};
+  bool used[kRing][kMaxLanes][HCORE_HIP_NPHASES] = {};
   bool pending[kRing] = {};
   int slot = 0;
-  double phase_ms[HCORE_HIP_NPHASES] = {};
-  int phase_count[HCORE_HIP_NPHASES] = {};
+  double phase_ms[kMaxLanes][HCORE_HIP_NPHASES] = {};
+  int phase_count[kMaxLanes][HCORE_HIP_NPHASES] = {};
 };
 
 namespace hcore {
@@ -220,14 +226,55 @@ static int plan_finalize_t(hcore_hip_plan* p) {
   const size_t es = sizeof(T);
   auto al = [](size_t x) { return (x + 63) & ~size_t(63); };  // elements, 512 B for FP64
 
-  // ---- workspace layout (elements of T) ----
+  // ---- stacked ranks, processing order, lanes and core variants ----
   std::vector<int> Kv(n);
+  for (int u = 0; u < n; ++u) Kv[u] = p->ra[u] + p->rc[u];
+  // processing order: largest stacked rank first (longest jobs start first)
+  std::vector<int> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return Kv[a] > Kv[b]; });
+
+  // lanes: K > KL (core in global memory) | 97..KL | <= 96; empty lanes dropped
+  const int lane_lo[3] = {KL + 1, 97, 0};
+  std::vector<std::vector<int>> lane_members(3);
+  for (int idx = 0; idx < n; ++idx) {
+    const int u = order[idx];
+    for (int l = 0; l < 3; ++l)
+      if (Kv[u] >= lane_lo[l]) { lane_members[l].push_back(u); break; }
+  }
+  if (n <= 64) {  // small batches: one lane
+    lane_members[2].clear();
+    for (int idx = 0; idx < n; ++idx) lane_members[2].push_back(order[idx]);
+    lane_members[0].clear();
+    lane_members[1].clear();
+  }
+  // core SVD variant per update: the heavy lanes use one variant for all their
+  // tiles (one launch), the bulk lane buckets by K
+  std::vector<int> variant(n);
+  for (int l = 0; l < 3; ++l) {
+    int kmax = 0;
+    for (int u : lane_members[l]) kmax = std::max(kmax, Kv[u]);
+    for (int u : lane_members[l])
+      variant[u] = core_variant_for(l < 2 && n > 64 ? kmax : Kv[u], KL);
+  }
+  auto global_core = [](int v) { return v == CORE_K192 || v == CORE_K256 || v == CORE_K512; };
+  // lane index as executed (empty lanes are dropped, so renumber)
+  p->lane_of.assign(n, 0);
+  {
+    int li = 0;
+    for (int l = 0; l < 3; ++l) {
+      if (lane_members[l].empty()) continue;
+      for (int u : lane_members[l]) p->lane_of[u] = li;
+      ++li;
+    }
+  }
+
+  // ---- workspace layout (elements of T) ----
   std::vector<size_t> o_Us(n), o_Vs(n), o_tA(n), o_tB(n), o_TA(n), o_TB(n), o_F(n), o_Tc(n),
       o_Tj(n), o_Zu(n), o_Zv(n), o_sg(n);
   size_t off = 0;
   for (int u = 0; u < n; ++u) {
-    const int K = p->ra[u] + p->rc[u];
-    Kv[u] = K;
+    const int K = Kv[u];
     o_Us[u] = off; off += al((size_t)M * K);
     o_Vs[u] = off; off += al((size_t)M * K);
     o_tA[u] = off; off += al(K);
@@ -236,7 +283,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
     o_TB[u] = off; off += al(tp_elems(K));
     o_F[u] = off; off += al((size_t)p->ra[u] * p->rb[u]);
     o_Tc[u] = off; off += al((size_t)K * K);
-    o_Tj[u] = off; off += (K > KL) ? al((size_t)K * K) : 0;
+    o_Tj[u] = off; off += global_core(variant[u]) ? al((size_t)K * K) : 0;
     o_Zu[u] = off; off += al((size_t)K * K);
     o_Zv[u] = off; off += al((size_t)K * K);
     o_sg[u] = off; off += al(K);
@@ -250,11 +297,6 @@ static int plan_finalize_t(hcore_hip_plan* p) {
   p->sigma_off = o_sg;
   const size_t ws_elems = off;
 
-  // processing order: largest stacked rank first (longest jobs start first)
-  std::vector<int> order(n);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return Kv[a] > Kv[b]; });
-
   // ---- device memory: [rank | status | sweeps | workspace | descriptors] ----
   Packer pk;
   int* d_sweeps = nullptr;
@@ -263,63 +305,72 @@ static int plan_finalize_t(hcore_hip_plan* p) {
   auto build = [&](char* wsbase, int* d_rank, int* d_status) {
     T* W = reinterpret_cast<T*>(wsbase);
     pk.buf.clear();
-    std::vector<StackProb<T>> stack(n);
     std::vector<GemmProb<T>> gp;
-    std::vector<GemmTask> tF, tG, tT, tZ, tW, tX, tC;
-    std::vector<QrProb<T>> qr;
-    // K buckets of the core SVD, one kernel variant each
+    struct LaneDesc {
+      std::vector<StackProb<T>> stack;
+      std::vector<GemmTask> tF, tG, tT, tZ;
+      std::vector<QrProb<T>> qr;
+      std::vector<std::vector<CoreProb<T>>> core_b;
+      std::vector<int> core_kmax;
+      std::vector<ApplyProb<T>> ap;
+    };
     constexpr int NV = CORE_K512 + 1;
-    std::vector<std::vector<CoreProb<T>>> core_b(NV);
-    std::vector<int> core_kmax(NV, 1);
-    std::vector<ApplyProb<T>> ap;
-    for (int idx = 0; idx < n; ++idx) {
-      const int u = order[idx];
-      const int K = Kv[u], ra = p->ra[u], rb = p->rb[u], rc = p->rc[u];
-      T* Us = W + o_Us[u];
-      T* Vs = W + o_Vs[u];
-      stack[idx] = {(const T*)p->CU[u], (const T*)p->CV[u], (const T*)p->AU[u], Us, Vs, rc, ra,
-                    p->ldc, p->lda};
-      // F = AV^T BV   (hcore_dgemm.c:175-178)
-      GemmProb<T> g{};
-      g.A = (const T*)p->AV[u]; g.B = (const T*)p->BV[u]; g.C = W + o_F[u];
-      g.m = ra; g.n = rb; g.k = M; g.lda = p->lda; g.ldb = p->ldb; g.ldc = ra;
-      g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_BETA_ZERO;
-      add_tasks(tF, (int)gp.size(), g.m, g.n); gp.push_back(g);
-      // G = BU F^T into Vs(:, rc:K)   (hcore_dgemm.c:199-205)
-      g = GemmProb<T>{};
-      g.A = (const T*)p->BU[u]; g.B = W + o_F[u]; g.C = Vs + (size_t)rc * M;
-      g.m = M; g.n = ra; g.k = rb; g.lda = p->ldb; g.ldb = ra; g.ldc = M;
-      g.alpha = T(1); g.beta = T(0); g.flags = GF_B_TRANS | GF_BETA_ZERO;
-      add_tasks(tG, (int)gp.size(), g.m, g.n); gp.push_back(g);
-      // T = R_A R_B^T   (hcore_dgemm.c:384-386)
-      g = GemmProb<T>{};
-      g.A = Us; g.B = Vs; g.C = W + o_Tc[u];
-      g.m = K; g.n = K; g.k = K; g.lda = M; g.ldb = M; g.ldc = K;
-      g.alpha = T(1); g.beta = T(0);
-      g.flags = GF_A_UPPER | GF_B_TRANS | GF_B_UPPER | GF_BETA_ZERO;
-      add_tasks(tT, (int)gp.size(), g.m, g.n); gp.push_back(g);
-      // Zv = T^T Zu = V_r S_r
-      g = GemmProb<T>{};
-      g.A = W + o_Tc[u]; g.B = W + o_Zu[u]; g.C = W + o_Zv[u];
-      g.m = K; g.n = K; g.k = K; g.lda = K; g.ldb = K; g.ldc = K;
-      g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_BETA_ZERO;
-      g.n_dev = d_rank + u; g.skip = d_status + u;
-      add_tasks(tZ, (int)gp.size(), g.m, g.n); gp.push_back(g);
-      // QR of both stacks
-      qr.push_back({Us, W + o_tA[u], W + o_TA[u], K});
-      qr.push_back({Vs, W + o_tB[u], W + o_TB[u], K});
-      CoreProb<T> c{W + o_Tc[u], (K > KL) ? W + o_Tj[u] : nullptr, W + o_Zu[u], W + o_sg[u],
-                    d_rank + u, d_status + u, d_sweeps + u, K};
-      const int b = core_variant_for(K, KL);
-      core_b[b].push_back(c);
-      core_kmax[b] = std::max(core_kmax[b], K);
-      T* CUo = (T*)(p->CUo[u] ? p->CUo[u] : p->CU[u]);
-      T* CVo = (T*)(p->CVo[u] ? p->CVo[u] : p->CV[u]);
-      ap.push_back({Us, W + o_TA[u], W + o_Zu[u], CUo, d_rank + u, d_status + u, K, p->ldco});
-      ap.push_back({Vs, W + o_TB[u], W + o_Zv[u], CVo, d_rank + u, d_status + u, K, p->ldco});
+    std::vector<LaneDesc> ld;
+    for (int l = 0; l < 3; ++l) {
+      if (lane_members[l].empty()) continue;
+      LaneDesc d;
+      d.core_b.resize(NV);
+      d.core_kmax.assign(NV, 1);
+      for (int u : lane_members[l]) {
+        const int K = Kv[u], ra = p->ra[u], rb = p->rb[u], rc = p->rc[u];
+        T* Us = W + o_Us[u];
+        T* Vs = W + o_Vs[u];
+        d.stack.push_back({(const T*)p->CU[u], (const T*)p->CV[u], (const T*)p->AU[u], Us, Vs,
+                           rc, ra, p->ldc, p->lda});
+        // F = AV^T BV   (hcore_dgemm.c:175-178)
+        GemmProb<T> g{};
+        g.A = (const T*)p->AV[u]; g.B = (const T*)p->BV[u]; g.C = W + o_F[u];
+        g.m = ra; g.n = rb; g.k = M; g.lda = p->lda; g.ldb = p->ldb; g.ldc = ra;
+        g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_BETA_ZERO;
+        add_tasks(d.tF, (int)gp.size(), g.m, g.n); gp.push_back(g);
+        // G = BU F^T into Vs(:, rc:K)   (hcore_dgemm.c:199-205)
+        g = GemmProb<T>{};
+        g.A = (const T*)p->BU[u]; g.B = W + o_F[u]; g.C = Vs + (size_t)rc * M;
+        g.m = M; g.n = ra; g.k = rb; g.lda = p->ldb; g.ldb = ra; g.ldc = M;
+        g.alpha = T(1); g.beta = T(0); g.flags = GF_B_TRANS | GF_BETA_ZERO;
+        add_tasks(d.tG, (int)gp.size(), g.m, g.n); gp.push_back(g);
+        // T = R_A R_B^T   (hcore_dgemm.c:384-386)
+        g = GemmProb<T>{};
+        g.A = Us; g.B = Vs; g.C = W + o_Tc[u];
+        g.m = K; g.n = K; g.k = K; g.lda = M; g.ldb = M; g.ldc = K;
+        g.alpha = T(1); g.beta = T(0);
+        g.flags = GF_A_UPPER | GF_B_TRANS | GF_B_UPPER | GF_BETA_ZERO;
+        add_tasks(d.tT, (int)gp.size(), g.m, g.n); gp.push_back(g);
+        // Zv = T^T Zu = V_r S_r
+        g = GemmProb<T>{};
+        g.A = W + o_Tc[u]; g.B = W + o_Zu[u]; g.C = W + o_Zv[u];
+        g.m = K; g.n = K; g.k = K; g.lda = K; g.ldb = K; g.ldc = K;
+        g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_BETA_ZERO;
+        g.n_dev = d_rank + u; g.skip = d_status + u;
+        add_tasks(d.tZ, (int)gp.size(), g.m, g.n); gp.push_back(g);
+        // QR of both stacks
+        d.qr.push_back({Us, W + o_tA[u], W + o_TA[u], K});
+        d.qr.push_back({Vs, W + o_tB[u], W + o_TB[u], K});
+        CoreProb<T> c{W + o_Tc[u], global_core(variant[u]) ? W + o_Tj[u] : nullptr, W + o_Zu[u],
+                      W + o_sg[u], d_rank + u, d_status + u, d_sweeps + u, K};
+        const int b = variant[u];
+        d.core_b[b].push_back(c);
+        d.core_kmax[b] = std::max(d.core_kmax[b], K);
+        T* CUo = (T*)(p->CUo[u] ? p->CUo[u] : p->CU[u]);
+        T* CVo = (T*)(p->CVo[u] ? p->CVo[u] : p->CV[u]);
+        d.ap.push_back({Us, W + o_TA[u], W + o_Zu[u], CUo, d_rank + u, d_status + u, K, p->ldco});
+        d.ap.push_back({Vs, W + o_TB[u], W + o_Zv[u], CVo, d_rank + u, d_status + u, K, p->ldco});
+      }
+      ld.push_back(std::move(d));
     }
     // syrk (hcore_dsyrk.c:103, 144, 165-181)
     std::vector<GemmProb<T>> sp;
+    std::vector<GemmTask> tW, tX, tC;
     for (int s = 0; s < p->ns; ++s) {
       const int K = p->sK[s];
       GemmProb<T> g{};
@@ -338,19 +389,25 @@ static int plan_finalize_t(hcore_hip_plan* p) {
       g.alpha = T(p->salpha); g.beta = T(1); g.flags = GF_B_TRANS | GF_LOWER_ZERO_UPPER;
       add_tasks(tC, (int)sp.size(), g.m, g.n); sp.push_back(g);
     }
-    p->o_stack = pk.add(stack);
     p->o_gp = pk.add(gp);
-    p->o_tF = pk.add(tF); p->n_tF = (int)tF.size();
-    p->o_tG = pk.add(tG); p->n_tG = (int)tG.size();
-    p->o_tT = pk.add(tT); p->n_tT = (int)tT.size();
-    p->o_tZ = pk.add(tZ); p->n_tZ = (int)tZ.size();
-    p->o_qr = pk.add(qr); p->n_qr = (int)qr.size();
-    p->core.clear();
-    for (int b = NV - 1; b >= 0; --b) {  // largest K first: the longest jobs start first
-      if (core_b[b].empty()) continue;
-      p->core.push_back({pk.add(core_b[b]), (int)core_b[b].size(), b, core_kmax[b]});
+    if (p->lanes.size() != ld.size()) p->lanes.resize(ld.size());
+    for (size_t l = 0; l < ld.size(); ++l) {
+      auto& L = p->lanes[l];
+      auto& d = ld[l];
+      L.n = (int)d.stack.size();
+      L.o_stack = pk.add(d.stack);
+      L.o_tF = pk.add(d.tF); L.n_tF = (int)d.tF.size();
+      L.o_tG = pk.add(d.tG); L.n_tG = (int)d.tG.size();
+      L.o_tT = pk.add(d.tT); L.n_tT = (int)d.tT.size();
+      L.o_tZ = pk.add(d.tZ); L.n_tZ = (int)d.tZ.size();
+      L.o_qr = pk.add(d.qr); L.n_qr = (int)d.qr.size();
+      L.o_apply = pk.add(d.ap); L.n_apply = (int)d.ap.size();
+      L.core.clear();
+      for (int b = NV - 1; b >= 0; --b) {  // largest K first: the longest jobs start first
+        if (d.core_b[b].empty()) continue;
+        L.core.push_back({pk.add(d.core_b[b]), (int)d.core_b[b].size(), b, d.core_kmax[b]});
+      }
     }
-    p->o_apply = pk.add(ap); p->n_apply = (int)ap.size();
     p->o_sp = pk.add(sp);
     p->o_tW = pk.add(tW); p->n_tW = (int)tW.size();
     p->o_tX = pk.add(tX); p->n_tX = (int)tX.size();
@@ -387,13 +444,14 @@ static int plan_finalize_t(hcore_hip_plan* p) {
                         g_ctx.stream));
   HCHECK(hipMemsetAsync(base, 0, head, g_ctx.stream));
   HCHECK(hipStreamSynchronize(g_ctx.stream));
-  if (p->core.size() > 1 && !p->ev_fork) {
-    HCHECK(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
-    for (int i = 1; i < (int)p->core.size() && i < hcore_hip_plan::kMaxAux; ++i) {
-      HCHECK(hipStreamCreateWithFlags(&p->aux[i], hipStreamNonBlocking));
-      HCHECK(hipEventCreateWithFlags(&p->ev_join[i], hipEventDisableTiming));
-    }
+  for (size_t l = 0; l < p->lanes.size(); ++l) {
+    auto& L = p->lanes[l];
+    if (l > 0) HCHECK(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+    HCHECK(hipEventCreateWithFlags(&L.ev_done, hipEventDisableTiming));
   }
+  HCHECK(hipEventCreateWithFlags(&p->ev_start, hipEventDisableTiming));
+  HCHECK(hipEventCreateWithFlags(&p->ev_syrk, hipEventDisableTiming));
+  if (p->ns > 0) HCHECK(hipStreamCreateWithFlags(&p->syrk_stream, hipStreamNonBlocking));
   p->finalized = true;
   return HCORE_HIP_OK;
 }
@@ -401,15 +459,16 @@ static int plan_finalize_t(hcore_hip_plan* p) {
 // Accumulate the phase times of one pending timed execution.
 static int collect_slot(hcore_hip_plan* p, int k) {
   if (!p->pending[k]) return HCORE_HIP_OK;
-  for (int i = 0; i < HCORE_HIP_NPHASES; ++i) {
-    if (!p->used[k][i]) continue;
-    HCHECK(hipEventSynchronize(p->ev[k][i][1]));
-    float ms = 0.f;
-    HCHECK(hipEventElapsedTime(&ms, p->ev[k][i][0], p->ev[k][i][1]));
-    p->phase_ms[i] += ms;
-    p->phase_count[i] += 1;
-    p->used[k][i] = false;
-  }
+  for (int l = 0; l < hcore_hip_plan::kMaxLanes; ++l)
+    for (int i = 0; i < HCORE_HIP_NPHASES; ++i) {
+      if (!p->used[k][l][i]) continue;
+      HCHECK(hipEventSynchronize(p->ev[k][l][i][1]));
+      float ms = 0.f;
+      HCHECK(hipEventElapsedTime(&ms, p->ev[k][l][i][0], p->ev[k][l][i][1]));
+      p->phase_ms[l][i] += ms;
+      p->phase_count[l][i] += 1;
+      p->used[k][l][i] = false;
+    }
   p->pending[k] = false;
   return HCORE_HIP_OK;
 }
@@ -434,51 +493,57 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
     if (st) return st;
     p->slot = (k + 1) % hcore_hip_plan::kRing;
   }
-  auto phase = [&](int id, bool run, auto&& fn) {
+  // phase `id` of lane `l` on stream `st`, bracketed by events when timing
+  auto phase = [&](int l, int id, hipStream_t st, bool run, auto&& fn) {
     if (!run) return;
-    if (tm) (void)hipEventRecord(p->ev[k][id][0], s);
+    if (tm) (void)hipEventRecord(p->ev[k][l][id][0], st);
     fn();
     if (tm) {
-      (void)hipEventRecord(p->ev[k][id][1], s);
-      p->used[k][id] = true;
+      (void)hipEventRecord(p->ev[k][l][id][1], st);
+      p->used[k][l][id] = true;
     }
   };
-  if (p->ng > 0) {
-    phase(0, true, [&] {
-      launch_stack<T>(reinterpret_cast<const StackProb<T>*>(D(p->o_stack)), p->ng, M,
-                      T(p->alpha), T(p->beta), s);
+  const int nl = (int)p->lanes.size();
+  (void)hipEventRecord(p->ev_start, s);
+  for (int l = 0; l < nl; ++l) {
+    auto& L = p->lanes[l];
+    hipStream_t ls = l == 0 ? s : L.stream;
+    if (l > 0) (void)hipStreamWaitEvent(ls, p->ev_start, 0);
+    phase(l, 0, ls, true, [&] {
+      launch_stack<T>(reinterpret_cast<const StackProb<T>*>(D(L.o_stack)), L.n, M, T(p->alpha),
+                      T(p->beta), ls);
     });
-    phase(1, true, [&] { launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(p->o_tF)), p->n_tF, s); });
-    phase(2, true, [&] { launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(p->o_tG)), p->n_tG, s); });
-    phase(3, true, [&] {
-      launch_geqrf<T>(reinterpret_cast<const QrProb<T>*>(D(p->o_qr)), p->n_qr, M, s);
+    phase(l, 1, ls, true, [&] { launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(L.o_tF)), L.n_tF, ls); });
+    phase(l, 2, ls, true, [&] { launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(L.o_tG)), L.n_tG, ls); });
+    phase(l, 3, ls, true, [&] {
+      launch_geqrf<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qr)), L.n_qr, M, ls);
     });
-    phase(4, true, [&] { launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(p->o_tT)), p->n_tT, s); });
-    // the K buckets run concurrently: fork onto auxiliary streams, join back
-    phase(5, !p->core.empty(), [&] {
-      const int nbk = (int)p->core.size();
-      if (nbk > 1) (void)hipEventRecord(p->ev_fork, s);
-      for (int i = 0; i < nbk; ++i) {
-        const auto& c = p->core[i];
-        hipStream_t sb = i == 0 ? s : p->aux[i];
-        if (i > 0) (void)hipStreamWaitEvent(sb, p->ev_fork, 0);
+    phase(l, 4, ls, true, [&] { launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(L.o_tT)), L.n_tT, ls); });
+    // the lane's K buckets, largest first (each bucket a kernel variant with its
+    // own LDS size); at most 4 streams per plan, one per hardware queue
+    phase(l, 5, ls, !L.core.empty(), [&] {
+      for (const auto& c : L.core)
         launch_core<T>(reinterpret_cast<const CoreProb<T>*>(D(c.off)), c.count, c.variant,
-                       c.kmax, p->rk, p->maxrk, T(p->acc), sb);
-        if (i > 0) (void)hipEventRecord(p->ev_join[i], sb);
-      }
-      for (int i = 1; i < nbk; ++i) (void)hipStreamWaitEvent(s, p->ev_join[i], 0);
+                       c.kmax, p->rk, p->maxrk, T(p->acc), ls);
     });
-    phase(7, true, [&] { launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(p->o_tZ)), p->n_tZ, s); });
-    phase(8, true, [&] {
-      launch_apply<T>(reinterpret_cast<const ApplyProb<T>*>(D(p->o_apply)), p->n_apply, M, s);
+    phase(l, 7, ls, true, [&] { launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(L.o_tZ)), L.n_tZ, ls); });
+    phase(l, 8, ls, true, [&] {
+      launch_apply<T>(reinterpret_cast<const ApplyProb<T>*>(D(L.o_apply)), L.n_apply, M, ls);
     });
+    if (l > 0) (void)hipEventRecord(L.ev_done, ls);
   }
   if (p->ns > 0) {
+    hipStream_t ss = p->syrk_stream;
+    (void)hipStreamWaitEvent(ss, p->ev_start, 0);
     const auto* sp = reinterpret_cast<const GemmProb<T>*>(D(p->o_sp));
-    phase(9, true, [&] { launch_gemm<T>(sp, reinterpret_cast<const GemmTask*>(D(p->o_tW)), p->n_tW, s); });
-    phase(10, true, [&] { launch_gemm<T>(sp, reinterpret_cast<const GemmTask*>(D(p->o_tX)), p->n_tX, s); });
-    phase(11, true, [&] { launch_gemm<T>(sp, reinterpret_cast<const GemmTask*>(D(p->o_tC)), p->n_tC, s); });
+    const int ls = hcore_hip_plan::kMaxLanes - 1;
+    phase(ls, 9, ss, true, [&] { launch_gemm<T>(sp, reinterpret_cast<const GemmTask*>(D(p->o_tW)), p->n_tW, ss); });
+    phase(ls, 10, ss, true, [&] { launch_gemm<T>(sp, reinterpret_cast<const GemmTask*>(D(p->o_tX)), p->n_tX, ss); });
+    phase(ls, 11, ss, true, [&] { launch_gemm<T>(sp, reinterpret_cast<const GemmTask*>(D(p->o_tC)), p->n_tC, ss); });
+    (void)hipEventRecord(p->ev_syrk, ss);
+    (void)hipStreamWaitEvent(s, p->ev_syrk, 0);
   }
+  for (int l = 1; l < nl; ++l) (void)hipStreamWaitEvent(s, p->lanes[l].ev_done, 0);
   if (tm) p->pending[k] = true;
   HCHECK(hipGetLastError());
   return HCORE_HIP_OK;
@@ -918,6 +983,12 @@ int HCORE_hip_plan_get_status(hcore_hip_plan* p, int* out) {
   return HCORE_HIP_OK;
 }
 
+int HCORE_hip_plan_get_lanes(hcore_hip_plan* p, int* out) {
+  if (!p || !p->finalized || !out) return HCORE_HIP_ERR_ARG;
+  std::copy(p->lane_of.begin(), p->lane_of.end(), out);
+  return HCORE_HIP_OK;
+}
+
 int HCORE_hip_plan_get_sweeps(hcore_hip_plan* p, int* out) {
   if (!p || !p->finalized || !out) return HCORE_HIP_ERR_ARG;
   HCHECK(hipDeviceSynchronize());
@@ -971,14 +1042,17 @@ int HCORE_hip_plan_flops(hcore_hip_plan* p, unsigned long* update, unsigned long
 void HCORE_hip_plan_destroy(hcore_hip_plan* p) {
   if (!p) return;
   for (int k = 0; k < hcore_hip_plan::kRing; ++k)
-    for (int i = 0; i < HCORE_HIP_NPHASES; ++i)
-      for (int j = 0; j < 2; ++j)
-        if (p->ev[k][i][j]) (void)hipEventDestroy(p->ev[k][i][j]);
-  for (int i = 0; i < hcore_hip_plan::kMaxAux; ++i) {
-    if (p->aux[i]) (void)hipStreamDestroy(p->aux[i]);
-    if (p->ev_join[i]) (void)hipEventDestroy(p->ev_join[i]);
+    for (int l = 0; l < hcore_hip_plan::kMaxLanes; ++l)
+      for (int i = 0; i < HCORE_HIP_NPHASES; ++i)
+        for (int j = 0; j < 2; ++j)
+          if (p->ev[k][l][i][j]) (void)hipEventDestroy(p->ev[k][l][i][j]);
+  for (auto& L : p->lanes) {
+    if (L.stream) (void)hipStreamDestroy(L.stream);
+    if (L.ev_done) (void)hipEventDestroy(L.ev_done);
   }
-  if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+  if (p->syrk_stream) (void)hipStreamDestroy(p->syrk_stream);
+  if (p->ev_start) (void)hipEventDestroy(p->ev_start);
+  if (p->ev_syrk) (void)hipEventDestroy(p->ev_syrk);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   if (p->dmem) {
@@ -990,10 +1064,11 @@ void HCORE_hip_plan_destroy(hcore_hip_plan* p) {
 
 int HCORE_hip_plan_set_timing(hcore_hip_plan* p, int enable) {
   if (!p) return HCORE_HIP_ERR_ARG;
-  if (enable && !p->ev[0][0][0])
+  if (enable && !p->ev[0][0][0][0])
     for (int k = 0; k < hcore_hip_plan::kRing; ++k)
-      for (int i = 0; i < HCORE_HIP_NPHASES; ++i)
-        for (int j = 0; j < 2; ++j) HCHECK(hipEventCreate(&p->ev[k][i][j]));
+      for (int l = 0; l < hcore_hip_plan::kMaxLanes; ++l)
+        for (int i = 0; i < HCORE_HIP_NPHASES; ++i)
+          for (int j = 0; j < 2; ++j) HCHECK(hipEventCreate(&p->ev[k][l][i][j]));
   p->timing = enable != 0;
   return HCORE_HIP_OK;
 }
@@ -1002,10 +1077,11 @@ int HCORE_hip_plan_phase_times(hcore_hip_plan* p, double* ms, int* count) {
   if (!p) return HCORE_HIP_ERR_ARG;
   int st = collect_all(p);
   if (st) return st;
-  for (int i = 0; i < HCORE_HIP_NPHASES; ++i) {
-    if (ms) ms[i] = p->phase_ms[i];
-    if (count) count[i] = p->phase_count[i];
-  }
+  for (int l = 0; l < hcore_hip_plan::kMaxLanes; ++l)
+    for (int i = 0; i < HCORE_HIP_NPHASES; ++i) {
+      if (ms) ms[l * HCORE_HIP_NPHASES + i] = p->phase_ms[l][i];
+      if (count) count[l * HCORE_HIP_NPHASES + i] = p->phase_count[l][i];
+    }
   return HCORE_HIP_OK;
 }
 

@@ -108,7 +108,7 @@ void launch_geqrf(const QrProb<T>* probs, int n, int nb, hipStream_t s);
 // core SVD variants by stacked rank K: one wavefront for K <= 64, one
 // workgroup with the core in LDS up to the LDS limit, 1024 threads with the
 // core in global memory beyond it
-enum : int { CORE_K16, CORE_K32, CORE_K64, CORE_K128, CORE_KLDS, CORE_K256, CORE_K512 };
+enum : int { CORE_K16, CORE_K32, CORE_K64, CORE_K128, CORE_KLDS, CORE_K192, CORE_K256, CORE_K512 };
 int core_variant_for(int K, int lds_max_k);
 constexpr int kCoreMaxK = 512;
 template <typename T>

@@ -219,10 +219,12 @@ void launch_stack(const StackProb<T>* probs, int n, int nb, T alpha, T beta, hip
 // Householder QR (LAPACK convention: H = I - tau v v^T, v(j) = 1, R(j,j) = beta)
 // ------------------------------------------------------------------------------
 // Panel factorization by one wavefront on the LDS panel Yp (PB columns of
-// NBP = 64*NBW rows).  Lane l owns rows l + 64 i.
+// NBP = 64*NBW rows).  Lane l owns rows l + 64 i.  Per column: one reduction
+// for the norm, one batched reduce-scatter for all remaining panel columns.
 template <typename T, int NBW, int PB>
-__device__ void panel_factor(T* Yp, T* tau_s, T* beta_s, int j0, int pb, int lane) {
+__device__ void panel_factor(T* Yp, T* tau_s, T* beta_s, T* wsc, int j0, int pb, int lane) {
   constexpr int NBP = 64 * NBW;
+  const T* Yl = Yp + lane;
   for (int c = 0; c < pb; ++c) {
     const int j = j0 + c;
     T* col = Yp + c * NBP;
@@ -241,27 +243,37 @@ __device__ void panel_factor(T* Yp, T* tau_s, T* beta_s, int j0, int pb, int lan
       beta = alpha >= T(0) ? -nrm : nrm;
       tau = (beta - alpha) / beta;
       const T scal = T(1) / (alpha - beta);
+      T v[NBW];
 #pragma unroll
       for (int i = 0; i < NBW; ++i) {
         const int row = lane + 64 * i;
-        if (row > j) col[row] *= scal;
+        const T x = col[row];
+        v[i] = row > j ? x * scal : (row == j ? T(1) : T(0));
+        if (row > j) col[row] = v[i];
       }
-      // apply H to the remaining panel columns
-      for (int k = c + 1; k < pb; ++k) {
-        T* ck = Yp + k * NBP;
-        T w = T(0);
+      // w_k = v^T Yp(:, k) for every remaining panel column, in one reduction
+      T w[PB];
 #pragma unroll
-        for (int i = 0; i < NBW; ++i) {
-          const int row = lane + 64 * i;
-          if (row > j) w = fma(col[row], ck[row], w);
-          else if (row == j) w += ck[row];
+      for (int k = 0; k < PB; ++k) {
+        T acc = T(0);
+        if (k > c && k < pb) {
+#pragma unroll
+          for (int i = 0; i < NBW; ++i) acc = fma(v[i], Yl[k * NBP + 64 * i], acc);
         }
-        w = wave_sum(w) * tau;
+        w[k] = acc;
+      }
+      reduce_scatter<PB>(w, lane);
+      if ((lane & ((64 / PB) - 1)) == 0) wsc[reduce_scatter_index<PB>(lane)] = w[0];
+      wave_sync();
 #pragma unroll
-        for (int i = 0; i < NBW; ++i) {
-          const int row = lane + 64 * i;
-          if (row > j) ck[row] = fma(-col[row], w, ck[row]);
-          else if (row == j) ck[row] -= w;
+      for (int k = 0; k < PB; ++k) {
+        if (k > c && k < pb) {
+          const T wk = wsc[k] * tau;
+#pragma unroll
+          for (int i = 0; i < NBW; ++i) {
+            T* p = Yp + k * NBP + lane + 64 * i;
+            *p = fma(-v[i], wk, *p);
+          }
         }
       }
     }
@@ -283,7 +295,8 @@ __global__ __launch_bounds__(kThreads) void k_geqrf(const QrProb<T>* __restrict_
   T* Sg = Tt + PB * PB;                    // PB x PB
   T* tau_s = Sg + PB * PB;                 // PB
   T* beta_s = tau_s + PB;                  // PB
-  T* scr = beta_s + PB;                    // 4 waves x 2 x NW
+  T* wsc = beta_s + PB;                    // PB (panel reduction exchange)
+  T* scr = wsc + PB;                       // 4 waves x 2 x NW
 
   const QrProb<T> P = probs[blockIdx.x];
   const int K = P.K;
@@ -307,7 +320,7 @@ __global__ __launch_bounds__(kThreads) void k_geqrf(const QrProb<T>* __restrict_
       beta_s[tid] = T(0);
     }
     __syncthreads();
-    if (wid == 0) panel_factor<T, NBW, PB>(Yp, tau_s, beta_s, j0, pb, lane);
+    if (wid == 0) panel_factor<T, NBW, PB>(Yp, tau_s, beta_s, wsc, j0, pb, lane);
     __syncthreads();
     // write the factored panel back: R above/on the diagonal, reflectors below
     #pragma unroll 1
@@ -328,20 +341,32 @@ __global__ __launch_bounds__(kThreads) void k_geqrf(const QrProb<T>* __restrict_
       else if (row == j0 + c) Yp[e] = T(1);
     }
     __syncthreads();
-    // Gram S(a,b) = v_a^T v_b, a < b, spread over the 4 wavefronts
+    // Gram S(a,b) = v_a^T v_b for the PB(PB-1)/2 pairs a < b: each wavefront
+    // takes a quarter of the pairs and reduces them together
     {
-#pragma unroll 1
-      for (int p = wid; p < PB * (PB - 1) / 2; p += 4) {
-          int b = 1, a = p;
-          while (a >= b) { a -= b; ++b; }
-          T s = T(0);
+      constexpr int NPAIR = PB * (PB - 1) / 2;
+      constexpr int PPW = (NPAIR + 3) / 4;  // pairs per wavefront
+      constexpr int NR = PPW <= 8 ? 8 : 16;
+      T acc[NR];
 #pragma unroll
-          for (int i = 0; i < NBW; ++i) {
-            s = fma(Yl[a * NBP + 64 * i], Yl[b * NBP + 64 * i], s);
-          }
-          s = wave_sum(s);
-          if (lane == 0) Sg[a + b * PB] = s;
+      for (int q = 0; q < NR; ++q) {
+        acc[q] = T(0);
+        const int pidx = wid * PPW + q;
+        if (q < PPW && pidx < NPAIR) {
+          int b = 1, a = pidx;
+          while (a >= b) { a -= b; ++b; }
+#pragma unroll
+          for (int i = 0; i < NBW; ++i) acc[q] = fma(Yl[a * NBP + 64 * i], Yl[b * NBP + 64 * i], acc[q]);
         }
+      }
+      reduce_scatter<NR>(acc, lane);
+      const int q = reduce_scatter_index<NR>(lane);
+      const int pidx = wid * PPW + q;
+      if ((lane & ((64 / NR) - 1)) == 0 && q < PPW && pidx < NPAIR) {
+        int b = 1, a = pidx;
+        while (a >= b) { a -= b; ++b; }
+        Sg[a + b * PB] = acc[0];
+      }
     }
     __syncthreads();
     // compact-WY factor (forward, column-wise): T(:c,c) = -tau_c T(:c,:c) S(:c,c)
@@ -440,7 +465,7 @@ constexpr int qr_cb() { return NBW >= 32 ? 1 : (NBW >= 16 ? 2 : 4); }
 template <typename T, int NBW>
 static size_t qr_smem() {
   constexpr int CB = qr_cb<NBW>();
-  return sizeof(T) * ((size_t)kPB * 64 * NBW + 2 * kPB * kPB + 2 * kPB + 4 * 2 * kPB * CB);
+  return sizeof(T) * ((size_t)kPB * 64 * NBW + 2 * kPB * kPB + 3 * kPB + 4 * 2 * kPB * CB);
 }
 
 template <typename T, int NBW>
@@ -486,14 +511,23 @@ __device__ __forceinline__ T gsum(T v) {
   return v;
 }
 
-// Squared column norms of the ncol columns of A (K rows, leading dim ld).
-template <typename T, int NT>
+// Squared column norms of the ncol columns of A (K rows, leading dim ld):
+// a group of G lanes per column, rows in registers (K <= G*RPL).
+template <typename T, int NT, int G, int RPL>
 __device__ void column_norms(const T* A, int ld, int ncol, int K, T* nrm) {
-  for (int j = threadIdx.x; j < ncol; j += NT) {
+  constexpr int NG = NT / G;
+  const int grp = threadIdx.x / G, gl = threadIdx.x % G;
+  for (int j = grp; j < ncol; j += NG) {
     const T* c = A + (size_t)j * ld;
     T s = T(0);
-    for (int row = 0; row < K; ++row) s = fma(c[row], c[row], s);
-    nrm[j] = s;
+#pragma unroll
+    for (int i = 0; i < RPL; ++i) {
+      const int row = gl + G * i;
+      const T x = row < K ? c[row] : T(0);
+      s = fma(x, x, s);
+    }
+    s = gsum<G>(s);
+    if (gl == 0) nrm[j] = s;
   }
 }
 
@@ -568,9 +602,10 @@ __device__ void jacobi_sweep(T* A, int ld, int ncol, int K, T tol, T thr2, T* nr
 // step whose largest remaining column norm^2 is <= thr2; returns that step k
 // (the numerical rank).  On return rdiag[c] = R(c,c) and the strict upper
 // triangle of A's first k rows holds R; piv[j] is the original column now in
-// position j.  Groups of G lanes update one column each; two block barriers
-// per step.
-template <typename T, int NT, int G>
+// position j.  Groups of G lanes update one column each, with the column's
+// rows (gl + G i, i < RPL, K <= G*RPL) in registers so that all its loads are
+// in flight together; two block barriers per step.
+template <typename T, int NT, int G, int RPL>
 __device__ int qrcp(T* A, int K, T thr2, T* cn, T* rdiag, int* piv, T* gv, int* gi, T* red) {
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   constexpr int NG = NT / G;
@@ -583,7 +618,12 @@ __device__ int qrcp(T* A, int K, T thr2, T* cn, T* rdiag, int* piv, T* gv, int* 
     for (int j = grp; j < K; j += NG) {
       const T* c = A + (size_t)j * K;
       T sq = T(0);
-      for (int i = gl; i < K; i += G) sq = fma(c[i], c[i], sq);
+#pragma unroll
+      for (int i = 0; i < RPL; ++i) {
+        const int row = gl + G * i;
+        const T x = row < K ? c[row] : T(0);
+        sq = fma(x, x, sq);
+      }
       sq = gsum<G>(sq);
       if (gl == 0) {
         cn[j] = sq;
@@ -636,24 +676,37 @@ __device__ int qrcp(T* A, int K, T thr2, T* cn, T* rdiag, int* piv, T* gv, int* 
       tau = (beta - alpha) / beta;
       scal = T(1) / (alpha - beta);
     }
-    // apply H = I - tau v v^T (v_s = 1, v_i = A(i,s)*scal) to columns j > s,
-    // recomputing each column's remaining norm exactly
+    // this lane's rows of the reflector v (v_s = 1, v_i = A(i,s)*scal, 0 above s)
+    T v[RPL];
+#pragma unroll
+    for (int i = 0; i < RPL; ++i) {
+      const int row = gl + G * i;
+      v[i] = row > s && row < K ? cs_[row] * scal : (row == s ? T(1) : T(0));
+    }
+    // apply H = I - tau v v^T to columns j > s, recomputing each column's
+    // remaining norm exactly
     T gbest = T(-1);
     int gidx = -1;
     for (int j = s + 1 + grp; j < K; j += NG) {
       T* cj = A + (size_t)j * K;
+      T x[RPL];
       T dot = T(0);
-      for (int i = s + gl; i < K; i += G) {
-        const T v = (i == s) ? T(1) : cs_[i] * scal;
-        dot = fma(v, cj[i], dot);
+#pragma unroll
+      for (int i = 0; i < RPL; ++i) {
+        const int row = gl + G * i;
+        x[i] = row >= s && row < K ? cj[row] : T(0);
+        dot = fma(v[i], x[i], dot);
       }
       const T w = gsum<G>(dot) * tau;
       T sq = T(0);
-      for (int i = s + gl; i < K; i += G) {
-        const T v = (i == s) ? T(1) : cs_[i] * scal;
-        const T a = fma(-v, w, cj[i]);
-        cj[i] = a;
-        if (i > s) sq = fma(a, a, sq);
+#pragma unroll
+      for (int i = 0; i < RPL; ++i) {
+        const int row = gl + G * i;
+        if (row >= s && row < K) {
+          const T a = fma(-v[i], w, x[i]);
+          cj[row] = a;
+          if (row > s) sq = fma(a, a, sq);
+        }
       }
       sq = gsum<G>(sq);
       if (gl == 0) cn[j] = sq;
@@ -792,7 +845,7 @@ __global__ __launch_bounds__(NT) void k_core(const CoreProb<T>* __restrict__ pro
     const T thr = T(1e-3) * acc / sqrt(T(K));
     thr2 = thr * thr;
   }
-  const int k = qrcp<T, NT, G>(A, K, thr2, nrm, rdiag, piv, gv, gi, red);
+  const int k = qrcp<T, NT, G, RPL>(A, K, thr2, nrm, rdiag, piv, gv, gi, red);
   block_sync<NT>();
   r_transpose<T, NT>(A, K, k, rdiag);
   block_sync<NT>();
@@ -800,7 +853,7 @@ __global__ __launch_bounds__(NT) void k_core(const CoreProb<T>* __restrict__ pro
   int sweep = 0;
   if (k > 1) {
     for (; sweep < kMaxSweeps; ++sweep) {
-      column_norms<T, NT>(A, K, k, K, nrm);
+      column_norms<T, NT, G, RPL>(A, K, k, K, nrm);
       if (tid == 0) flags[0] = 0;
       block_sync<NT>();
       jacobi_sweep<T, NT, G, RPL>(A, K, k, K, tol, thr2, nrm, &flags[0]);
@@ -827,6 +880,7 @@ int core_lds_max_k(size_t elem) {
   X(CORE_K64, 64, 2, 32, true)                          \
   X(CORE_K128, 256, 4, 32, true)                        \
   X(CORE_KLDS, 512, 4, 36, true)                        \
+  X(CORE_K192, 768, 8, 24, false)                       \
   X(CORE_K256, 1024, 16, 16, false)                     \
   X(CORE_K512, 1024, 32, 16, false)
 
@@ -864,6 +918,7 @@ int core_variant_for(int K, int kl) {
   if (K <= 64) return CORE_K64;
   if (K <= 128) return CORE_K128;
   if (K <= kl && K <= 144) return CORE_KLDS;
+  if (K <= 192) return CORE_K192;
   if (K <= 256) return CORE_K256;
   return CORE_K512;
 }

@@ -88,6 +88,8 @@ const int *HCORE_hip_plan_ranks_device(const hcore_hip_plan *plan);
 /* Copy new ranks / per-update status (HCORE_HIP_*) to host; synchronizes. */
 int HCORE_hip_plan_get_ranks(hcore_hip_plan *plan, int *ranks_host);
 int HCORE_hip_plan_get_status(hcore_hip_plan *plan, int *status_host);
+/* Copy the lane each update runs in (index into the timing lanes). */
+int HCORE_hip_plan_get_lanes(hcore_hip_plan *plan, int *lanes_host);
 /* Copy the number of Jacobi sweeps each update's core SVD used (diagnostic). */
 int HCORE_hip_plan_get_sweeps(hcore_hip_plan *plan, int *sweeps_host);
 /* Copy the K singular values of update u (descending, FP64) to host. */
@@ -100,13 +102,18 @@ int HCORE_hip_plan_flops(hcore_hip_plan *plan, unsigned long *update, unsigned l
 
 void HCORE_hip_plan_destroy(hcore_hip_plan *plan);
 
-/* Per-phase timing: when enabled, HCORE_hip_plan_execute records a HIP event
- * pair around every kernel launch on the execution stream.  Phases, in
- * order: 0 stack, 1 F=AV^T BV, 2 G=BU F^T, 3 geqrf, 4 T=R_A R_B^T,
- * 5 core SVD (all K buckets, run concurrently), 6 reserved, 7 Zv=T^T Zu,
- * 8 reassembly, 9 syrk W, 10 syrk X, 11 syrk CD.  Times accumulate over
- * executions. */
+/* Per-phase timing.  A plan of more than 64 updates splits them into up to
+ * three lanes by stacked rank K (core beyond LDS, 97..LDS limit, <= 96), each
+ * running its kernels on its own stream; the dsyrk updates run on a fourth.  When timing is enabled,
+ * HCORE_hip_plan_execute records a HIP event pair around every kernel launch
+ * on the stream it runs on.  Phases: 0 stack, 1 F=AV^T BV, 2 G=BU F^T,
+ * 3 geqrf, 4 T=R_A R_B^T, 5 core SVD (all K buckets of the lane, run
+ * concurrently), 6 reserved, 7 Zv=T^T Zu, 8 reassembly, 9 syrk W,
+ * 10 syrk X, 11 syrk CD.  HCORE_hip_plan_phase_times fills
+ * HCORE_HIP_NLANES * HCORE_HIP_NPHASES entries (index lane*NPHASES+phase;
+ * lane 3 holds the syrk phases); times accumulate over executions. */
 #define HCORE_HIP_NPHASES 12
+#define HCORE_HIP_NLANES 4
 int HCORE_hip_plan_set_timing(hcore_hip_plan *plan, int enable);
 int HCORE_hip_plan_phase_times(hcore_hip_plan *plan, double *ms, int *count);
 
