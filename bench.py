@@ -9,13 +9,16 @@ maxrank 512, FP64.  Tiles are synthetic (seeded), resident in HBM before the
 timed region; the recompressed factors are written out of place so every
 step performs identical work (the diagonal dsyrk updates run in place).
 
-A "step" is one execution of the whole batch (one plan execution: 12 kernel
-launches on one stream).  value = tile updates per second (2016 per step).
+A "step" is one execution of the whole batch: one plan execution, whose
+updates run in up to three lanes by stacked rank (each on its own stream)
+plus the diagonal dsyrk updates on a fourth, captured once into a hipGraph
+and replayed.  value = tile updates per second (2016 per step), timed over
+graph replays with HIP events on the launching stream.
 
 Also measured in the same run:
-  * per-kernel durations with HIP events around each launch, on the stream
-    the kernels run on -> the roofline of the dominant kernel;
-  * the same batch replayed through a hipGraph;
+  * per-kernel durations: the same step executed eagerly with a HIP event
+    pair around every launch, on the stream the kernel runs on -> the
+    roofline of the dominant kernel;
   * the CPU baseline: the oracle (the reference's LAPACK call sequence,
     scipy-openblas, one update per OpenMP thread) on the host cores.
 """
@@ -152,29 +155,38 @@ def main():
     b_unique = int(np.sum(8 * nb * (2 * rc.astype(np.int64) + 2 * r))) + panel_bytes + \
         ns * 8 * 2 * nb * nb
 
-    # ---- timed region: eager execution, HIP events around every launch ----------
-    plan.set_timing(True)
-    for _ in range(args.warmup):
-        plan.execute(stream)
+    # ---- timed region: the step replayed through its captured hipGraph --------
+    plan.execute(stream, graph=True)     # capture + first replay
     stream.synchronize()
-    plan.phase_times()                    # collect warm-up phases ...
-    plan.set_timing(False)
-    plan.set_timing(True)
-    ms0, c0 = plan.phase_times()          # ... and start from these totals
-    e0, e1 = hc.Event(), hc.Event()
+    for _ in range(args.warmup):
+        plan.execute(stream, graph=True)
+    stream.synchronize()
     hc.device_synchronize()
+    e0, e1 = hc.Event(), hc.Event()
     e0.record(stream)
     for _ in range(args.steps):
-        plan.execute(stream)
+        plan.execute(stream, graph=True)
     e1.record(stream)
     stream.synchronize()
     hc.device_synchronize()
-    total_ms = e0.elapsed_ms(e1)
-    ms1, c1 = plan.phase_times()
-    phase_ms = (ms1 - ms0) / np.maximum(c1 - c0, 1)   # (lane, phase) average per launch
-    ms_per_step = total_ms / args.steps
+    ms_per_step = e0.elapsed_ms(e1) / args.steps
     updates = n + ns
     value = updates / (ms_per_step / 1e3)
+
+    # ---- per-kernel durations: the same step executed eagerly with a HIP event
+    # pair around every launch, on the stream each kernel runs on ------------------
+    plan.set_timing(True)
+    ms0, c0 = plan.phase_times()
+    q0, q1 = hc.Event(), hc.Event()
+    q0.record(stream)
+    for _ in range(args.steps):
+        plan.execute(stream)
+    q1.record(stream)
+    stream.synchronize()
+    eager_ms = q0.elapsed_ms(q1) / args.steps
+    ms1, c1 = plan.phase_times()
+    plan.set_timing(False)
+    phase_ms = (ms1 - ms0) / np.maximum(c1 - c0, 1)   # (lane, phase) average per launch
     lane_names = ["lane0", "lane1", "lane2", "syrk"]
     for l in range(3):
         m = lane_of == l
@@ -184,32 +196,12 @@ def main():
         row = " ".join("%s %.3f" % (PHASES[i], phase_ms[l, i]) for i in range(12) if c1[l, i] > c0[l, i])
         if row:
             log("  lane %-10s %s" % (lane_names[l], row))
-
-    # ---- eager without per-launch events, and the same batch as a hipGraph -----
-    plan.set_timing(False)
-    q0, q1 = hc.Event(), hc.Event()
-    q0.record(stream)
-    for _ in range(args.steps):
-        plan.execute(stream)
-    q1.record(stream)
-    stream.synchronize()
-    eager_plain_ms = q0.elapsed_ms(q1) / args.steps
-    plan.execute(stream, graph=True)
-    stream.synchronize()
-    g0, g1 = hc.Event(), hc.Event()
-    g0.record(stream)
-    for _ in range(args.steps):
-        plan.execute(stream, graph=True)
-    g1.record(stream)
-    stream.synchronize()
-    graph_ms = g0.elapsed_ms(g1) / args.steps
-    log("eager+events %.3f ms/step, eager %.3f ms/step, graph %.3f ms/step" %
-        (ms_per_step, eager_plain_ms, graph_ms))
+    log("graph %.3f ms/step (timed), eager with per-launch events %.3f ms/step" %
+        (ms_per_step, eager_ms))
 
     # ---- dominant kernel roofline ------------------------------------------------
     # the (lane, kernel) with the largest average launch duration; its
     # algorithmic work is that of the updates in the lane
-    lane_of = plan.lanes()
     dl, dp = np.unravel_index(int(np.argmax(phase_ms)), phase_ms.shape)
     dname = PHASES[dp]
     sel = lane_of == dl
@@ -286,7 +278,8 @@ def main():
                                "1953 HCORE_dgemm + 63 HCORE_dsyrk, uneven ranks 8-160, "
                                "acc=1e-8, maxrank=512",
                    "grid": args.grid, "nb": nb, "updates_per_step": updates,
-                   "precision": "fp64", "graph_ms_per_step": graph_ms},
+                   "precision": "fp64", "execution": "hipGraph replay of the step",
+                   "eager_ms_per_step": eager_ms},
         "roofline": roof,
         "cpu_baseline": cpu,
         "gflops": F_total / (ms_per_step / 1e3) / 1e9,
