@@ -87,6 +87,7 @@ def lib():
         L.HCORE_hip_plan_get_status.argtypes = [_vp, _ip]
         L.HCORE_hip_plan_get_sigma.argtypes = [_vp, i, _dp, _ip]
         L.HCORE_hip_plan_get_sweeps.argtypes = [_vp, _ip]
+        L.HCORE_hip_plan_get_core_profile.argtypes = [_vp, C.POINTER(C.c_longlong), _dp]
         L.HCORE_hip_plan_get_lanes.argtypes = [_vp, _ip]
         L.HCORE_hip_plan_workspace_bytes.argtypes = [_vp]
         L.HCORE_hip_plan_workspace_bytes.restype = C.c_longlong
@@ -381,6 +382,24 @@ class Plan:
         out = np.zeros(max(self.n, 1), dtype=np.int32)
         _check(self.L.HCORE_hip_plan_get_sweeps(self.h, out.ctypes.data_as(_ip)), "get_sweeps")
         return out[:self.n]
+
+    def core_profile(self):
+        """(n, 5) array per update: microseconds in the pivoted QR, the Jacobi
+        sweeps and the epilogue, the whole core kernel, and the QR's numerical
+        rank (zeros for updates whose core kernel records no stamps)."""
+        raw = np.zeros((max(self.n, 1), 8), dtype=np.int64)
+        tick = C.c_double(0)
+        _check(self.L.HCORE_hip_plan_get_core_profile(
+            self.h, raw.ctypes.data_as(C.POINTER(C.c_longlong)), C.byref(tick)), "core_profile")
+        raw = raw[:self.n]
+        us = tick.value / 1e3
+        out = np.zeros((self.n, 5))
+        out[:, 0] = (raw[:, 1] - raw[:, 0]) * us
+        out[:, 1] = (raw[:, 2] - raw[:, 1]) * us
+        out[:, 2] = (raw[:, 3] - raw[:, 2]) * us
+        out[:, 3] = (raw[:, 3] - raw[:, 0]) * us
+        out[:, 4] = raw[:, 4]
+        return out
 
     def sigma(self, u):
         K = C.c_int(0)

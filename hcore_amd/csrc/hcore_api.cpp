@@ -75,6 +75,7 @@ struct Context {
   hipStream_t stream = nullptr;
   int policy = HCORE_HIP_ON_ERROR_EXIT;
   int core_kl_d = 0, core_kl_f = 0;
+  bool core_reg = true;  // register-resident core SVD for 33 <= K <= 192
   // grow-only device arena used by the single-call entry points
   char* arena = nullptr;
   size_t arena_cap = 0;
@@ -86,6 +87,9 @@ static thread_local int t_last_status = HCORE_HIP_OK;
 __attribute__((constructor)) static void policy_from_env() {
   const char* pol = getenv("HCORE_ON_ERROR");
   if (pol && strcmp(pol, "return") == 0) g_ctx.policy = HCORE_HIP_ON_ERROR_RETURN;
+  // HCORE_CORE_REG=0 selects the LDS / global-memory core SVD kernels for all K
+  const char* reg = getenv("HCORE_CORE_REG");
+  if (reg && strcmp(reg, "0") == 0) g_ctx.core_reg = false;
 }
 
 static int hip_fail(hipError_t e, const char* what) {
@@ -187,6 +191,11 @@ struct hcore_hip_plan {
     int n = 0;
     size_t o_stack = 0, o_tF = 0, o_tG = 0, o_tT = 0, o_tZ = 0, o_qr = 0, o_apply = 0;
     int n_tF = 0, n_tG = 0, n_tT = 0, n_tZ = 0, n_qr = 0, n_apply = 0;
+    int stack_cols = 0;  // largest 2rc+ra of the lane
+    int kmax = 0;        // largest stacked rank of the lane
+    bool wy = false;     // reassembly by the compact-WY solve (kmax <= kWyMaxK)
+    size_t o_tB = 0, o_wy = 0, o_tX = 0;
+    int n_tB = 0, n_wy = 0, n_tX = 0;
     std::vector<CoreLaunch> core;  // one launch per K bucket
     hipStream_t stream = nullptr;  // lanes > 0 (lane 0 uses the caller's stream)
     hipEvent_t ev_done = nullptr;
@@ -198,6 +207,7 @@ struct hcore_hip_plan {
   hipStream_t syrk_stream = nullptr;
   hipEvent_t ev_start = nullptr, ev_syrk = nullptr;
   int* d_sweeps = nullptr;
+  long long* d_clk = nullptr;  // kCoreClk core stamps per update
   // graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -234,8 +244,11 @@ static int plan_finalize_t(hcore_hip_plan* p) {
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return Kv[a] > Kv[b]; });
 
-  // lanes: K > KL (core in global memory) | 97..KL | <= 96; empty lanes dropped
-  const int lane_lo[3] = {KL + 1, 97, 0};
+  // lanes by stacked rank (empty lanes dropped): with the register-resident
+  // core K > 128 | 65..128 | <= 64, else K > KL (core in global memory) |
+  // 97..KL | <= 96
+  const bool reg = g_ctx.core_reg;
+  const int lane_lo[3] = {reg ? 129 : KL + 1, reg ? 65 : 97, 0};
   std::vector<std::vector<int>> lane_members(3);
   for (int idx = 0; idx < n; ++idx) {
     const int u = order[idx];
@@ -255,9 +268,20 @@ static int plan_finalize_t(hcore_hip_plan* p) {
     int kmax = 0;
     for (int u : lane_members[l]) kmax = std::max(kmax, Kv[u]);
     for (int u : lane_members[l])
-      variant[u] = core_variant_for(l < 2 && n > 64 ? kmax : Kv[u], KL);
+      variant[u] = core_variant_for(l < 2 && n > 64 ? kmax : Kv[u], KL, reg);
   }
-  auto global_core = [](int v) { return v == CORE_K192 || v == CORE_K256 || v == CORE_K512; };
+  // variants that need the K x K global scratch (work matrix, or the R^T
+  // transposition of the register-resident kernels)
+  auto global_core = [](int v) { return v == CORE_K192 || v == CORE_K256 || v == CORE_K512 ||
+                                        v >= CORE_R64; };
+  // reassembly per lane: the compact-WY solve and GEMMs up to kWyMaxK,
+  // the per-panel reflector application beyond
+  bool lane_wy[3];
+  for (int l = 0; l < 3; ++l) {
+    int kmax = 0;
+    for (int u : lane_members[l]) kmax = std::max(kmax, Kv[u]);
+    lane_wy[l] = kmax <= kWyMaxK;
+  }
   // lane index as executed (empty lanes are dropped, so renumber)
   p->lane_of.assign(n, 0);
   {
@@ -271,7 +295,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
 
   // ---- workspace layout (elements of T) ----
   std::vector<size_t> o_Us(n), o_Vs(n), o_tA(n), o_tB(n), o_TA(n), o_TB(n), o_F(n), o_Tc(n),
-      o_Tj(n), o_Zu(n), o_Zv(n), o_sg(n);
+      o_Tj(n), o_Zu(n), o_Zv(n), o_sg(n), o_S(n), o_M(n);
   size_t off = 0;
   for (int u = 0; u < n; ++u) {
     const int K = Kv[u];
@@ -283,9 +307,11 @@ static int plan_finalize_t(hcore_hip_plan* p) {
     o_TB[u] = off; off += al(tp_elems(K));
     o_F[u] = off; off += al((size_t)p->ra[u] * p->rb[u]);
     o_Tc[u] = off; off += al((size_t)K * K);
-    o_Tj[u] = off; off += global_core(variant[u]) ? al((size_t)K * K) : 0;
+    o_Tj[u] = off; off += global_core(variant[u]) ? al((size_t)K * (K + 64)) : 0;  // ld <= K + 2G - 1
     o_Zu[u] = off; off += al((size_t)K * K);
     o_Zv[u] = off; off += al((size_t)K * K);
+    o_S[u] = off; off += 2 * al((size_t)K * K);   // Gram Y^T Y of both stacks
+    o_M[u] = off; off += 2 * al((size_t)K * K);   // compact-WY solves of both sides
     o_sg[u] = off; off += al(K);
   }
   std::vector<size_t> o_W(p->ns), o_X(p->ns);
@@ -300,6 +326,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
   // ---- device memory: [rank | status | sweeps | workspace | descriptors] ----
   Packer pk;
   int* d_sweeps = nullptr;
+  long long* d_clk = nullptr;
   // descriptor contents reference device addresses, so the base must be known
   // first: size the descriptors with a dry run, then allocate, then fill.
   auto build = [&](char* wsbase, int* d_rank, int* d_status) {
@@ -308,13 +335,16 @@ static int plan_finalize_t(hcore_hip_plan* p) {
     std::vector<GemmProb<T>> gp;
     struct LaneDesc {
       std::vector<StackProb<T>> stack;
-      std::vector<GemmTask> tF, tG, tT, tZ;
+      std::vector<GemmTask> tF, tG, tT, tZ, tB, tX;
+      std::vector<WySolveProb<T>> wy;
+      int kmax = 0;
+      bool wy_lane = false;
       std::vector<QrProb<T>> qr;
       std::vector<std::vector<CoreProb<T>>> core_b;
       std::vector<int> core_kmax;
       std::vector<ApplyProb<T>> ap;
     };
-    constexpr int NV = CORE_K512 + 1;
+    constexpr int NV = CORE_NVARIANTS;
     std::vector<LaneDesc> ld;
     for (int l = 0; l < 3; ++l) {
       if (lane_members[l].empty()) continue;
@@ -353,19 +383,61 @@ static int plan_finalize_t(hcore_hip_plan* p) {
         g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_BETA_ZERO;
         g.n_dev = d_rank + u; g.skip = d_status + u;
         add_tasks(d.tZ, (int)gp.size(), g.m, g.n); gp.push_back(g);
+        if (lane_wy[l]) {
+          T* SA = W + o_S[u];
+          T* SB = SA + al((size_t)K * K);
+          T* MA = W + o_M[u];
+          T* MB = MA + al((size_t)K * K);
+          T* Zs[2] = {W + o_Zu[u], W + o_Zv[u]};
+          T* Ys[2] = {Us, Vs};
+          T* Ss[2] = {SA, SB};
+          T* Ms[2] = {MA, MB};
+          T* Xo[2] = {(T*)(p->CUo[u] ? p->CUo[u] : p->CU[u]), (T*)(p->CVo[u] ? p->CVo[u] : p->CV[u])};
+          for (int side = 0; side < 2; ++side) {
+            // Gram S = Y^T Y of the unit lower trapezoidal reflector block (with T)
+            g = GemmProb<T>{};
+            g.A = Ys[side]; g.B = Ys[side]; g.C = Ss[side];
+            g.m = K; g.n = K; g.k = M; g.lda = M; g.ldb = M; g.ldc = K;
+            g.alpha = T(1); g.beta = T(0);
+            g.flags = GF_A_TRANS | GF_A_UNIT_LOWER | GF_B_UNIT_LOWER | GF_BETA_ZERO;
+            add_tasks(d.tT, (int)gp.size(), g.m, g.n); gp.push_back(g);
+            // B = Y1^T Z (Z = Zu with Zv's launch, Z = Zv after it)
+            g = GemmProb<T>{};
+            g.A = Ys[side]; g.B = Zs[side]; g.C = Ms[side];
+            g.m = K; g.n = K; g.k = K; g.lda = M; g.ldb = K; g.ldc = K;
+            g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_A_UNIT_LOWER | GF_BETA_ZERO;
+            g.n_dev = d_rank + u; g.skip = d_status + u;
+            add_tasks(side == 0 ? d.tZ : d.tB, (int)gp.size(), g.m, g.n); gp.push_back(g);
+            // X = [Z; 0] - Y M into the caller's factor
+            g = GemmProb<T>{};
+            g.A = Ys[side]; g.B = Ms[side]; g.C = Xo[side];
+            g.m = M; g.n = K; g.k = K; g.lda = M; g.ldb = K; g.ldc = p->ldco;
+            g.alpha = T(-1); g.beta = T(1); g.flags = GF_A_UNIT_LOWER | GF_C0;
+            g.C0 = Zs[side]; g.ldc0 = K; g.m0 = K;
+            g.n_dev = d_rank + u; g.skip = d_status + u;
+            add_tasks(d.tX, (int)gp.size(), g.m, g.n); gp.push_back(g);
+            d.wy.push_back({Ss[side], side == 0 ? W + o_TA[u] : W + o_TB[u], Ms[side], d_rank + u,
+                            d_status + u, K});
+          }
+        }
         // QR of both stacks
         d.qr.push_back({Us, W + o_tA[u], W + o_TA[u], K});
         d.qr.push_back({Vs, W + o_tB[u], W + o_TB[u], K});
         CoreProb<T> c{W + o_Tc[u], global_core(variant[u]) ? W + o_Tj[u] : nullptr, W + o_Zu[u],
-                      W + o_sg[u], d_rank + u, d_status + u, d_sweeps + u, K};
+                      W + o_sg[u], d_rank + u, d_status + u, d_sweeps + u,
+                      d_clk ? d_clk + (size_t)u * kCoreClk : nullptr, K};
         const int b = variant[u];
         d.core_b[b].push_back(c);
         d.core_kmax[b] = std::max(d.core_kmax[b], K);
         T* CUo = (T*)(p->CUo[u] ? p->CUo[u] : p->CU[u]);
         T* CVo = (T*)(p->CVo[u] ? p->CVo[u] : p->CV[u]);
-        d.ap.push_back({Us, W + o_TA[u], W + o_Zu[u], CUo, d_rank + u, d_status + u, K, p->ldco});
-        d.ap.push_back({Vs, W + o_TB[u], W + o_Zv[u], CVo, d_rank + u, d_status + u, K, p->ldco});
+        if (!lane_wy[l]) {
+          d.ap.push_back({Us, W + o_TA[u], W + o_Zu[u], CUo, d_rank + u, d_status + u, K, p->ldco});
+          d.ap.push_back({Vs, W + o_TB[u], W + o_Zv[u], CVo, d_rank + u, d_status + u, K, p->ldco});
+        }
+        d.kmax = std::max(d.kmax, K);
       }
+      d.wy_lane = lane_wy[l];
       ld.push_back(std::move(d));
     }
     // syrk (hcore_dsyrk.c:103, 144, 165-181)
@@ -396,17 +468,29 @@ static int plan_finalize_t(hcore_hip_plan* p) {
       auto& d = ld[l];
       L.n = (int)d.stack.size();
       L.o_stack = pk.add(d.stack);
+      L.stack_cols = 0;
+      for (const auto& sp_ : d.stack) L.stack_cols = std::max(L.stack_cols, 2 * sp_.rc + sp_.ra);
       L.o_tF = pk.add(d.tF); L.n_tF = (int)d.tF.size();
       L.o_tG = pk.add(d.tG); L.n_tG = (int)d.tG.size();
       L.o_tT = pk.add(d.tT); L.n_tT = (int)d.tT.size();
       L.o_tZ = pk.add(d.tZ); L.n_tZ = (int)d.tZ.size();
       L.o_qr = pk.add(d.qr); L.n_qr = (int)d.qr.size();
       L.o_apply = pk.add(d.ap); L.n_apply = (int)d.ap.size();
+      L.o_tB = pk.add(d.tB); L.n_tB = (int)d.tB.size();
+      L.o_tX = pk.add(d.tX); L.n_tX = (int)d.tX.size();
+      L.o_wy = pk.add(d.wy); L.n_wy = (int)d.wy.size();
+      L.kmax = d.kmax;
+      L.wy = d.wy_lane;
       L.core.clear();
-      for (int b = NV - 1; b >= 0; --b) {  // largest K first: the longest jobs start first
+      for (int b = 0; b < NV; ++b) {
         if (d.core_b[b].empty()) continue;
         L.core.push_back({pk.add(d.core_b[b]), (int)d.core_b[b].size(), b, d.core_kmax[b]});
       }
+      // largest K first: the longest jobs start first
+      std::stable_sort(L.core.begin(), L.core.end(),
+                       [](const hcore_hip_plan::CoreLaunch& x, const hcore_hip_plan::CoreLaunch& y) {
+                         return x.kmax > y.kmax;
+                       });
     }
     p->o_sp = pk.add(sp);
     p->o_tW = pk.add(tW); p->n_tW = (int)tW.size();
@@ -414,7 +498,9 @@ static int plan_finalize_t(hcore_hip_plan* p) {
     p->o_tC = pk.add(tC); p->n_tC = (int)tC.size();
   };
   build(nullptr, nullptr, nullptr);  // dry run: descriptor size
-  const size_t head = ((size_t)std::max(n, 1) * 3 * sizeof(int) + 255) & ~size_t(255);
+  const size_t head_i = ((size_t)std::max(n, 1) * 3 * sizeof(int) + 255) & ~size_t(255);
+  const size_t head = head_i + (((size_t)std::max(n, 1) * kCoreClk * sizeof(long long) + 255) &
+                                ~size_t(255));
   const size_t wsb = (ws_elems * es + 255) & ~size_t(255);
   const size_t total = head + wsb + pk.buf.size() + 256;
   char* base = nullptr;
@@ -437,6 +523,8 @@ static int plan_finalize_t(hcore_hip_plan* p) {
   p->d_status = p->d_rank + std::max(n, 1);
   p->d_sweeps = p->d_status + std::max(n, 1);
   d_sweeps = p->d_sweeps;
+  p->d_clk = reinterpret_cast<long long*>(base + head_i);
+  d_clk = p->d_clk;
   p->ws = base + head;
   p->desc = base + head + wsb;
   build(p->ws, p->d_rank, p->d_status);
@@ -510,8 +598,8 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
     hipStream_t ls = l == 0 ? s : L.stream;
     if (l > 0) (void)hipStreamWaitEvent(ls, p->ev_start, 0);
     phase(l, 0, ls, true, [&] {
-      launch_stack<T>(reinterpret_cast<const StackProb<T>*>(D(L.o_stack)), L.n, M, T(p->alpha),
-                      T(p->beta), ls);
+      launch_stack<T>(reinterpret_cast<const StackProb<T>*>(D(L.o_stack)), L.n, M, L.stack_cols,
+                      T(p->alpha), T(p->beta), ls);
     });
     phase(l, 1, ls, true, [&] { launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(L.o_tF)), L.n_tF, ls); });
     phase(l, 2, ls, true, [&] { launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(L.o_tG)), L.n_tG, ls); });
@@ -527,9 +615,20 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
                        c.kmax, p->rk, p->maxrk, T(p->acc), ls);
     });
     phase(l, 7, ls, true, [&] { launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(L.o_tZ)), L.n_tZ, ls); });
-    phase(l, 8, ls, true, [&] {
-      launch_apply<T>(reinterpret_cast<const ApplyProb<T>*>(D(L.o_apply)), L.n_apply, M, ls);
-    });
+    if (L.wy) {
+      phase(l, 6, ls, true, [&] {
+        launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(L.o_tB)), L.n_tB, ls);
+        launch_wy_solve<T>(reinterpret_cast<const WySolveProb<T>*>(D(L.o_wy)), L.n_wy, L.kmax,
+                           L.kmax, ls);
+      });
+      phase(l, 8, ls, true, [&] {
+        launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(L.o_tX)), L.n_tX, ls);
+      });
+    } else {
+      phase(l, 8, ls, true, [&] {
+        launch_apply<T>(reinterpret_cast<const ApplyProb<T>*>(D(L.o_apply)), L.n_apply, M, ls);
+      });
+    }
     if (l > 0) (void)hipEventRecord(L.ev_done, ls);
   }
   if (p->ns > 0) {
@@ -993,6 +1092,20 @@ int HCORE_hip_plan_get_sweeps(hcore_hip_plan* p, int* out) {
   if (!p || !p->finalized || !out) return HCORE_HIP_ERR_ARG;
   HCHECK(hipDeviceSynchronize());
   if (p->ng) HCHECK(hipMemcpy(out, p->d_sweeps, sizeof(int) * p->ng, hipMemcpyDeviceToHost));
+  return HCORE_HIP_OK;
+}
+
+int HCORE_hip_plan_get_core_profile(hcore_hip_plan* p, long long* out, double* tick_ns) {
+  if (!p || !p->finalized) return HCORE_HIP_ERR_ARG;
+  if (tick_ns) {
+    int khz = 0;
+    HCHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, g_ctx.device));
+    *tick_ns = khz > 0 ? 1e6 / khz : 0.0;
+  }
+  if (!out) return HCORE_HIP_OK;
+  HCHECK(hipDeviceSynchronize());
+  if (p->ng)
+    HCHECK(hipMemcpy(out, p->d_clk, sizeof(long long) * kCoreClk * p->ng, hipMemcpyDeviceToHost));
   return HCORE_HIP_OK;
 }
 

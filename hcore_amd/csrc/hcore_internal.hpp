@@ -27,6 +27,9 @@ enum : int {
   GF_B_UPPER = 8,           // stored B(p,q) read as 0 when p > q
   GF_LOWER_ZERO_UPPER = 16, // write i >= j only; strict upper triangle of C set to 0
   GF_BETA_ZERO = 32,        // C is not read
+  GF_A_UNIT_LOWER = 64,     // stored A(p,q) read as 0 when p < q and 1 when p == q
+  GF_B_UNIT_LOWER = 128,    // same for B
+  GF_C0 = 256,              // the beta term reads C0 (rows < m0; 0 below) instead of C
 };
 
 template <typename T>
@@ -40,6 +43,8 @@ struct GemmProb {
   int flags;
   const int* n_dev;   // if set, n is clamped to *n_dev (device-side rank)
   const int* skip;    // if set and *skip != 0 the problem is skipped
+  const T* C0;        // GF_C0: source of the beta term, leading dimension ldc0
+  int ldc0, m0;
 };
 
 struct GemmTask {
@@ -77,6 +82,23 @@ struct CoreProb {
   int* rank;    // new rank r
   int* status;  // ST_*
   int* sweeps;  // Jacobi sweeps used (diagnostic, may be null)
+  long long* clk;  // kCoreClk wall-clock stamps (diagnostic, may be null)
+  int K;
+};
+// core stamps: [0] start, [1] after the pivoted QR, [2] after the Jacobi
+// sweeps, [3] end, [4] numerical rank k of the pivoted QR
+constexpr int kCoreClk = 8;
+
+// ---- compact-WY solve: M = T Y1^T Z for Q [Z; 0] = [Z; 0] - Y M --------------
+// With S = Y^T Y and the per-panel factors T_bb, M solves
+// (diag(1/tau) + striu(S)) M = Y1^T Z; on entry M holds Y1^T Z.
+template <typename T>
+struct WySolveProb {
+  const T* S;    // K x K Gram Y^T Y (ld K; its lower triangle is read)
+  const T* Tp;   // per-panel compact-WY factors (kPB x kPB each)
+  T* M;          // K x r, ld K: Y1^T Z on entry, M on exit
+  const int* rank;
+  const int* status;
   int K;
 };
 
@@ -95,27 +117,47 @@ struct ApplyProb {
 constexpr int kThreads = 256;
 constexpr int kPB = 8;  // Householder panel width
 
+// Leading dimension of the core SVD's work matrix: the smallest odd multiple
+// of the group width G that is >= K.  Groups of G lanes read G consecutive
+// rows of different columns at the same time; an odd multiple of G between
+// columns puts the groups of a wavefront on distinct LDS banks.
+__host__ __device__ constexpr int core_ld(int K, int G) {
+  const int q = (K + G - 1) / G;
+  return G * (q | 1);
+}
+constexpr int kCoreLdsG = 4;  // group width of the largest LDS-resident core variant
+
 // Largest K whose K x K FP64 core fits in LDS next to the Jacobi bookkeeping.
 int core_lds_max_k(size_t elem_size);
 
 // ---- launchers (hcore_kernels.hip) -------------------------------------------
 template <typename T>
 void launch_gemm(const GemmProb<T>* probs, const GemmTask* tasks, int ntasks, hipStream_t s);
+constexpr int kStackCols = 4;  // stacked columns per workgroup of k_stack
 template <typename T>
-void launch_stack(const StackProb<T>* probs, int n, int nb, T alpha, T beta, hipStream_t s);
+void launch_stack(const StackProb<T>* probs, int n, int nb, int maxcols, T alpha, T beta,
+                  hipStream_t s);
 template <typename T>
 void launch_geqrf(const QrProb<T>* probs, int n, int nb, hipStream_t s);
 // core SVD variants by stacked rank K: one wavefront for K <= 64, one
 // workgroup with the core in LDS up to the LDS limit, 1024 threads with the
 // core in global memory beyond it
-enum : int { CORE_K16, CORE_K32, CORE_K64, CORE_K128, CORE_KLDS, CORE_K192, CORE_K256, CORE_K512 };
-int core_variant_for(int K, int lds_max_k);
+// (LDS/global variants), then the register-resident variants (k_core_reg)
+enum : int {
+  CORE_K16, CORE_K32, CORE_K64, CORE_K128, CORE_KLDS, CORE_K192, CORE_K256, CORE_K512,
+  CORE_R64, CORE_R128, CORE_R192, CORE_NVARIANTS
+};
+int core_variant_for(int K, int lds_max_k, bool reg);
 constexpr int kCoreMaxK = 512;
 template <typename T>
 void launch_core(const CoreProb<T>* probs, int n, int variant, int kmax, int rk, int maxrk,
                  T acc, hipStream_t s);
 template <typename T>
 void launch_apply(const ApplyProb<T>* probs, int n, int nb, hipStream_t s);
+// blocked back substitution of the compact-WY solve, 64 columns per workgroup
+constexpr int kWyMaxK = 256;
+template <typename T>
+void launch_wy_solve(const WySolveProb<T>* probs, int n, int kmax, int rmax, hipStream_t s);
 
 template <typename T>
 void prepare_kernels();

@@ -16,6 +16,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <limits>
 #include <type_traits>
 
 #include "hcore_internal.hpp"
@@ -88,13 +89,58 @@ __device__ __forceinline__ float eps_of<float>() { return FLT_EPSILON * 0.5f; }
 // batched GEMM on 64x64 tiles, 4x4 outputs per thread, k staged through LDS
 // ------------------------------------------------------------------------------
 constexpr int GT = 64;  // output tile edge
-constexpr int GK = 16;  // k chunk
+constexpr int GK = 32;  // k chunk
+
+// Load the k chunk starting at k0 of op(A) (rows row0..) and op(B) (columns
+// col0..) into registers: 8 elements of each per thread, with consecutive
+// lanes on consecutive addresses for every transpose combination.
+template <typename T>
+__device__ __forceinline__ T gemm_masked(const T* X, int ld, int p, int q, bool upper, bool unit_lower) {
+  if (upper && p > q) return T(0);
+  if (unit_lower && p <= q) return p == q ? T(1) : T(0);
+  return X[p + (size_t)q * ld];
+}
+
+template <typename T>
+__device__ __forceinline__ void gemm_fetch(const GemmProb<T>& P, int n, int row0, int col0, int k0,
+                                           bool ta, bool tb, bool ua, bool ub, T (&ra)[8],
+                                           T (&rb)[8]) {
+  const bool la = (P.flags & GF_A_UNIT_LOWER) != 0, lb = (P.flags & GF_B_UNIT_LOWER) != 0;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    int i, kk;
+    if (!ta) { i = t & 63; kk = (t >> 6) + 4 * q; }
+    else     { kk = t & 31; i = (t >> 5) + 8 * q; }
+    const int gi = row0 + i, gk = k0 + kk;
+    T v = T(0);
+    if (gi < P.m && gk < P.k) {
+      const int p = ta ? gk : gi, qq = ta ? gi : gk;  // stored (row, col)
+      v = gemm_masked(P.A, P.lda, p, qq, ua, la);
+    }
+    ra[q] = v;
+  }
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    int j, kk;
+    if (tb) { j = t & 63; kk = (t >> 6) + 4 * q; }
+    else    { kk = t & 31; j = (t >> 5) + 8 * q; }
+    const int gj = col0 + j, gk = k0 + kk;
+    T v = T(0);
+    if (gj < n && gk < P.k) {
+      const int p = tb ? gj : gk, qq = tb ? gk : gj;
+      v = gemm_masked(P.B, P.ldb, p, qq, ub, lb);
+    }
+    rb[q] = v;
+  }
+}
 
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_gemm(const GemmProb<T>* __restrict__ probs,
                                                    const GemmTask* __restrict__ tasks) {
-  __shared__ T As[GK][GT];
-  __shared__ T Bs[GK][GT];
+  // +1 column of padding: the transposed fills write 32 consecutive k at one i
+  __shared__ T As[GK][GT + 1];
+  __shared__ T Bs[GK][GT + 1];
   const GemmTask tk = tasks[blockIdx.x];
   const GemmProb<T> P = probs[tk.prob];
   if (P.skip && *P.skip) return;
@@ -120,37 +166,21 @@ __global__ __launch_bounds__(kThreads) void k_gemm(const GemmProb<T>* __restrict
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
 
+  // software pipeline: the next chunk's global loads are in flight while the
+  // current chunk is multiplied out of LDS
+  T ra[8], rb[8];
+  gemm_fetch(P, n, row0, col0, 0, ta, tb, ua, ub, ra, rb);
   for (int k0 = 0; k0 < P.k; k0 += GK) {
-    // A tile: As[kk][i] = op(A)(row0+i, k0+kk)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      int i, kk;
-      if (!ta) { i = t & 63; kk = (t >> 6) + 4 * q; }
-      else     { kk = t & 15; i = (t >> 4) + 16 * q; }
-      const int gi = row0 + i, gk = k0 + kk;
-      T v = T(0);
-      if (gi < P.m && gk < P.k) {
-        const int p = ta ? gk : gi, qq = ta ? gi : gk;  // stored (row, col)
-        if (!(ua && p > qq)) v = P.A[p + (size_t)qq * P.lda];
-      }
-      As[kk][i] = v;
-    }
-    // B tile: Bs[kk][j] = op(B)(k0+kk, col0+j)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      int j, kk;
-      if (tb) { j = t & 63; kk = (t >> 6) + 4 * q; }
-      else    { kk = t & 15; j = (t >> 4) + 16 * q; }
-      const int gj = col0 + j, gk = k0 + kk;
-      T v = T(0);
-      if (gj < n && gk < P.k) {
-        const int p = tb ? gj : gk, qq = tb ? gk : gj;
-        if (!(ub && p > qq)) v = P.B[p + (size_t)qq * P.ldb];
-      }
-      Bs[kk][j] = v;
+    for (int q = 0; q < 8; ++q) {
+      if (!ta) As[(t >> 6) + 4 * q][t & 63] = ra[q];
+      else     As[t & 31][(t >> 5) + 8 * q] = ra[q];
+      if (tb) Bs[(t >> 6) + 4 * q][t & 63] = rb[q];
+      else    Bs[t & 31][(t >> 5) + 8 * q] = rb[q];
     }
     __syncthreads();
-#pragma unroll
+    if (k0 + GK < P.k) gemm_fetch(P, n, row0, col0, k0 + GK, ta, tb, ua, ub, ra, rb);
+#pragma unroll 8
     for (int kk = 0; kk < GK; ++kk) {
       T av[4], bv[4];
 #pragma unroll
@@ -165,6 +195,7 @@ __global__ __launch_bounds__(kThreads) void k_gemm(const GemmProb<T>* __restrict
     __syncthreads();
   }
   const bool bz = (P.flags & GF_BETA_ZERO) != 0;
+  const bool c0 = (P.flags & GF_C0) != 0;
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -174,7 +205,11 @@ __global__ __launch_bounds__(kThreads) void k_gemm(const GemmProb<T>* __restrict
         T* c = P.C + i + (size_t)j * P.ldc;
         if (lzu && i < j) { *c = T(0); continue; }
         T v = P.alpha * acc[a][b];
-        if (!bz) v = fma(P.beta, *c, v);
+        if (c0) {
+          if (i < P.m0) v = fma(P.beta, P.C0[i + (size_t)j * P.ldc0], v);
+        } else if (!bz) {
+          v = fma(P.beta, *c, v);
+        }
         *c = v;
       }
     }
@@ -192,27 +227,35 @@ void launch_gemm(const GemmProb<T>* probs, const GemmTask* tasks, int ntasks, hi
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_stack(const StackProb<T>* __restrict__ probs, int nb,
                                                     T alpha, T beta) {
+  // blockIdx.y selects kStackCols consecutive columns of the 2rc+ra output columns
   const StackProb<T> P = probs[blockIdx.x];
   const int ncol = 2 * P.rc + P.ra;
-  for (int col = 0; col < ncol; ++col) {
-    for (int row = threadIdx.x; row < nb; row += kThreads) {
-      if (col < P.rc) {
-        P.Us[row + (size_t)col * nb] = beta * P.CU[row + (size_t)col * P.ldc];
-      } else if (col < P.rc + P.ra) {
-        const int c = col - P.rc;
-        P.Us[row + (size_t)col * nb] = alpha * P.AU[row + (size_t)c * P.lda];
-      } else {
-        const int c = col - P.rc - P.ra;
-        P.Vs[row + (size_t)c * nb] = P.CV[row + (size_t)c * P.ldc];
-      }
+  const int c0 = blockIdx.y * kStackCols;
+  if (c0 >= ncol) return;
+  const int cn = min(kStackCols, ncol - c0);
+  const int total = cn * nb;
+#pragma unroll 4
+  for (int e = threadIdx.x; e < total; e += kThreads) {
+    const int cc = e / nb, row = e - cc * nb;
+    const int col = c0 + cc;
+    if (col < P.rc) {
+      P.Us[row + (size_t)col * nb] = beta * P.CU[row + (size_t)col * P.ldc];
+    } else if (col < P.rc + P.ra) {
+      const int c = col - P.rc;
+      P.Us[row + (size_t)col * nb] = alpha * P.AU[row + (size_t)c * P.lda];
+    } else {
+      const int c = col - P.rc - P.ra;
+      P.Vs[row + (size_t)c * nb] = P.CV[row + (size_t)c * P.ldc];
     }
   }
 }
 
 template <typename T>
-void launch_stack(const StackProb<T>* probs, int n, int nb, T alpha, T beta, hipStream_t s) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_stack<T>, dim3(n), dim3(kThreads), 0, s, probs, nb, alpha, beta);
+void launch_stack(const StackProb<T>* probs, int n, int nb, int maxcols, T alpha, T beta,
+                  hipStream_t s) {
+  if (n <= 0 || maxcols <= 0) return;
+  const int ny = (maxcols + kStackCols - 1) / kStackCols;
+  hipLaunchKernelGGL(k_stack<T>, dim3(n, ny), dim3(kThreads), 0, s, probs, nb, alpha, beta);
 }
 
 // ------------------------------------------------------------------------------
@@ -598,7 +641,7 @@ __device__ void jacobi_sweep(T* A, int ld, int ncol, int K, T tol, T thr2, T* nr
 }
 
 // Householder QR with column pivoting of the K x K matrix A (column-major,
-// leading dimension K), in place, reflectors discarded.  Stops at the first
+// leading dimension ld), in place, reflectors discarded.  Stops at the first
 // step whose largest remaining column norm^2 is <= thr2; returns that step k
 // (the numerical rank).  On return rdiag[c] = R(c,c) and the strict upper
 // triangle of A's first k rows holds R; piv[j] is the original column now in
@@ -606,7 +649,7 @@ __device__ void jacobi_sweep(T* A, int ld, int ncol, int K, T tol, T thr2, T* nr
 // rows (gl + G i, i < RPL, K <= G*RPL) in registers so that all its loads are
 // in flight together; two block barriers per step.
 template <typename T, int NT, int G, int RPL>
-__device__ int qrcp(T* A, int K, T thr2, T* cn, T* rdiag, int* piv, T* gv, int* gi, T* red) {
+__device__ int qrcp(T* A, int K, int ld, T thr2, T* cn, T* rdiag, int* piv, T* gv, int* gi, T* red) {
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   constexpr int NG = NT / G;
   constexpr int NW = NT / 64;
@@ -616,7 +659,7 @@ __device__ int qrcp(T* A, int K, T thr2, T* cn, T* rdiag, int* piv, T* gv, int* 
     T best = T(-1);
     int bi = -1;
     for (int j = grp; j < K; j += NG) {
-      const T* c = A + (size_t)j * K;
+      const T* c = A + (size_t)j * ld;
       T sq = T(0);
 #pragma unroll
       for (int i = 0; i < RPL; ++i) {
@@ -647,8 +690,8 @@ __device__ int qrcp(T* A, int K, T thr2, T* cn, T* rdiag, int* piv, T* gv, int* 
     if (p < 0 || !(best > thr2)) { k = s; break; }
     // swap columns s and p; partial norm^2 of the new column s below row s
     T part = T(0);
-    T* cs_ = A + (size_t)s * K;
-    T* cp_ = A + (size_t)p * K;
+    T* cs_ = A + (size_t)s * ld;
+    T* cp_ = A + (size_t)p * ld;
     for (int i = tid; i < K; i += NT) {
       const T as = cs_[i], ap = cp_[i];
       if (p != s) { cs_[i] = ap; cp_[i] = as; }
@@ -688,7 +731,7 @@ __device__ int qrcp(T* A, int K, T thr2, T* cn, T* rdiag, int* piv, T* gv, int* 
     T gbest = T(-1);
     int gidx = -1;
     for (int j = s + 1 + grp; j < K; j += NG) {
-      T* cj = A + (size_t)j * K;
+      T* cj = A + (size_t)j * ld;
       T x[RPL];
       T dot = T(0);
 #pragma unroll
@@ -723,7 +766,7 @@ __device__ int qrcp(T* A, int K, T thr2, T* cn, T* rdiag, int* piv, T* gv, int* 
 // diagonal) with R^T restricted to k columns: column c < k of the result is
 // row c of R (zeros above the diagonal).
 template <typename T, int NT>
-__device__ void r_transpose(T* A, int K, int k, const T* rdiag) {
+__device__ void r_transpose(T* A, int K, int ld, int k, const T* rdiag) {
   const size_t npairs = (size_t)K * (K - 1) / 2;
   for (size_t e = threadIdx.x; e < npairs; e += NT) {
     // pair e -> (lo, hi), lo < hi, enumerated column by column of the upper triangle
@@ -732,18 +775,18 @@ __device__ void r_transpose(T* A, int K, int k, const T* rdiag) {
     while ((size_t)(hi + 1) * hi / 2 <= e) ++hi;
     const int lo = (int)(e - (size_t)hi * (hi - 1) / 2);
     if (lo >= k) continue;
-    const T v = A[lo + (size_t)hi * K];  // R(lo, hi)
-    A[hi + (size_t)lo * K] = v;          // R^T(hi, lo)
-    A[lo + (size_t)hi * K] = T(0);
+    const T v = A[lo + (size_t)hi * ld];  // R(lo, hi)
+    A[hi + (size_t)lo * ld] = v;          // R^T(hi, lo)
+    A[lo + (size_t)hi * ld] = T(0);
   }
-  for (int c = threadIdx.x; c < k; c += NT) A[c + (size_t)c * K] = rdiag[c];
+  for (int c = threadIdx.x; c < k; c += NT) A[c + (size_t)c * ld] = rdiag[c];
 }
 
 // Shared epilogue: singular values (norms of the k orthogonalised columns of
 // W = R^T V), descending order, truncation (hcore_dgemm.c:444-464) and the
 // left singular vectors U = P W / sigma (rows permuted back by piv).
 template <typename T, int NT>
-__device__ void core_epilogue(const CoreProb<T>& P, const T* W, int k, const int* piv, T* sig,
+__device__ void core_epilogue(const CoreProb<T>& P, const T* W, int ld, int k, const int* piv, T* sig,
                               int* perm, int* flags, int rk, int maxrk, T acc) {
   const int K = P.K;
   const int tid = threadIdx.x;
@@ -752,7 +795,7 @@ __device__ void core_epilogue(const CoreProb<T>& P, const T* W, int k, const int
   for (int j = tid; j < K; j += NT) {
     T s = T(0);
     if (j < k) {
-      const T* cj = W + (size_t)j * K;
+      const T* cj = W + (size_t)j * ld;
       for (int row = 0; row < K; ++row) s = fma(cj[row], cj[row], s);
       s = sqrt(s);
     }
@@ -795,7 +838,7 @@ __device__ void core_epilogue(const CoreProb<T>& P, const T* W, int k, const int
     const int i = e / K, row = e - i * K;
     const int j = perm[i];
     const T sj = sig[j];
-    P.Zu[piv[row] + (size_t)i * K] = (sj > T(0) && j < k) ? W[row + (size_t)j * K] / sj : T(0);
+    P.Zu[piv[row] + (size_t)i * K] = (sj > T(0) && j < k) ? W[row + (size_t)j * ld] / sj : T(0);
   }
 }
 
@@ -832,11 +875,12 @@ __global__ __launch_bounds__(NT) void k_core(const CoreProb<T>* __restrict__ pro
   if (tid == 0) flags[2] = 0;
   block_sync<NT>();
   bool bad = false;
+  const int ld = core_ld(K, G);
   for (int e = tid; e < K * K; e += NT) {
     const int c = e / K, row = e - c * K;
     const T v = P.Tc[c + (size_t)row * K];
     bad |= !(fabs(v) <= T(1e300));
-    A[e] = v;
+    A[row + (size_t)c * ld] = v;
   }
   if (bad) flags[2] = 1;
   block_sync<NT>();
@@ -845,31 +889,31 @@ __global__ __launch_bounds__(NT) void k_core(const CoreProb<T>* __restrict__ pro
     const T thr = T(1e-3) * acc / sqrt(T(K));
     thr2 = thr * thr;
   }
-  const int k = qrcp<T, NT, G, RPL>(A, K, thr2, nrm, rdiag, piv, gv, gi, red);
+  const int k = qrcp<T, NT, G, RPL>(A, K, ld, thr2, nrm, rdiag, piv, gv, gi, red);
   block_sync<NT>();
-  r_transpose<T, NT>(A, K, k, rdiag);
+  r_transpose<T, NT>(A, K, ld, k, rdiag);
   block_sync<NT>();
   const T tol = T(K) * eps_of<T>();
   int sweep = 0;
   if (k > 1) {
     for (; sweep < kMaxSweeps; ++sweep) {
-      column_norms<T, NT, G, RPL>(A, K, k, K, nrm);
+      column_norms<T, NT, G, RPL>(A, ld, k, K, nrm);
       if (tid == 0) flags[0] = 0;
       block_sync<NT>();
-      jacobi_sweep<T, NT, G, RPL>(A, K, k, K, tol, thr2, nrm, &flags[0]);
+      jacobi_sweep<T, NT, G, RPL>(A, ld, k, K, tol, thr2, nrm, &flags[0]);
       const int rotated = flags[0];
       block_sync<NT>();
       if (!rotated) break;
     }
   }
   if (tid == 0 && P.sweeps) *P.sweeps = sweep + 1;
-  core_epilogue<T, NT>(P, A, k, piv, sig, perm, flags, rk, maxrk, acc);
+  core_epilogue<T, NT>(P, A, ld, k, piv, sig, perm, flags, rk, maxrk, acc);
 }
 
 int core_lds_max_k(size_t elem) {
   const size_t lds = 160 * 1024 - 256;
   int k = 1;
-  while (core_head_bytes(elem, k + 1) + elem * (size_t)(k + 1) * (k + 1) <= lds) ++k;
+  while (core_head_bytes(elem, k + 1) + elem * (size_t)(k + 1) * core_ld(k + 1, kCoreLdsG) <= lds) ++k;
   return k;
 }
 
@@ -879,10 +923,634 @@ int core_lds_max_k(size_t elem) {
   X(CORE_K32, 64, 4, 8, true)                           \
   X(CORE_K64, 64, 2, 32, true)                          \
   X(CORE_K128, 256, 4, 32, true)                        \
-  X(CORE_KLDS, 512, 4, 36, true)                        \
+  X(CORE_KLDS, 512, kCoreLdsG, 36, true)                \
   X(CORE_K192, 768, 8, 24, false)                       \
   X(CORE_K256, 1024, 16, 16, false)                     \
   X(CORE_K512, 1024, 32, 16, false)
+
+// ------------------------------------------------------------------------------
+// register-resident core SVD (K <= 64*RPL): the pivoted QR of T^T and the
+// one-sided Jacobi sweeps keep the whole matrix in VGPRs.
+//
+// QRCP: wave w owns the columns j = c*W + w (slot c < CPW) of T^T, lane l the
+// rows l + 64 i.  Per step the owner of the pivot column builds the reflector
+// and hands it to the other waves through LDS; every wave updates its own
+// columns with one batched wave reduction and downdates their norms (exact
+// recomputation once a norm has lost half of its digits).  Two block
+// barriers per step.
+//
+// Jacobi: R^T (K x k) is cut into 2W blocks of B columns; wave w holds two
+// blocks (a, b).  An outer step rotates every pair between its two blocks
+// (B rounds of B disjoint pairs, one batched reduction each, the rotation
+// parameters computed by one lane group per pair), then the blocks move one
+// place along the ring top_1 .. top_{W-1}, bot_{W-1} .. bot_0 (top_0 fixed;
+// the circle method), so that 2W-1 outer steps meet every pair of blocks
+// once.  The pairs inside each block are rotated in the first outer step of a
+// sweep.  Only the block exchange touches LDS.
+// ------------------------------------------------------------------------------
+__host__ __device__ constexpr int pow2_ceil(int n) {
+  int p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+template <typename T>
+__device__ __forceinline__ bool is_finite_val(T v) {
+  return fabs(v) <= std::numeric_limits<T>::max();
+}
+
+// Sum each of d[0..NV) over the 64 lanes; every lane receives all NV totals.
+// xs: the wave's own LDS exchange area (N values).
+template <int N, int NV, typename T>
+__device__ __forceinline__ void wave_sums(T (&d)[NV], T* xs, int lane) {
+  static_assert(NV <= N, "reduction width");
+  T v[N];
+#pragma unroll
+  for (int q = 0; q < N; ++q) v[q] = q < NV ? d[q] : T(0);
+  reduce_scatter<N>(v, lane);
+  if ((lane & ((64 / N) - 1)) == 0) xs[reduce_scatter_index<N>(lane)] = v[0];
+  wave_sync();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) d[q] = xs[q];
+  wave_sync();
+}
+
+// Rotation that orthogonalises two columns with squared norms app, aqq and
+// inner product apq (Hestenes): returns false when the pair is left alone
+// (converged, or a column below the truncation floor thr2).
+template <typename T>
+__device__ __forceinline__ bool jacobi_params(T app, T aqq, T apq, T tol, T thr2, T& cs, T& sn,
+                                              T& t) {
+  cs = T(1);
+  sn = T(0);
+  t = T(0);
+  if (!(fmin(app, aqq) > thr2 && fabs(apq) > tol * sqrt(app * aqq))) return false;
+  const T zeta = (aqq - app) / (T(2) * apq);
+  if (fabs(zeta) > T(1e30)) t = T(0.5) / zeta;
+  else t = (zeta >= T(0) ? T(1) : T(-1)) / (fabs(zeta) + sqrt(fma(zeta, zeta, T(1))));
+  cs = T(1) / sqrt(fma(t, t, T(1)));
+  sn = cs * t;
+  return true;
+}
+
+// Round-robin (circle method) pairing of n players, round t, pair q < n/2.
+__host__ __device__ constexpr int rr_u(int n, int t, int q) {
+  return q == 0 ? 0 : ((q - 1 + t) % (n - 1)) + 1;
+}
+__host__ __device__ constexpr int rr_v(int n, int t, int q) { return ((n - 2 - q + t) % (n - 1)) + 1; }
+
+// Pair q of round t over the wave's 2B columns (0..B-1: block a, B..2B-1:
+// block b).  CROSS: (a_q, b_{(q+t) mod B}); otherwise the round-robin inside
+// each block (pairs q < B/2 in a, the others in b).
+template <bool CROSS, int B>
+__device__ __forceinline__ void jacobi_pair(int t, int q, int& u, int& v) {
+  if (CROSS) {
+    u = q;
+    v = B + (q + t) % B;
+  } else {
+    constexpr int H = B / 2;
+    const int base = q < H ? 0 : B, qq = q < H ? q : q - H;
+    u = base + rr_u(B, t, qq);
+    v = base + rr_v(B, t, qq);
+  }
+}
+
+template <typename T, int B, int RPL>
+__device__ __forceinline__ T& jreg(T (&a)[B][RPL], T (&b)[B][RPL], int idx, int i) {
+  return idx < B ? a[idx][i] : b[idx - B][i];
+}
+
+// One round of B disjoint pairs.  nrm: the wave's 2B squared column norms
+// (LDS), rot: 2B values of LDS scratch, xs: reduction exchange.
+template <bool CROSS, typename T, int B, int RPL>
+__device__ __forceinline__ void jacobi_round(T (&a)[B][RPL], T (&b)[B][RPL], int t, T* nrm, T* rot,
+                                             int lane, T tol, T thr2, int* rotflag) {
+  constexpr int NB = pow2_ceil(B);
+  T d[NB];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    d[q] = T(0);
+    if (q < B) {
+      int u, v;
+      jacobi_pair<CROSS, B>(t, q, u, v);
+#pragma unroll
+      for (int i = 0; i < RPL; ++i) d[q] = fma(jreg(a, b, u, i), jreg(a, b, v, i), d[q]);
+    }
+  }
+  reduce_scatter<NB>(d, lane);
+  const int idx = reduce_scatter_index<NB>(lane);
+  if ((lane & ((64 / NB) - 1)) == 0 && idx < B) {
+    int u, v;
+    jacobi_pair<CROSS, B>(t, idx, u, v);
+    const T app = nrm[u], aqq = nrm[v], apq = d[0];
+    T cs, sn, tt;
+    if (jacobi_params(app, aqq, apq, tol, thr2, cs, sn, tt)) {
+      nrm[u] = fma(-tt, apq, app);
+      nrm[v] = fma(tt, apq, aqq);
+      *rotflag = 1;
+    }
+    rot[2 * idx] = cs;
+    rot[2 * idx + 1] = sn;
+  }
+  wave_sync();
+#pragma unroll
+  for (int q = 0; q < B; ++q) {
+    const T cs = rot[2 * q], sn = rot[2 * q + 1];
+    if (sn != T(0)) {
+      int u, v;
+      jacobi_pair<CROSS, B>(t, q, u, v);
+#pragma unroll
+      for (int i = 0; i < RPL; ++i) {
+        const T xu = jreg(a, b, u, i), xv = jreg(a, b, v, i);
+        jreg(a, b, u, i) = cs * xu - sn * xv;
+        jreg(a, b, v, i) = sn * xu + cs * xv;
+      }
+    }
+  }
+  wave_sync();
+}
+
+// Move the blocks one place along the ring (top_0 stays): through LDS, one
+// 64-row slice at a time.  Squared norms travel with their blocks.
+template <typename T, int W, int B, int RPL>
+__device__ __forceinline__ void block_exchange(T (&a)[B][RPL], T (&b)[B][RPL], T* big, T* nst,
+                                               T* nrm, int w, int lane) {
+  const int posA = w, posB = 2 * W - 1 - w;
+  const int dA = w == 0 ? 0 : posA + 1;
+  const int dB = posB == 2 * W - 1 ? 1 : posB + 1;
+  if (lane < B) {
+    nst[dA * B + lane] = nrm[lane];
+    nst[dB * B + lane] = nrm[B + lane];
+  }
+#pragma unroll
+  for (int i = 0; i < RPL; ++i) {
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      big[(dA * B + q) * 64 + lane] = a[q][i];
+      big[(dB * B + q) * 64 + lane] = b[q][i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      a[q][i] = big[(posA * B + q) * 64 + lane];
+      b[q][i] = big[(posB * B + q) * 64 + lane];
+    }
+    if (i == 0 && lane < B) {
+      nrm[lane] = nst[posA * B + lane];
+      nrm[B + lane] = nst[posB * B + lane];
+    }
+    __syncthreads();
+  }
+}
+
+// Pivot candidate of one wave: the largest remaining squared norm among its
+// unprocessed slots (slot c's norm in lane c), ties to the smallest column.
+template <typename T, int W, int CPW>
+__device__ __forceinline__ void qrcp_candidate(T nl, unsigned long long done_mask, int w, int lane,
+                                               T* cand_v, int* cand_i) {
+  T bv = lane < CPW && !((done_mask >> lane) & 1) ? nl : T(-1);
+  int bi = bv >= T(0) ? lane * W + w : -1;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const T ov = __shfl_xor(bv, off, 64);
+    const int oi = __shfl_xor(bi, off, 64);
+    if (oi >= 0 && (ov > bv || (ov == bv && (bi < 0 || oi < bi)))) {
+      bv = ov;
+      bi = oi;
+    }
+  }
+  if (lane == 0) {
+    cand_v[w] = bv;
+    cand_i[w] = bi;
+  }
+}
+
+template <typename T, int W, int RPL, int CPW, int B>
+struct CoreReg {
+  static constexpr int NT = 64 * W;
+  static constexpr int KM = 64 * RPL;
+  static constexpr int NQ = pow2_ceil(CPW);
+  static constexpr int N2 = pow2_ceil(2 * B);
+  static constexpr int NBLK = 2 * W;
+  static constexpr int XS = NQ > N2 ? NQ : N2;
+  static constexpr int BIG = NBLK * B * 64;
+  static_assert(CPW * W >= KM && NBLK * B >= KM && B % 2 == 0 && N2 <= 64 && NQ <= 64,
+                "register core shape");
+  static constexpr size_t n_elem = (size_t)BIG + 2 * KM + W * XS + 2 * W * B + NBLK * B +
+                                   2 * W * B + 2 * W + 2;
+  static constexpr size_t n_int = 2 * W + 4 * KM + 8;
+  static constexpr size_t smem = n_elem * sizeof(T) + n_int * sizeof(int);
+};
+
+template <typename T, int W, int RPL, int CPW, int B>
+__global__ __launch_bounds__(64 * W) void k_core_reg(const CoreProb<T>* __restrict__ probs, int rk,
+                                                      int maxrk, T acc) {
+  using C = CoreReg<T, W, RPL, CPW, B>;
+  constexpr int NT = C::NT, KM = C::KM;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* big = reinterpret_cast<T*>(smem_raw);  // transposition / block exchange
+  T* vbuf = big + C::BIG;                   // KM: reflector
+  T* sig = vbuf + KM;                       // KM
+  T* xs_all = sig + KM;                     // W x XS
+  T* nrm_all = xs_all + W * C::XS;          // W x 2B block norms
+  T* nst = nrm_all + W * 2 * B;             // NBLK x B norms in transit
+  T* rot_all = nst + C::NBLK * B;           // W x 2B rotations
+  T* cand_v = rot_all + W * 2 * B;          // 2 x W pivot candidates
+  T* misc = cand_v + 2 * W;                 // [0] tau
+  int* cand_i = reinterpret_cast<int*>(misc + 2);  // 2 x W
+  int* piv = cand_i + 2 * W;                       // KM: column at each position
+  int* posv = piv + KM;                            // KM: position of each column
+  int* perm = posv + KM;                           // KM: sorted order
+  int* rnk = perm + KM;                            // KM: sorted position of each column
+  int* flags = rnk + KM;  // [0] r, [1] nonfinite, [2..4] rotated (sweep mod 3)
+
+  const CoreProb<T> P = probs[blockIdx.x];
+  const int K = P.K;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  T* xs = xs_all + w * C::XS;
+  if (tid == 0) {
+    flags[1] = 0;
+    flags[2] = 0;
+    if (P.clk) P.clk[0] = (long long)wall_clock64();
+  }
+  for (int j = tid; j < KM; j += NT) posv[j] = -1;
+  __syncthreads();
+
+  // ---- T^T into registers, exact column norms ----
+  T x[CPW][RPL];
+  bool bad = false;
+#pragma unroll
+  for (int c = 0; c < CPW; ++c) {
+    const int j = c * W + w;
+#pragma unroll
+    for (int i = 0; i < RPL; ++i) {
+      const int row = lane + 64 * i;
+      T v = T(0);
+      if (j < K && row < K) v = P.Tc[j + (size_t)row * K];
+      bad |= !is_finite_val(v);
+      x[c][i] = v;
+    }
+  }
+  if (bad) flags[1] = 1;
+  T thr2 = T(0);
+  if (rk == 0 && acc > T(0)) {
+    const T thr = T(1e-3) * acc / sqrt(T(K));
+    thr2 = thr * thr;
+  }
+  const T tol3z = sqrt(eps_of<T>() * T(2));
+  // Remaining squared norm of slot c lives in lane c (nl; n0: at its last exact
+  // computation); done_mask: processed (or absent) slots, uniform in the wave.
+  unsigned long long done_mask = 0;
+#pragma unroll
+  for (int c = 0; c < CPW; ++c)
+    if (c * W + w >= K) done_mask |= 1ull << c;
+  T nl, n0;
+  {
+    T d[CPW];
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+      d[c] = T(0);
+#pragma unroll
+      for (int i = 0; i < RPL; ++i) d[c] = fma(x[c][i], x[c][i], d[c]);
+    }
+    wave_sums<C::NQ>(d, xs, lane);
+    nl = lane < CPW ? xs[lane] : T(0);
+    n0 = nl;
+    wave_sync();
+    qrcp_candidate<T, W, CPW>(nl, done_mask, w, lane, cand_v, cand_i);
+  }
+  __syncthreads();
+
+  // ---- QR with column pivoting of T^T ----
+  int k = K;
+  for (int s = 0; s < K; ++s) {
+    const int par = s & 1;
+    T best = T(-1);
+    int p = -1;
+    for (int w2 = 0; w2 < W; ++w2) {
+      const T v = cand_v[par * W + w2];
+      const int ii = cand_i[par * W + w2];
+      if (ii >= 0 && (v > best || (v == best && ii < p))) { best = v; p = ii; }
+    }
+    if (p < 0 || !(best > thr2)) { k = s; break; }
+    const int ls = s & 63, is = s >> 6;
+    if (w == p % W) {
+      const int cp = p / W;
+      T col[RPL];
+#pragma unroll
+      for (int i = 0; i < RPL; ++i) col[i] = T(0);
+#pragma unroll
+      for (int c = 0; c < CPW; ++c)
+        if (c == cp) {
+#pragma unroll
+          for (int i = 0; i < RPL; ++i) col[i] = x[c][i];
+        }
+      T part = T(0), sel = T(0);
+#pragma unroll
+      for (int i = 0; i < RPL; ++i) {
+        const int row = lane + 64 * i;
+        if (row > s && row < K) part = fma(col[i], col[i], part);
+        if (i == is) sel = col[i];
+      }
+      const T xn2 = wave_sum(part);
+      const T alpha = __shfl(sel, ls, 64);
+      T beta = alpha, tau = T(0), scal = T(0);
+      if (xn2 > T(0)) {
+        const T nrm = sqrt(fma(alpha, alpha, xn2));
+        beta = alpha >= T(0) ? -nrm : nrm;
+        tau = (beta - alpha) / beta;
+        scal = T(1) / (alpha - beta);
+      }
+#pragma unroll
+      for (int i = 0; i < RPL; ++i) {
+        const int row = lane + 64 * i;
+        vbuf[row] = row > s && row < K ? col[i] * scal : (row == s ? T(1) : T(0));
+      }
+      if (lane == 0) {
+        misc[0] = tau;
+        piv[s] = p;
+      }
+#pragma unroll
+      for (int c = 0; c < CPW; ++c)
+        if (c == cp) {
+#pragma unroll
+          for (int i = 0; i < RPL; ++i) {
+            const int row = lane + 64 * i;
+            if (row == s) x[c][i] = beta;
+            else if (row > s) x[c][i] = T(0);
+          }
+        }
+      done_mask |= 1ull << cp;
+    }
+    __syncthreads();
+    const T tau = misc[0];
+    T v[RPL];
+#pragma unroll
+    for (int i = 0; i < RPL; ++i) v[i] = vbuf[lane + 64 * i];
+    T d[CPW];
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+      d[c] = T(0);
+      if (!((done_mask >> c) & 1)) {
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) d[c] = fma(v[i], x[c][i], d[c]);
+      }
+    }
+    wave_sums<C::NQ>(d, xs, lane);
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+      if (!((done_mask >> c) & 1)) {
+        const T wc = tau * d[c];
+        T sel = T(0);
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) {
+          x[c][i] = fma(-v[i], wc, x[c][i]);
+          if (i == is) sel = x[c][i];
+        }
+        if (lane == ls) xs[c] = sel;  // row s of the slot: R(s, j)
+      }
+    }
+    wave_sync();
+    const bool live = lane < CPW && !((done_mask >> lane) & 1);
+    if (live) {
+      const T rs = xs[lane];
+      nl = fma(-rs, rs, nl);
+    }
+    wave_sync();
+    if (__any(live && n0 > thr2 && nl <= tol3z * n0)) {  // norms lost half their digits
+#pragma unroll
+      for (int c = 0; c < CPW; ++c) {
+        d[c] = T(0);
+        if (!((done_mask >> c) & 1)) {
+#pragma unroll
+          for (int i = 0; i < RPL; ++i) {
+            const int row = lane + 64 * i;
+            if (row > s && row < K) d[c] = fma(x[c][i], x[c][i], d[c]);
+          }
+        }
+      }
+      wave_sums<C::NQ>(d, xs, lane);
+      if (live) {
+        nl = xs[lane];
+        n0 = nl;
+      }
+      wave_sync();
+    }
+    qrcp_candidate<T, W, CPW>(nl, done_mask, w, lane, cand_v + (par ^ 1) * W, cand_i + (par ^ 1) * W);
+    __syncthreads();
+  }
+
+  // positions: the k pivots, then the remaining columns in index order
+  if (tid == 0) {
+    if (P.clk) {
+      P.clk[1] = (long long)wall_clock64();
+      P.clk[4] = k;
+    }
+    for (int s = 0; s < k; ++s) posv[piv[s]] = s;
+    int cnt = k;
+    for (int j = 0; j < K; ++j)
+      if (posv[j] < 0) {
+        piv[cnt] = j;
+        posv[j] = cnt++;
+      }
+  }
+  __syncthreads();
+
+  // ---- R^T (K x k) into the Jacobi blocks, through LDS in chunks ----
+  T a[B][RPL], b[B][RPL];
+#pragma unroll
+  for (int q = 0; q < B; ++q)
+#pragma unroll
+    for (int i = 0; i < RPL; ++i) {
+      a[q][i] = T(0);
+      b[q][i] = T(0);
+    }
+  const int cA = 2 * w * B, cB = (2 * w + 1) * B;  // first column of the home blocks
+  {
+    // R^T(jp, c) = R(c, jp) for c < k, through the update's global scratch
+    // (ld K): written from the QR layout, read back in the block layout, so
+    // the two register sets are never live together
+    T* Rt = P.Tj;
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+      const int j = c * W + w;
+      if (j < K) {
+        const int jp = posv[j];
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) {
+          const int row = lane + 64 * i;
+          if (row < k) Rt[jp + (size_t)row * K] = x[c][i];
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+#pragma unroll
+      for (int i = 0; i < RPL; ++i) {
+        const int row = lane + 64 * i;
+        if (row < K) {
+          if (cA + q < k) a[q][i] = Rt[row + (size_t)(cA + q) * K];
+          if (cB + q < k) b[q][i] = Rt[row + (size_t)(cB + q) * K];
+        }
+      }
+    }
+  }
+
+  // ---- block one-sided Jacobi on R^T ----
+  T* nrm = nrm_all + w * 2 * B;
+  T* rot = rot_all + w * 2 * B;
+  {
+    T d[2 * B];
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      d[q] = T(0);
+      d[B + q] = T(0);
+#pragma unroll
+      for (int i = 0; i < RPL; ++i) {
+        d[q] = fma(a[q][i], a[q][i], d[q]);
+        d[B + q] = fma(b[q][i], b[q][i], d[B + q]);
+      }
+    }
+    wave_sums<C::N2>(d, xs, lane);
+    if (lane < 2 * B) nrm[lane] = xs[lane];
+    wave_sync();
+  }
+  const T tol = T(K) * eps_of<T>();
+  int sweep = 0;
+  if (k > 1) {
+    for (; sweep < kMaxSweeps; ++sweep) {
+      int* rf = &flags[2 + sweep % 3];
+      if (tid == 0) flags[2 + (sweep + 1) % 3] = 0;
+#pragma unroll 1
+      for (int o = 0; o < 2 * W - 1; ++o) {
+        if (o == 0) {
+#pragma unroll
+          for (int t = 0; t < B - 1; ++t)
+            jacobi_round<false>(a, b, t, nrm, rot, lane, tol, thr2, rf);
+        }
+#pragma unroll
+        for (int t = 0; t < B; ++t) jacobi_round<true>(a, b, t, nrm, rot, lane, tol, thr2, rf);
+        block_exchange<T, W, B, RPL>(a, b, big, nst, nrm, w, lane);
+      }
+      if (!*rf) break;
+    }
+  }
+  if (tid == 0) {
+    if (P.sweeps) *P.sweeps = sweep + 1;
+    if (P.clk) P.clk[2] = (long long)wall_clock64();
+  }
+
+  // ---- singular values, order, truncation, left singular vectors ----
+  {
+    T d[2 * B];
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      d[q] = T(0);
+      d[B + q] = T(0);
+#pragma unroll
+      for (int i = 0; i < RPL; ++i) {
+        d[q] = fma(a[q][i], a[q][i], d[q]);
+        d[B + q] = fma(b[q][i], b[q][i], d[B + q]);
+      }
+    }
+    wave_sums<C::N2>(d, xs, lane);
+    if (lane < 2 * B) {
+      const int c = lane < B ? cA + lane : cB + lane - B;
+      if (c < K) {
+        const T s = c < k ? sqrt(xs[lane]) : T(0);
+        sig[c] = s;
+        if (!is_finite_val(s)) flags[1] = 1;
+      }
+    }
+  }
+  if (tid == 0) flags[0] = K;
+  __syncthreads();
+  if (flags[1]) {  // NaN/Inf: the reference's gesvd would fail (info != 0)
+    if (tid == 0) {
+      *P.status = ST_NONFINITE;
+      *P.rank = 0;
+    }
+    return;
+  }
+  for (int j = tid; j < K; j += NT) {  // descending order, ties by index
+    const T sj = sig[j];
+    int pos = 0;
+    for (int i = 0; i < K; ++i) {
+      const T si = sig[i];
+      pos += (si > sj) || (si == sj && i < j);
+    }
+    perm[pos] = j;
+    rnk[j] = pos;
+  }
+  __syncthreads();
+  int r;
+  if (rk != 0) {
+    r = rk < K ? rk : K;
+  } else {  // first i >= 1 with sigma_i < acc (r >= 1)  (hcore_dgemm.c:444-464)
+    for (int i = tid + 1; i < K; i += NT)
+      if (sig[perm[i]] < acc) atomicMin(&flags[0], i);
+    __syncthreads();
+    r = flags[0];
+  }
+  for (int i = tid; i < K; i += NT) P.sigma[i] = sig[perm[i]];
+  if (tid == 0) {
+    *P.rank = r;
+    *P.status = r > maxrk ? ST_R_GT_MAXRK : ST_OK;
+  }
+#pragma unroll
+  for (int q = 0; q < 2 * B; ++q) {
+    const int c = q < B ? cA + q : cB + q - B;
+    if (c < K) {
+      const int i = rnk[c];
+      if (i < r) {
+        const T s = sig[c];
+        const T inv = (s > T(0) && c < k) ? T(1) / s : T(0);
+#pragma unroll
+        for (int ii = 0; ii < RPL; ++ii) {
+          const int row = lane + 64 * ii;
+          if (row < K) {
+            const T val = q < B ? a[q][ii] : b[q - B][ii];
+            P.Zu[piv[row] + (size_t)i * K] = val * inv;
+          }
+        }
+      }
+    }
+  }
+  if (tid == 0 && P.clk) P.clk[3] = (long long)wall_clock64();
+}
+
+// register core variants: (waves, rows per lane, QRCP columns per wave, Jacobi block width)
+#define HCORE_CORE_REG_VARIANTS(X) \
+  X(CORE_R64, 4, 1, 16, 8)         \
+  X(CORE_R128, 8, 2, 16, 8)        \
+  X(CORE_R192, 8, 3, 24, 12)
+
+template <typename T>
+static void launch_core_reg(const CoreProb<T>* probs, int n, int variant, int rk, int maxrk, T acc,
+                            hipStream_t s) {
+  static bool attr = false;
+  if (!attr) {
+#define HCORE_SET_ATTR_R(V, W, RPL, CPW, B)                                                   \
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_core_reg<T, W, RPL, CPW, B>),    \
+                            hipFuncAttributeMaxDynamicSharedMemorySize,                       \
+                            (int)CoreReg<T, W, RPL, CPW, B>::smem);
+    HCORE_CORE_REG_VARIANTS(HCORE_SET_ATTR_R)
+#undef HCORE_SET_ATTR_R
+    attr = true;
+  }
+  if (n <= 0) return;
+  switch (variant) {
+#define HCORE_LAUNCH_R(V, W, RPL, CPW, B)                                                      \
+  case V:                                                                                      \
+    hipLaunchKernelGGL((k_core_reg<T, W, RPL, CPW, B>), dim3(n), dim3(64 * W),                \
+                       (CoreReg<T, W, RPL, CPW, B>::smem), s, probs, rk, maxrk, acc);          \
+    break;
+    HCORE_CORE_REG_VARIANTS(HCORE_LAUNCH_R)
+#undef HCORE_LAUNCH_R
+    default: break;
+  }
+}
 
 template <typename T>
 void launch_core(const CoreProb<T>* probs, int n, int variant, int kmax, int rk, int maxrk, T acc,
@@ -896,12 +1564,16 @@ void launch_core(const CoreProb<T>* probs, int n, int variant, int kmax, int rk,
 #undef HCORE_SET_ATTR
     attr = true;
   }
+  if (variant >= CORE_R64) {
+    launch_core_reg<T>(probs, n, variant, rk, maxrk, acc, s);
+    return;
+  }
   if (n <= 0) return;
   const size_t head = core_head_bytes(sizeof(T), kmax);
   switch (variant) {
 #define HCORE_LAUNCH(V, NT, G, RPL, LDS)                                                     \
   case V: {                                                                                  \
-    const size_t sm = head + (LDS ? sizeof(T) * (size_t)kmax * kmax : 0);                    \
+    const size_t sm = head + (LDS ? sizeof(T) * (size_t)kmax * core_ld(kmax, G) : 0);        \
     hipLaunchKernelGGL((k_core<T, NT, G, RPL, LDS>), dim3(n), dim3(NT), sm, s, probs, kmax,  \
                        rk, maxrk, acc);                                                      \
     break;                                                                                   \
@@ -912,9 +1584,14 @@ void launch_core(const CoreProb<T>* probs, int n, int variant, int kmax, int rk,
   }
 }
 
-int core_variant_for(int K, int kl) {
+int core_variant_for(int K, int kl, bool reg) {
   if (K <= 16) return CORE_K16;
   if (K <= 32) return CORE_K32;
+  if (reg) {
+    if (K <= 64) return CORE_R64;
+    if (K <= 128) return CORE_R128;
+    if (K <= 192) return CORE_R192;
+  }
   if (K <= 64) return CORE_K64;
   if (K <= 128) return CORE_K128;
   if (K <= kl && K <= 144) return CORE_KLDS;
@@ -1060,8 +1737,90 @@ void launch_apply(const ApplyProb<T>* probs, int n, int nb, hipStream_t s) {
 
 // Set the large-LDS attributes of every kernel variant up front, so that no
 // attribute call happens while a stream is being captured into a graph.
+// ------------------------------------------------------------------------------
+// compact-WY solve M = T Y1^T Z by blocked back substitution (K <= kWyMaxK):
+// for the panels b from last to first,
+//   M_b = T_bb (B_b - S(b, >b) M_{>b}),  S = Y^T Y,  B = Y1^T Z (in M on entry),
+// which is M = T B for the forward compact-WY T (T^{-1} = diag(1/tau) +
+// striu(S)).  A workgroup holds 64 columns of M in LDS; wave w computes rows
+// 2w, 2w+1 of each panel, lane j column j.  The rows of S are wave-uniform
+// (scalar loads).
+// ------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_wy_solve(const WySolveProb<T>* __restrict__ probs,
+                                                       int ncb) {
+  static_assert(kPB == 8 && kThreads == 256, "two rows of a panel per wavefront");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const WySolveProb<T> P = probs[blockIdx.x / ncb];
+  if (*P.status != ST_OK) return;
+  const int r = *P.rank;
+  const int j0 = (blockIdx.x % ncb) * 64;
+  if (j0 >= r) return;
+  const int K = P.K;
+  T* Mb = reinterpret_cast<T*>(smem_raw);  // K x 64, row c at Mb[c * 64]
+  T* RH = Mb + (size_t)K * 64;              // kPB x 64 right-hand sides
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int col = j0 + lane;
+  const bool live = col < r;
+  for (int e = threadIdx.x; e < K * 64; e += kThreads) {
+    const int c = e >> 6, j = e & 63;
+    Mb[e] = j0 + j < r ? P.M[c + (size_t)(j0 + j) * K] : T(0);
+  }
+  __syncthreads();
+  const int nblk = (K + kPB - 1) / kPB;
+  for (int b = nblk - 1; b >= 0; --b) {
+    const int base = b * kPB, hi = min(K, base + kPB);
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int c = base + 2 * w + e;
+      if (c < hi) {
+        const T* Sc = P.S + (size_t)c * K;  // S(c2, c) = S(c, c2), c2 > c
+        T acc0 = T(0), acc1 = T(0);
+        int c2 = hi;
+        for (; c2 + 1 < K; c2 += 2) {
+          acc0 = fma(Sc[c2], Mb[c2 * 64 + lane], acc0);
+          acc1 = fma(Sc[c2 + 1], Mb[(c2 + 1) * 64 + lane], acc1);
+        }
+        if (c2 < K) acc0 = fma(Sc[c2], Mb[c2 * 64 + lane], acc0);
+        RH[(2 * w + e) * 64 + lane] = Mb[c * 64 + lane] - (acc0 + acc1);
+      }
+    }
+    __syncthreads();
+    const T* Tb = P.Tp + (size_t)b * kPB * kPB;  // T_bb(a, c) = Tb[a + c * kPB]
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int a = 2 * w + e, c = base + a;
+      if (c < hi) {
+        T v = T(0);
+        for (int cc = a; cc < hi - base; ++cc) v = fma(Tb[a + cc * kPB], RH[cc * 64 + lane], v);
+        Mb[c * 64 + lane] = v;
+      }
+    }
+    __syncthreads();
+  }
+  if (live)
+    for (int c = 0; c < K; ++c) P.M[c + (size_t)col * K] = Mb[c * 64 + lane];
+}
+
+template <typename T>
+void launch_wy_solve(const WySolveProb<T>* probs, int n, int kmax, int rmax, hipStream_t s) {
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wy_solve<T>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)(sizeof(T) * (kWyMaxK + kPB) * 64));
+    attr = true;
+  }
+  if (n <= 0 || rmax <= 0) return;
+  const int ncb = (rmax + 63) / 64;
+  const size_t sm = sizeof(T) * ((size_t)kmax + kPB) * 64;
+  hipLaunchKernelGGL(k_wy_solve<T>, dim3(n * ncb), dim3(kThreads), sm, s, probs, ncb);
+}
+
 template <typename T>
 void prepare_kernels() {
+  launch_wy_solve<T>(nullptr, 0, 1, 1, nullptr);
   launch_geqrf<T>(nullptr, 0, 128, nullptr);
   launch_geqrf<T>(nullptr, 0, 256, nullptr);
   launch_geqrf<T>(nullptr, 0, 512, nullptr);
@@ -1073,15 +1832,17 @@ void prepare_kernels() {
   launch_apply<T>(nullptr, 0, 1024, nullptr);
   launch_apply<T>(nullptr, 0, 2048, nullptr);
   launch_core<T>(nullptr, 0, CORE_K16, 1, 0, 1, T(0), nullptr);
+  launch_core_reg<T>(nullptr, 0, CORE_R64, 0, 1, T(0), nullptr);
 }
 
 // ---- explicit instantiations ---------------------------------------------------
 #define HCORE_INSTANTIATE(T)                                                                  \
   template void launch_gemm<T>(const GemmProb<T>*, const GemmTask*, int, hipStream_t);        \
-  template void launch_stack<T>(const StackProb<T>*, int, int, T, T, hipStream_t);            \
+  template void launch_stack<T>(const StackProb<T>*, int, int, int, T, T, hipStream_t);       \
   template void launch_geqrf<T>(const QrProb<T>*, int, int, hipStream_t);                     \
   template void launch_core<T>(const CoreProb<T>*, int, int, int, int, int, T, hipStream_t);  \
   template void launch_apply<T>(const ApplyProb<T>*, int, int, hipStream_t);                 \
+  template void launch_wy_solve<T>(const WySolveProb<T>*, int, int, int, hipStream_t);        \
   template void prepare_kernels<T>();
 HCORE_INSTANTIATE(double)
 HCORE_INSTANTIATE(float)

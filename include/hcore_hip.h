@@ -92,6 +92,11 @@ int HCORE_hip_plan_get_status(hcore_hip_plan *plan, int *status_host);
 int HCORE_hip_plan_get_lanes(hcore_hip_plan *plan, int *lanes_host);
 /* Copy the number of Jacobi sweeps each update's core SVD used (diagnostic). */
 int HCORE_hip_plan_get_sweeps(hcore_hip_plan *plan, int *sweeps_host);
+/* Core SVD profile (diagnostic): 8 values per update, filled by the
+ * register-resident core kernels (33 <= K <= 192): wall-clock stamps [0] start,
+ * [1] after the pivoted QR, [2] after the Jacobi sweeps, [3] end, and [4] the
+ * numerical rank of the pivoted QR; *tick_ns receives the clock period. */
+int HCORE_hip_plan_get_core_profile(hcore_hip_plan *plan, long long *out, double *tick_ns);
 /* Copy the K singular values of update u (descending, FP64) to host. */
 int HCORE_hip_plan_get_sigma(hcore_hip_plan *plan, int u, double *sigma_host, int *K_out);
 /* Bytes of device workspace held by the plan. */
