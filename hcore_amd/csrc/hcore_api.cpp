@@ -76,6 +76,7 @@ struct Context {
   int policy = HCORE_HIP_ON_ERROR_EXIT;
   int core_kl_d = 0, core_kl_f = 0;
   bool core_reg = true;  // register-resident core SVD for 33 <= K <= 192
+  bool qr_reg = true;    // register-resident Householder QR for K <= kQrRegMaxK
   // grow-only device arena used by the single-call entry points
   char* arena = nullptr;
   size_t arena_cap = 0;
@@ -90,6 +91,9 @@ __attribute__((constructor)) static void policy_from_env() {
   // HCORE_CORE_REG=0 selects the LDS / global-memory core SVD kernels for all K
   const char* reg = getenv("HCORE_CORE_REG");
   if (reg && strcmp(reg, "0") == 0) g_ctx.core_reg = false;
+  // HCORE_QR_REG=0 selects the blocked Householder QR for all stacks
+  const char* qreg = getenv("HCORE_QR_REG");
+  if (qreg && strcmp(qreg, "0") == 0) g_ctx.qr_reg = false;
 }
 
 static int hip_fail(hipError_t e, const char* what) {
@@ -194,7 +198,8 @@ struct hcore_hip_plan {
     int stack_cols = 0;  // largest 2rc+ra of the lane
     int kmax = 0;        // largest stacked rank of the lane
     bool wy = false;     // reassembly by the compact-WY solve (kmax <= kWyMaxK)
-    size_t o_tB = 0, o_wy = 0, o_tX = 0;
+    size_t o_tB = 0, o_wy = 0, o_tX = 0, o_qrr = 0, o_qrp = 0;
+    int n_qrr = 0, qrr_kmax = 0, n_qrp = 0;
     int n_tB = 0, n_wy = 0, n_tX = 0;
     std::vector<CoreLaunch> core;  // one launch per K bucket
     hipStream_t stream = nullptr;  // lanes > 0 (lane 0 uses the caller's stream)
@@ -339,7 +344,8 @@ static int plan_finalize_t(hcore_hip_plan* p) {
       std::vector<WySolveProb<T>> wy;
       int kmax = 0;
       bool wy_lane = false;
-      std::vector<QrProb<T>> qr;
+      std::vector<QrProb<T>> qr, qrr, qrp;  // blocked / register single pass / register panels
+      int qrr_kmax = 0;
       std::vector<std::vector<CoreProb<T>>> core_b;
       std::vector<int> core_kmax;
       std::vector<ApplyProb<T>> ap;
@@ -383,6 +389,8 @@ static int plan_finalize_t(hcore_hip_plan* p) {
         g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_BETA_ZERO;
         g.n_dev = d_rank + u; g.skip = d_status + u;
         add_tasks(d.tZ, (int)gp.size(), g.m, g.n); gp.push_back(g);
+        // register-resident QR (it also yields the Gram S and the panel factors)
+        const bool qreg = lane_wy[l] && g_ctx.qr_reg && K <= kQrRegMaxK && M <= 1024;
         if (lane_wy[l]) {
           T* SA = W + o_S[u];
           T* SB = SA + al((size_t)K * K);
@@ -400,7 +408,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
             g.m = K; g.n = K; g.k = M; g.lda = M; g.ldb = M; g.ldc = K;
             g.alpha = T(1); g.beta = T(0);
             g.flags = GF_A_TRANS | GF_A_UNIT_LOWER | GF_B_UNIT_LOWER | GF_BETA_ZERO;
-            add_tasks(d.tT, (int)gp.size(), g.m, g.n); gp.push_back(g);
+            if (!qreg) { add_tasks(d.tT, (int)gp.size(), g.m, g.n); gp.push_back(g); }
             // B = Y1^T Z (Z = Zu with Zv's launch, Z = Zv after it)
             g = GemmProb<T>{};
             g.A = Ys[side]; g.B = Zs[side]; g.C = Ms[side];
@@ -421,8 +429,18 @@ static int plan_finalize_t(hcore_hip_plan* p) {
           }
         }
         // QR of both stacks
-        d.qr.push_back({Us, W + o_tA[u], W + o_TA[u], K});
-        d.qr.push_back({Vs, W + o_tB[u], W + o_TB[u], K});
+        if (qreg) {  // one register-resident pass (also yields S)
+          T* SA = W + o_S[u];
+          d.qrr.push_back({Us, W + o_tA[u], W + o_TA[u], SA, K});
+          d.qrr.push_back({Vs, W + o_tB[u], W + o_TB[u], SA + al((size_t)K * K), K});
+          d.qrr_kmax = std::max(d.qrr_kmax, K);
+        } else if (g_ctx.qr_reg && M <= 1024) {  // register panels + in-place trailing updates
+          d.qrp.push_back({Us, W + o_tA[u], W + o_TA[u], nullptr, K});
+          d.qrp.push_back({Vs, W + o_tB[u], W + o_TB[u], nullptr, K});
+        } else {  // blocked Householder QR with LDS panels
+          d.qr.push_back({Us, W + o_tA[u], W + o_TA[u], nullptr, K});
+          d.qr.push_back({Vs, W + o_tB[u], W + o_TB[u], nullptr, K});
+        }
         CoreProb<T> c{W + o_Tc[u], global_core(variant[u]) ? W + o_Tj[u] : nullptr, W + o_Zu[u],
                       W + o_sg[u], d_rank + u, d_status + u, d_sweeps + u,
                       d_clk ? d_clk + (size_t)u * kCoreClk : nullptr, K};
@@ -475,6 +493,8 @@ static int plan_finalize_t(hcore_hip_plan* p) {
       L.o_tT = pk.add(d.tT); L.n_tT = (int)d.tT.size();
       L.o_tZ = pk.add(d.tZ); L.n_tZ = (int)d.tZ.size();
       L.o_qr = pk.add(d.qr); L.n_qr = (int)d.qr.size();
+      L.o_qrr = pk.add(d.qrr); L.n_qrr = (int)d.qrr.size(); L.qrr_kmax = d.qrr_kmax;
+      L.o_qrp = pk.add(d.qrp); L.n_qrp = (int)d.qrp.size();
       L.o_apply = pk.add(d.ap); L.n_apply = (int)d.ap.size();
       L.o_tB = pk.add(d.tB); L.n_tB = (int)d.tB.size();
       L.o_tX = pk.add(d.tX); L.n_tX = (int)d.tX.size();
@@ -604,6 +624,8 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
     phase(l, 1, ls, true, [&] { launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(L.o_tF)), L.n_tF, ls); });
     phase(l, 2, ls, true, [&] { launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(L.o_tG)), L.n_tG, ls); });
     phase(l, 3, ls, true, [&] {
+      launch_qr_fused<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qrp)), L.n_qrp, M, ls);
+      launch_qr_reg<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qrr)), L.n_qrr, M, L.qrr_kmax, ls);
       launch_geqrf<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qr)), L.n_qr, M, ls);
     });
     phase(l, 4, ls, true, [&] { launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(L.o_tT)), L.n_tT, ls); });

@@ -69,6 +69,7 @@ struct QrProb {
   T* A;    // nb x K: R on and above the diagonal, reflectors below
   T* tau;  // K
   T* Tp;   // ceil(K/PB) blocks of PB x PB compact-WY factors
+  T* S;    // K x K Gram Y^T Y, lower triangle (single-panel register QR only)
   int K;
 };
 
@@ -139,6 +140,15 @@ void launch_stack(const StackProb<T>* probs, int n, int nb, int maxcols, T alpha
                   hipStream_t s);
 template <typename T>
 void launch_geqrf(const QrProb<T>* probs, int n, int nb, hipStream_t s);
+// register-resident Householder QR: one thread per row (nb <= 1024), K <= kQrRegMaxK
+constexpr int kQrRegMaxK = 32;
+template <typename T>
+void launch_qr_reg(const QrProb<T>* probs, int n, int nb, int kmax, hipStream_t s);
+// whole-stack QR in one workgroup (nb <= 1024): kQrPanel-column panels
+// factored in registers, each applied to the trailing columns in place
+constexpr int kQrPanel = 32;
+template <typename T>
+void launch_qr_fused(const QrProb<T>* probs, int n, int nb, hipStream_t s);
 // core SVD variants by stacked rank K: one wavefront for K <= 64, one
 // workgroup with the core in LDS up to the LDS limit, 1024 threads with the
 // core in global memory beyond it

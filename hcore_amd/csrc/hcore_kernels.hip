@@ -535,6 +535,291 @@ void launch_geqrf(const QrProb<T>* probs, int n, int nb, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------------------
+// register-resident Householder QR of the panel of columns [j0, j0 + KMAX) of
+// an nb x K stack (nb <= NT): thread i holds row i of the panel.  Per column g
+// one fused reduction gives
+//   s_t = sum_{i>g} x_ig x_it  for every panel column t,
+// from which, with row g broadcast through LDS: the norm (t = g), the
+// reflector products w_t = tau (x_gt + s_t / (alpha - beta)) (t > g) and the
+// Gram entries y_t^T v_g (t < g).  Outputs: the panel in LAPACK layout (R on
+// and above the diagonal, reflectors below), tau, the kPB x kPB compact-WY
+// blocks on the diagonal of the panel's factor and the panel's Gram lower
+// triangle (S, when the panel is the whole stack).  Two block barriers per
+// column; the panel is read and written once.
+// ------------------------------------------------------------------------------
+template <int J, typename T, int NT, int KMAX>
+__device__ __forceinline__ void qr_reg_col(T (&x)[KMAX], const QrProb<T>& P, int K, int j0,
+                                           int row, int lane, int wid, T (*part)[KMAX], T* rowj,
+                                           T* wv, T (*Sl)[KMAX + 1], T* taus, T* sc) {
+  constexpr int NW = NT / 64;
+  constexpr int CH = 16;  // values per batched wave reduction
+  const int g = j0 + J;   // global column
+  if (g >= K) return;     // uniform
+  const T xj = row > g ? x[J] : T(0);
+#pragma unroll
+  for (int c0 = 0; c0 < KMAX; c0 += CH) {
+    T sv[CH];
+#pragma unroll
+    for (int q = 0; q < CH; ++q) sv[q] = xj * x[c0 + q];
+    reduce_scatter<CH>(sv, lane);
+    if ((lane & ((64 / CH) - 1)) == 0) part[wid][c0 + reduce_scatter_index<CH>(lane)] = sv[0];
+  }
+  if (row == g) {
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) rowj[k] = x[k];
+  }
+  __syncthreads();
+  if (row < KMAX) {
+    const int t = row;
+    T tot = T(0), ss = T(0);
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      tot += part[w][t];
+      ss += part[w][J];
+    }
+    const T alpha = rowj[J];
+    T beta = alpha, tau = T(0), scal = T(0);
+    if (ss > T(0)) {
+      const T nrm = sqrt(fma(alpha, alpha, ss));
+      beta = alpha >= T(0) ? -nrm : nrm;
+      tau = (beta - alpha) / beta;
+      scal = T(1) / (alpha - beta);
+    }
+    const T dotv = fma(scal, tot, rowj[t]);  // y_t^T v_g (t < J), v_g^T x_t (t > J)
+    wv[t] = (t > J && j0 + t < K) ? tau * dotv : T(0);
+    if (t < J) Sl[J][t] = dotv;
+    if (t == 0) {
+      sc[0] = beta;
+      sc[1] = scal;
+      taus[J] = tau;
+      P.tau[g] = tau;
+    }
+  }
+  __syncthreads();
+  const T beta = sc[0], scal = sc[1];
+  if (row > g) {
+    const T v = x[J] * scal;
+    x[J] = v;
+#pragma unroll
+    for (int k = J + 1; k < KMAX; ++k) x[k] = fma(-v, wv[k], x[k]);
+  } else if (row == g) {
+    x[J] = beta;
+#pragma unroll
+    for (int k = J + 1; k < KMAX; ++k) x[k] -= wv[k];
+  }
+  if constexpr (J + 1 < KMAX)
+    qr_reg_col<J + 1, T, NT, KMAX>(x, P, K, j0, row, lane, wid, part, rowj, wv, Sl, taus, sc);
+}
+
+template <typename T, int NT, int KMAX>
+__global__ __launch_bounds__(NT) void k_qr_reg(const QrProb<T>* __restrict__ probs, int nb, int j0) {
+  constexpr int NW = NT / 64;
+  static_assert(KMAX % 16 == 0 && KMAX <= 64 && NT >= KMAX && KMAX % kPB == 0, "register QR shape");
+  __shared__ T part[NW][KMAX];
+  __shared__ T rowj[KMAX];
+  __shared__ T wv[KMAX];
+  __shared__ T Sl[KMAX][KMAX + 1];  // Sl[j][t] = y_t^T y_j, t < j (panel columns)
+  __shared__ T taus[KMAX];
+  __shared__ T Tf[KMAX][KMAX + 1];  // the panel's compact-WY factor
+  __shared__ T sc[2];               // beta, 1 / (alpha - beta)
+  const QrProb<T> P = probs[blockIdx.x];
+  const int K = P.K;
+  if (j0 >= K) return;
+  const int pw = min(KMAX, K - j0);  // panel width
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int row = tid;
+  T x[KMAX];
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k)
+    x[k] = (row < nb && k < pw) ? P.A[row + (size_t)(j0 + k) * nb] : T(0);
+  qr_reg_col<0, T, NT, KMAX>(x, P, K, j0, row, lane, wid, part, rowj, wv, Sl, taus, sc);
+  if (row < nb && row >= j0) {
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k)
+      if (k < pw) P.A[row + (size_t)(j0 + k) * nb] = x[k];
+  }
+  if (P.S) {  // whole stack in one panel: its Gram lower triangle
+    for (int e = tid; e < K * K; e += NT) {
+      const int jj = e % K, t = e / K;
+      if (t < jj) P.S[jj + (size_t)t * K] = Sl[jj][t];
+    }
+  }
+  // panel factor, column by column: T(:c,c) = -tau_c T(:c,:c) S(:c,c), T(c,c) = tau_c
+  if (wid == 0) {
+    for (int c = 0; c < KMAX; ++c) {
+      if (lane < KMAX) {
+        T val = T(0);
+        if (c < pw) {
+          if (lane < c) {
+            T acc = T(0);
+            for (int b = lane; b < c; ++b) acc = fma(Tf[lane][b], Sl[c][b], acc);
+            val = -taus[c] * acc;
+          } else if (lane == c) {
+            val = taus[c];
+          }
+        }
+        Tf[lane][c] = val;
+      }
+      wave_sync();
+    }
+    // diagonal kPB x kPB blocks (the per-panel factors of the blocked format)
+    for (int e = lane; e < KMAX * kPB; e += 64) {
+      const int blk = e / (kPB * kPB), w2 = e % (kPB * kPB);
+      const int a = w2 % kPB, c = w2 / kPB;
+      if (j0 + blk * kPB < K)
+        P.Tp[(size_t)(j0 / kPB + blk) * kPB * kPB + w2] = Tf[blk * kPB + a][blk * kPB + c];
+    }
+  }
+}
+
+// Whole-stack QR in one workgroup (nb <= NT, any K): the panels of PB columns
+// are factored in registers by the column recursion above; after each panel
+// its compact-WY factor (LDS) updates the trailing columns, kTrailCols at a
+// time: X <- (I - Y T Y^T)^T X = X - Y T^T (Y^T X), with the reflectors kept
+// in the registers that held the panel.  Each thread only ever touches its
+// own row of the stack, so the panels and the trailing columns need no
+// inter-thread ordering through global memory.
+constexpr int kTrailCols = 4;
+template <typename T, int NT, int PB>
+__global__ __launch_bounds__(NT) void k_qr_fused(const QrProb<T>* __restrict__ probs, int nb) {
+  constexpr int NW = NT / 64;
+  constexpr int CB = kTrailCols;
+  constexpr int CH = 16;
+  static_assert(PB % CH == 0 && PB <= 64 && NT >= PB * CB && PB % kPB == 0, "fused QR shape");
+  __shared__ T part[NW][PB];
+  __shared__ T rowj[PB];
+  __shared__ T wv[PB];
+  __shared__ T Sl[PB][PB + 1];
+  __shared__ T taus[PB];
+  __shared__ T Tf[PB][PB + 1];
+  __shared__ T sc[2];
+  __shared__ T tpart[NW][CB][PB];
+  __shared__ T Wm[PB][CB];
+  __shared__ T W2[PB][CB];
+  const QrProb<T> P = probs[blockIdx.x];
+  const int K = P.K;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int row = tid;
+  for (int j0 = 0; j0 < K; j0 += PB) {
+    const int pw = min(PB, K - j0);
+    T x[PB];
+#pragma unroll
+    for (int k = 0; k < PB; ++k)
+      x[k] = (row < nb && k < pw) ? P.A[row + (size_t)(j0 + k) * nb] : T(0);
+    qr_reg_col<0, T, NT, PB>(x, P, K, j0, row, lane, wid, part, rowj, wv, Sl, taus, sc);
+    if (row < nb && row >= j0) {
+#pragma unroll
+      for (int k = 0; k < PB; ++k)
+        if (k < pw) P.A[row + (size_t)(j0 + k) * nb] = x[k];
+    }
+    // panel factor (wave 0), its diagonal kPB blocks to the blocked format
+    if (wid == 0) {
+      for (int c = 0; c < PB; ++c) {
+        if (lane < PB) {
+          T val = T(0);
+          if (c < pw) {
+            if (lane < c) {
+              T acc = T(0);
+              for (int b = lane; b < c; ++b) acc = fma(Tf[lane][b], Sl[c][b], acc);
+              val = -taus[c] * acc;
+            } else if (lane == c) {
+              val = taus[c];
+            }
+          }
+          Tf[lane][c] = val;
+        }
+        wave_sync();
+      }
+      for (int e = lane; e < PB * kPB; e += 64) {
+        const int blk = e / (kPB * kPB), w2 = e % (kPB * kPB);
+        const int a = w2 % kPB, c = w2 / kPB;
+        if (j0 + blk * kPB < K)
+          P.Tp[(size_t)(j0 / kPB + blk) * kPB * kPB + w2] = Tf[blk * kPB + a][blk * kPB + c];
+      }
+    }
+    __syncthreads();
+    if (j0 + PB >= K) break;
+    // the panel's reflectors (unit lower trapezoidal) in place of its columns
+#pragma unroll
+    for (int a = 0; a < PB; ++a) {
+      const int g = j0 + a;
+      x[a] = (a < pw && row < nb) ? (row > g ? x[a] : (row == g ? T(1) : T(0))) : T(0);
+    }
+    for (int c0 = j0 + PB; c0 < K; c0 += CB) {
+      const int cb = min(CB, K - c0);
+      T xc[CB];
+#pragma unroll
+      for (int c = 0; c < CB; ++c)
+        xc[c] = (row < nb && row >= j0 && c < cb) ? P.A[row + (size_t)(c0 + c) * nb] : T(0);
+#pragma unroll
+      for (int c = 0; c < CB; ++c) {
+#pragma unroll
+        for (int a0 = 0; a0 < PB; a0 += CH) {
+          T sv[CH];
+#pragma unroll
+          for (int q = 0; q < CH; ++q) sv[q] = x[a0 + q] * xc[c];
+          reduce_scatter<CH>(sv, lane);
+          if ((lane & ((64 / CH) - 1)) == 0)
+            tpart[wid][c][a0 + reduce_scatter_index<CH>(lane)] = sv[0];
+          __builtin_amdgcn_sched_barrier(0);  // one reduction's registers live at a time
+        }
+      }
+      __syncthreads();
+      if (tid < PB * CB) {
+        const int a = tid % PB, c = tid / PB;
+        T acc = T(0);
+#pragma unroll
+        for (int w = 0; w < NW; ++w) acc += tpart[w][c][a];
+        Wm[a][c] = acc;
+      }
+      __syncthreads();
+      if (tid < PB * CB) {  // W2 = T^T W (T upper triangular)
+        const int a = tid % PB, c = tid / PB;
+        T acc = T(0);
+        for (int b = 0; b <= a; ++b) acc = fma(Tf[b][a], Wm[b][c], acc);
+        W2[a][c] = acc;
+      }
+      __syncthreads();
+      if (row < nb && row >= j0) {
+#pragma unroll
+        for (int c = 0; c < CB; ++c) {
+          T v = xc[c];
+#pragma unroll
+          for (int a = 0; a < PB; ++a) v = fma(-x[a], W2[a][c], v);
+          if (c < cb) P.A[row + (size_t)(c0 + c) * nb] = v;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+void launch_qr_reg(const QrProb<T>* probs, int n, int nb, int kmax, hipStream_t s) {
+  if (n <= 0) return;
+  const bool k16 = kmax <= 16;
+  if (nb <= 256) {
+    if (k16) hipLaunchKernelGGL((k_qr_reg<T, 256, 16>), dim3(n), dim3(256), 0, s, probs, nb, 0);
+    else hipLaunchKernelGGL((k_qr_reg<T, 256, 32>), dim3(n), dim3(256), 0, s, probs, nb, 0);
+  } else if (nb <= 512) {
+    if (k16) hipLaunchKernelGGL((k_qr_reg<T, 512, 16>), dim3(n), dim3(512), 0, s, probs, nb, 0);
+    else hipLaunchKernelGGL((k_qr_reg<T, 512, 32>), dim3(n), dim3(512), 0, s, probs, nb, 0);
+  } else {
+    if (k16) hipLaunchKernelGGL((k_qr_reg<T, 1024, 16>), dim3(n), dim3(1024), 0, s, probs, nb, 0);
+    else hipLaunchKernelGGL((k_qr_reg<T, 1024, 32>), dim3(n), dim3(1024), 0, s, probs, nb, 0);
+  }
+}
+
+template <typename T>
+void launch_qr_fused(const QrProb<T>* probs, int n, int nb, hipStream_t s) {
+  if (n <= 0) return;
+  // 32-column panels; 16 at 1024 threads, whose register budget is 128 VGPRs
+  if (nb <= 256) hipLaunchKernelGGL((k_qr_fused<T, 256, 32>), dim3(n), dim3(256), 0, s, probs, nb);
+  else if (nb <= 512) hipLaunchKernelGGL((k_qr_fused<T, 512, 32>), dim3(n), dim3(512), 0, s, probs, nb);
+  else hipLaunchKernelGGL((k_qr_fused<T, 1024, 16>), dim3(n), dim3(1024), 0, s, probs, nb);
+}
+
+// ------------------------------------------------------------------------------
 // core SVD: one-sided (Hestenes) Jacobi on the columns of T, round-robin
 // ordering, then sort, truncation (hcore_dgemm.c:444-464) and U_r = G_r / sigma
 // ------------------------------------------------------------------------------
@@ -1840,6 +2125,8 @@ void prepare_kernels() {
   template void launch_gemm<T>(const GemmProb<T>*, const GemmTask*, int, hipStream_t);        \
   template void launch_stack<T>(const StackProb<T>*, int, int, int, T, T, hipStream_t);       \
   template void launch_geqrf<T>(const QrProb<T>*, int, int, hipStream_t);                     \
+  template void launch_qr_reg<T>(const QrProb<T>*, int, int, int, hipStream_t);               \
+  template void launch_qr_fused<T>(const QrProb<T>*, int, int, hipStream_t);                  \
   template void launch_core<T>(const CoreProb<T>*, int, int, int, int, int, T, hipStream_t);  \
   template void launch_apply<T>(const ApplyProb<T>*, int, int, hipStream_t);                 \
   template void launch_wy_solve<T>(const WySolveProb<T>*, int, int, int, hipStream_t);        \

@@ -159,19 +159,27 @@ def _trailing_plan(hc, step, out_of_place):
     return p, keep
 
 
-def test_trailing_step_batch_vs_oracle(hc, orc):
-    """A small TLR Cholesky trailing step (8x8 grid, nb=256) in one plan:
-    every update within 10*acc of the dense update and the oracle's rank;
-    every diagonal tile equal to the oracle's dsyrk to 1e-12 relative."""
+@pytest.mark.parametrize("grid", [8, 13])
+def test_trailing_step_batch_vs_oracle(hc, orc, grid):
+    """A small TLR Cholesky trailing step (grid x grid tiles, nb=256) in one
+    plan: every update within 10*acc of the dense update and the oracle's
+    rank; every diagonal tile equal to the oracle's dsyrk to 1e-12 relative.
+    grid 8 (21 updates) runs as one lane; grid 13 (66 updates, stacked ranks
+    up to 187) splits into the three rank lanes, the largest with the panel
+    QR and the register-resident core."""
     from hcore_amd.tiles import TrailingStep
-    step = TrailingStep(grid=8, nb=256, key="gpu-test")
-    _say("case: trailing step grid 8 nb 256, updates", len(step.pairs))
+    step = TrailingStep(grid=grid, nb=256, key="gpu-test")
+    _say("case: trailing step grid", grid, "nb 256, updates", len(step.pairs))
     p, keep = _trailing_plan(hc, step, out_of_place=False)
     panel, CUd, CVd, _, _, Dd = keep
     p.execute()
     hc.device_synchronize()
     ranks = p.ranks()
     assert np.all(p.status() == 0)
+    ra, rb, rc = step.ranks()
+    lanes = p.lanes()
+    if len(step.pairs) > 64:
+        assert len(set(lanes.tolist())) == 3 and np.all(lanes[ra + rc > 128] == 0)
     for u, (i, j) in enumerate(step.pairs):
         AU, AV = step.panel[i]
         BU, BV = step.panel[j]
@@ -191,13 +199,15 @@ def test_trailing_step_batch_vs_oracle(hc, orc):
     p.destroy()
 
 
-def test_graph_replays_match_eager(hc):
+@pytest.mark.parametrize("grid", [6, 13])
+def test_graph_replays_match_eager(hc, grid):
     """The plan captured in a hipGraph and replayed several times (out of
     place; host copies and synchronisation between replays): every replay
-    equals the eager result bit for bit."""
+    equals the eager result bit for bit.  grid 13 captures the fork/join of
+    the three rank lanes over their streams."""
     from hcore_amd.tiles import TrailingStep
-    step = TrailingStep(grid=6, nb=256, key="graph-test", diag=False)
-    _say("case: graph replay grid 6 nb 256")
+    step = TrailingStep(grid=grid, nb=256, key="graph-test", diag=False)
+    _say("case: graph replay grid", grid, "nb 256")
     p, keep = _trailing_plan(hc, step, out_of_place=True)
     _, _, _, outU, outV, _ = keep
     p.execute()
