@@ -35,8 +35,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12         # B/s
 FP64_PEAK = 78.6e12       # FLOP/s (MI355X dense FP64, matrix and vector)
-PHASES = ["stack", "F=AV^T BV", "G=BU F^T", "geqrf", "T=RA RB^T", "core_svd",
-          "reserved", "Zv=T^T Zu", "reassembly", "syrk_W", "syrk_X", "syrk_CD"]
+PHASES = ["stack", "F=AV^T BV", "G=BU F^T", "geqrf", "T=RA RB^T+Gram", "core_svd",
+          "wy_solve", "Zv=T^T Zu", "reassembly", "syrk_W", "syrk_X", "syrk_CD"]
 
 
 def log(*a):
@@ -224,6 +224,16 @@ def main():
     roof["kernel"] = dname
     roof["lane"] = lane_names[dl]
     roof["avg_launch_ms"] = float(phase_ms[dl, dp])
+    # HBM bytes per launch of that kernel from the committed rocprofv3 counter
+    # passes (FETCH_SIZE and WRITE_SIZE, calibrated on the stacking copy kernel)
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tr = json.load(fh)
+        if tr.get("phase") == dname and int(K[sel].min()) > 128 and tr.get("lane") == "K>128":
+            roof["traffic"] = tr["hbm_bytes_per_launch"]
+            roof["traffic_kernel"] = tr["kernel"]
+            roof["traffic_source"] = tr["source"]
 
     # ---- CPU baseline (oracle on the host cores) ------------------------------------
     cpu = None
