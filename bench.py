@@ -49,6 +49,69 @@ def q_flops(m, n):
         int(np.float32(17 * n) / np.float32(3.0))
 
 
+def graph_ms(plan, stream, steps, warmup):
+    """Average step time of a finalized plan over hipGraph replays (HIP events)."""
+    import hcore_amd as hc
+    plan.execute(stream, graph=True)
+    for _ in range(warmup):
+        plan.execute(stream, graph=True)
+    stream.synchronize()
+    e0, e1 = hc.Event(), hc.Event()
+    e0.record(stream)
+    for _ in range(steps):
+        plan.execute(stream, graph=True)
+    e1.record(stream)
+    stream.synchronize()
+    return e0.elapsed_ms(e1) / steps
+
+
+def reduced_precision_leg(step, stream, steps, warmup, acc, sample):
+    """FP64 and mixed-precision plans of the same gemm batch at one accuracy:
+    step times and the worst relative error of sampled updates against the
+    FP64 dense update of the original FP64 tiles."""
+    import hcore_amd as hc
+    nb = step.nb
+    ra, rb, rc = step.ranks()
+    K = ra + rc
+    out = {"acc": acc}
+    for name, prec, dt in (("fp64", hc.FP64, np.float64), ("mixed", hc.MIXED, np.float32)):
+        B = lambda x, cap=None: hc.DeviceBuffer.from_numpy(np.asfortranarray(x.astype(dt)), cap)
+        panel = {i: (B(u), B(v)) for i, (u, v) in step.panel.items()}
+        CUd = [B(c[0], k) for c, k in zip(step.C, K)]
+        CVd = [B(c[1], k) for c, k in zip(step.C, K)]
+        OUd = [hc.DeviceBuffer(nb, k, dt) for k in K]
+        OVd = [hc.DeviceBuffer(nb, k, dt) for k in K]
+        plan = hc.Plan(nb, prec)
+        plan.add_dgemm([panel[i][0].ptr for i, j in step.pairs], [panel[i][1].ptr for i, j in step.pairs],
+                       ra, nb, [panel[j][0].ptr for i, j in step.pairs],
+                       [panel[j][1].ptr for i, j in step.pairs], rb, nb,
+                       [t.ptr for t in CUd], [t.ptr for t in CVd], rc, nb, -1.0, 1.0, 0, 512, acc,
+                       CUout=[t.ptr for t in OUd], CVout=[t.ptr for t in OVd], ldco=nb)
+        plan.finalize()
+        plan.execute(stream)
+        stream.synchronize()
+        assert np.all(plan.status() == 0)
+        ranks = plan.ranks()
+        worst = 0.0
+        for u in sample:
+            i, j = step.pairs[u]
+            AU, AV = step.panel[i]
+            BU, BV = step.panel[j]
+            CU, CV = step.C[u]
+            E = CU @ CV.T - (AU @ AV.T) @ (BU @ BV.T).T
+            r = int(ranks[u])
+            got = OUd[u].numpy(r).astype(np.float64) @ OVd[u].numpy(r).astype(np.float64).T
+            worst = max(worst, float(np.linalg.norm(got - E) / np.linalg.norm(E)))
+        ms = graph_ms(plan, stream, steps, warmup)
+        out[name] = {"ms_per_step": ms, "value": len(step.pairs) / (ms / 1e3),
+                     "worst_rel_err": worst, "mean_rank": float(np.mean(ranks))}
+        plan.destroy()
+        for t in CUd + CVd + OUd + OVd + [x for pr in panel.values() for x in pr]:
+            t.free()
+    out["mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -57,6 +120,7 @@ def main():
     ap.add_argument("--grid", type=int, default=64)
     ap.add_argument("--nb", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the reduced-precision leg")
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--check", type=int, default=24, help="updates checked against the oracle")
     args = ap.parse_args()
@@ -235,6 +299,17 @@ def main():
             roof["traffic_kernel"] = tr["kernel"]
             roof["traffic_source"] = tr["source"]
 
+    # ---- reduced precision: FP64 vs mixed plans of the gemm batch at acc 1e-6 -------
+    # (the mixed plan's FP32 tiles alone carry a relative rounding error of about
+    # 3e-8, so it is measured at the accuracy it can meet)
+    mixed = None
+    if not args.no_mixed:
+        mixed = reduced_precision_leg(step, stream, args.steps, args.warmup, 1e-6, sample)
+        log("acc 1e-6: fp64 %.3f ms (worst %.2e), mixed %.3f ms (worst %.2e), mixed/fp64 %.2fx" %
+            (mixed["fp64"]["ms_per_step"], mixed["fp64"]["worst_rel_err"],
+             mixed["mixed"]["ms_per_step"], mixed["mixed"]["worst_rel_err"],
+             mixed["mixed_over_fp64"]))
+
     # ---- CPU baseline (oracle on the host cores) ------------------------------------
     cpu = None
     if not args.no_cpu:
@@ -291,6 +366,7 @@ def main():
                    "precision": "fp64", "execution": "hipGraph replay of the step",
                    "eager_ms_per_step": eager_ms},
         "roofline": roof,
+        "reduced_precision": mixed,
         "cpu_baseline": cpu,
         "gflops": F_total / (ms_per_step / 1e3) / 1e9,
         "step_roofline_frac": max(F_total / (ms_per_step / 1e3) / FP64_PEAK,

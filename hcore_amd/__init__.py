@@ -22,7 +22,7 @@ INCLUDE_DIR = os.path.join(ROOT, "include")
 # status codes (include/hcore_hip.h)
 OK, ERR_ZERO_RANK, ERR_SUM_RANKS, ERR_K_GT_MAXRK, ERR_R_GT_MAXRK, ERR_NUMERIC, \
     ERR_RUNTIME, ERR_ARG = range(8)
-FP64, FP32 = 0, 1
+FP64, FP32, MIXED = 0, 1, 2
 ON_ERROR_EXIT, ON_ERROR_RETURN = 0, 1
 
 

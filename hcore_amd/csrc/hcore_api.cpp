@@ -115,7 +115,6 @@ static int ctx_init_locked(int device) {
   }
   if (device < 0) HCHECK(hipGetDevice(&device));
   HCHECK(hipSetDevice(device));
-  HCHECK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
   g_ctx.device = device;
   g_ctx.core_kl_d = core_lds_max_k(sizeof(double));
   g_ctx.core_kl_f = core_lds_max_k(sizeof(float));
@@ -123,6 +122,14 @@ static int ctx_init_locked(int device) {
   prepare_kernels<float>();
   g_ctx.ready = true;
   return HCORE_HIP_OK;
+}
+
+// Stream of the single-tile entry points, created on first use: the box
+// exposes four hardware queues per process (HIP's default), and a batched
+// plan's lanes should not share them with an idle library stream.
+static hipStream_t single_stream() {
+  if (!g_ctx.stream) (void)hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking);
+  return g_ctx.stream;
 }
 
 static int ctx_ensure() {
@@ -184,12 +191,14 @@ struct hcore_hip_plan {
   int* d_rank = nullptr;
   int* d_status = nullptr;
   std::vector<size_t> sigma_off;  // per update, element offset into ws
+  bool sigma_double = true;       // sigma stored in FP64
   char* ws = nullptr;
   // the batch is split into lanes by stacked rank; each lane runs its whole
   // pipeline on its own stream so the long large-K chains overlap the rest
   struct CoreLaunch {
     size_t off;
     int count, variant, kmax;
+    bool dbl;  // FP64 core of a mixed-precision plan
   };
   struct Lane {
     int n = 0;
@@ -200,6 +209,8 @@ struct hcore_hip_plan {
     bool wy = false;     // reassembly by the compact-WY solve (kmax <= kWyMaxK)
     size_t o_tB = 0, o_wy = 0, o_tX = 0, o_qrr = 0, o_qrp = 0;
     int n_qrr = 0, qrr_kmax = 0, n_qrp = 0;
+    size_t o_cvin = 0, o_cvout = 0;  // mixed: FP32 <-> FP64 conversions around the core
+    int n_cv = 0, cv_max = 0;
     int n_tB = 0, n_wy = 0, n_tX = 0;
     std::vector<CoreLaunch> core;  // one launch per K bucket
     hipStream_t stream = nullptr;  // lanes > 0 (lane 0 uses the caller's stream)
@@ -209,8 +220,7 @@ struct hcore_hip_plan {
   std::vector<int> lane_of;  // lane index of each update (HCORE_hip_plan_get_lanes)
   size_t o_gp = 0, o_sp = 0, o_tW = 0, o_tX = 0, o_tC = 0;
   int n_tW = 0, n_tX = 0, n_tC = 0;
-  hipStream_t syrk_stream = nullptr;
-  hipEvent_t ev_start = nullptr, ev_syrk = nullptr;
+  hipEvent_t ev_start = nullptr;
   int* d_sweeps = nullptr;
   long long* d_clk = nullptr;  // kCoreClk core stamps per update
   // graph
@@ -233,11 +243,18 @@ struct hcore_hip_plan {
 
 namespace hcore {
 
+// Streams of the lanes after the first (default priority, non-blocking).
+static hipError_t lane_stream_create(hipStream_t* s) {
+  return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
+}
+
 template <typename T>
 static int plan_finalize_t(hcore_hip_plan* p) {
   const int M = p->M;
   const int n = p->ng;
-  const int KL = core_kl<T>();
+  // mixed precision: FP32 everywhere except the core SVD and truncation (FP64)
+  const bool mixed = p->prec == HCORE_HIP_MIXED;
+  const int KL = mixed ? core_kl<double>() : core_kl<T>();
   const size_t es = sizeof(T);
   auto al = [](size_t x) { return (x + 63) & ~size_t(63); };  // elements, 512 B for FP64
 
@@ -300,7 +317,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
 
   // ---- workspace layout (elements of T) ----
   std::vector<size_t> o_Us(n), o_Vs(n), o_tA(n), o_tB(n), o_TA(n), o_TB(n), o_F(n), o_Tc(n),
-      o_Tj(n), o_Zu(n), o_Zv(n), o_sg(n), o_S(n), o_M(n);
+      o_Tj(n), o_Zu(n), o_Zv(n), o_sg(n), o_S(n), o_M(n), o_c64(n);
   size_t off = 0;
   for (int u = 0; u < n; ++u) {
     const int K = Kv[u];
@@ -318,6 +335,10 @@ static int plan_finalize_t(hcore_hip_plan* p) {
     o_S[u] = off; off += 2 * al((size_t)K * K);   // Gram Y^T Y of both stacks
     o_M[u] = off; off += 2 * al((size_t)K * K);   // compact-WY solves of both sides
     o_sg[u] = off; off += al(K);
+    if (mixed) {  // FP64 core: T, scratch, left vectors and sigma, in T-sized elements
+      const size_t d = (size_t)K * K + (size_t)K * (K + 64) + (size_t)K * K + K;
+      o_c64[u] = off; off += al(d * sizeof(double) / es);
+    }
   }
   std::vector<size_t> o_W(p->ns), o_X(p->ns);
   for (int s = 0; s < p->ns; ++s) {
@@ -326,6 +347,14 @@ static int plan_finalize_t(hcore_hip_plan* p) {
     o_X[s] = off; off += al((size_t)M * K);
   }
   p->sigma_off = o_sg;
+  p->sigma_double = sizeof(T) == 8;
+  if (mixed) {  // sigma of the FP64 core, as an offset in doubles
+    for (int u = 0; u < n; ++u) {
+      const int K = Kv[u];
+      p->sigma_off[u] = (o_c64[u] * es) / sizeof(double) + 2 * (size_t)K * K + (size_t)K * (K + 64);
+    }
+    p->sigma_double = true;
+  }
   const size_t ws_elems = off;
 
   // ---- device memory: [rank | status | sweeps | workspace | descriptors] ----
@@ -347,6 +376,9 @@ static int plan_finalize_t(hcore_hip_plan* p) {
       std::vector<QrProb<T>> qr, qrr, qrp;  // blocked / register single pass / register panels
       int qrr_kmax = 0;
       std::vector<std::vector<CoreProb<T>>> core_b;
+      std::vector<std::vector<CoreProb<double>>> core_b64;  // mixed: the FP64 core
+      std::vector<ConvProb> cv_in, cv_out;
+      int cv_max = 0;
       std::vector<int> core_kmax;
       std::vector<ApplyProb<T>> ap;
     };
@@ -356,6 +388,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
       if (lane_members[l].empty()) continue;
       LaneDesc d;
       d.core_b.resize(NV);
+      d.core_b64.resize(NV);
       d.core_kmax.assign(NV, 1);
       for (int u : lane_members[l]) {
         const int K = Kv[u], ra = p->ra[u], rb = p->rb[u], rc = p->rc[u];
@@ -441,11 +474,25 @@ static int plan_finalize_t(hcore_hip_plan* p) {
           d.qr.push_back({Us, W + o_tA[u], W + o_TA[u], nullptr, K});
           d.qr.push_back({Vs, W + o_tB[u], W + o_TB[u], nullptr, K});
         }
-        CoreProb<T> c{W + o_Tc[u], global_core(variant[u]) ? W + o_Tj[u] : nullptr, W + o_Zu[u],
-                      W + o_sg[u], d_rank + u, d_status + u, d_sweeps + u,
-                      d_clk ? d_clk + (size_t)u * kCoreClk : nullptr, K};
         const int b = variant[u];
-        d.core_b[b].push_back(c);
+        if (mixed) {
+          double* c64 = reinterpret_cast<double*>(W + o_c64[u]);
+          double* Tc64 = c64;
+          double* Tj64 = Tc64 + (size_t)K * K;
+          double* Zu64 = Tj64 + (size_t)K * (K + 64);
+          double* sg64 = Zu64 + (size_t)K * K;
+          CoreProb<double> c{Tc64, Tj64, Zu64, sg64, d_rank + u, d_status + u, d_sweeps + u,
+                             d_clk ? d_clk + (size_t)u * kCoreClk : nullptr, K};
+          d.core_b64[b].push_back(c);
+          d.cv_in.push_back({W + o_Tc[u], Tc64, K * K});
+          d.cv_out.push_back({Zu64, W + o_Zu[u], K * K});
+          d.cv_max = std::max(d.cv_max, K * K);
+        } else {
+          CoreProb<T> c{W + o_Tc[u], global_core(variant[u]) ? W + o_Tj[u] : nullptr, W + o_Zu[u],
+                        W + o_sg[u], d_rank + u, d_status + u, d_sweeps + u,
+                        d_clk ? d_clk + (size_t)u * kCoreClk : nullptr, K};
+          d.core_b[b].push_back(c);
+        }
         d.core_kmax[b] = std::max(d.core_kmax[b], K);
         T* CUo = (T*)(p->CUo[u] ? p->CUo[u] : p->CU[u]);
         T* CVo = (T*)(p->CVo[u] ? p->CVo[u] : p->CV[u]);
@@ -503,9 +550,13 @@ static int plan_finalize_t(hcore_hip_plan* p) {
       L.wy = d.wy_lane;
       L.core.clear();
       for (int b = 0; b < NV; ++b) {
-        if (d.core_b[b].empty()) continue;
-        L.core.push_back({pk.add(d.core_b[b]), (int)d.core_b[b].size(), b, d.core_kmax[b]});
+        if (!d.core_b[b].empty())
+          L.core.push_back({pk.add(d.core_b[b]), (int)d.core_b[b].size(), b, d.core_kmax[b], false});
+        if (!d.core_b64[b].empty())
+          L.core.push_back({pk.add(d.core_b64[b]), (int)d.core_b64[b].size(), b, d.core_kmax[b], true});
       }
+      L.o_cvin = pk.add(d.cv_in); L.o_cvout = pk.add(d.cv_out);
+      L.n_cv = (int)d.cv_in.size(); L.cv_max = d.cv_max;
       // largest K first: the longest jobs start first
       std::stable_sort(L.core.begin(), L.core.end(),
                        [](const hcore_hip_plan::CoreLaunch& x, const hcore_hip_plan::CoreLaunch& y) {
@@ -548,18 +599,15 @@ static int plan_finalize_t(hcore_hip_plan* p) {
   p->ws = base + head;
   p->desc = base + head + wsb;
   build(p->ws, p->d_rank, p->d_status);
-  HCHECK(hipMemcpyAsync(p->desc, pk.buf.data(), pk.buf.size(), hipMemcpyHostToDevice,
-                        g_ctx.stream));
-  HCHECK(hipMemsetAsync(base, 0, head, g_ctx.stream));
-  HCHECK(hipStreamSynchronize(g_ctx.stream));
+  HCHECK(hipMemcpy(p->desc, pk.buf.data(), pk.buf.size(), hipMemcpyHostToDevice));
+  HCHECK(hipMemset(base, 0, head));
+  HCHECK(hipDeviceSynchronize());
   for (size_t l = 0; l < p->lanes.size(); ++l) {
     auto& L = p->lanes[l];
-    if (l > 0) HCHECK(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+    if (l > 0) HCHECK(lane_stream_create(&L.stream));
     HCHECK(hipEventCreateWithFlags(&L.ev_done, hipEventDisableTiming));
   }
   HCHECK(hipEventCreateWithFlags(&p->ev_start, hipEventDisableTiming));
-  HCHECK(hipEventCreateWithFlags(&p->ev_syrk, hipEventDisableTiming));
-  if (p->ns > 0) HCHECK(hipStreamCreateWithFlags(&p->syrk_stream, hipStreamNonBlocking));
   p->finalized = true;
   return HCORE_HIP_OK;
 }
@@ -612,6 +660,17 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
     }
   };
   const int nl = (int)p->lanes.size();
+  // dsyrk updates (hcore_dsyrk.c:103, 144, 165-181): on the stream of the last
+  // lane, after its recompressions, so that a plan uses at most three streams
+  // (the caller's and two lanes) next to the null stream: four hardware queues
+  auto syrk = [&](hipStream_t ss) {
+    if (p->ns <= 0) return;
+    const auto* sp = reinterpret_cast<const GemmProb<T>*>(D(p->o_sp));
+    const int lt = hcore_hip_plan::kMaxLanes - 1;
+    phase(lt, 9, ss, true, [&] { launch_gemm<T>(sp, reinterpret_cast<const GemmTask*>(D(p->o_tW)), p->n_tW, ss); });
+    phase(lt, 10, ss, true, [&] { launch_gemm<T>(sp, reinterpret_cast<const GemmTask*>(D(p->o_tX)), p->n_tX, ss); });
+    phase(lt, 11, ss, true, [&] { launch_gemm<T>(sp, reinterpret_cast<const GemmTask*>(D(p->o_tC)), p->n_tC, ss); });
+  };
   (void)hipEventRecord(p->ev_start, s);
   for (int l = 0; l < nl; ++l) {
     auto& L = p->lanes[l];
@@ -632,9 +691,16 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
     // the lane's K buckets, largest first (each bucket a kernel variant with its
     // own LDS size); at most 4 streams per plan, one per hardware queue
     phase(l, 5, ls, !L.core.empty(), [&] {
-      for (const auto& c : L.core)
-        launch_core<T>(reinterpret_cast<const CoreProb<T>*>(D(c.off)), c.count, c.variant,
-                       c.kmax, p->rk, p->maxrk, T(p->acc), ls);
+      launch_convert(reinterpret_cast<const ConvProb*>(D(L.o_cvin)), L.n_cv, L.cv_max, true, ls);
+      for (const auto& c : L.core) {
+        if (c.dbl)
+          launch_core<double>(reinterpret_cast<const CoreProb<double>*>(D(c.off)), c.count,
+                              c.variant, c.kmax, p->rk, p->maxrk, p->acc, ls);
+        else
+          launch_core<T>(reinterpret_cast<const CoreProb<T>*>(D(c.off)), c.count, c.variant,
+                         c.kmax, p->rk, p->maxrk, T(p->acc), ls);
+      }
+      launch_convert(reinterpret_cast<const ConvProb*>(D(L.o_cvout)), L.n_cv, L.cv_max, false, ls);
     });
     phase(l, 7, ls, true, [&] { launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(L.o_tZ)), L.n_tZ, ls); });
     if (L.wy) {
@@ -651,19 +717,10 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
         launch_apply<T>(reinterpret_cast<const ApplyProb<T>*>(D(L.o_apply)), L.n_apply, M, ls);
       });
     }
+    if (l == nl - 1) syrk(ls);  // the dsyrk updates follow the last (bulk) lane
     if (l > 0) (void)hipEventRecord(L.ev_done, ls);
   }
-  if (p->ns > 0) {
-    hipStream_t ss = p->syrk_stream;
-    (void)hipStreamWaitEvent(ss, p->ev_start, 0);
-    const auto* sp = reinterpret_cast<const GemmProb<T>*>(D(p->o_sp));
-    const int ls = hcore_hip_plan::kMaxLanes - 1;
-    phase(ls, 9, ss, true, [&] { launch_gemm<T>(sp, reinterpret_cast<const GemmTask*>(D(p->o_tW)), p->n_tW, ss); });
-    phase(ls, 10, ss, true, [&] { launch_gemm<T>(sp, reinterpret_cast<const GemmTask*>(D(p->o_tX)), p->n_tX, ss); });
-    phase(ls, 11, ss, true, [&] { launch_gemm<T>(sp, reinterpret_cast<const GemmTask*>(D(p->o_tC)), p->n_tC, ss); });
-    (void)hipEventRecord(p->ev_syrk, ss);
-    (void)hipStreamWaitEvent(s, p->ev_syrk, 0);
-  }
+  if (nl == 0) syrk(s);
   for (int l = 1; l < nl; ++l) (void)hipStreamWaitEvent(s, p->lanes[l].ev_done, 0);
   if (tm) p->pending[k] = true;
   HCHECK(hipGetLastError());
@@ -737,7 +794,7 @@ static void single_gemm(int M, T alpha, T* AU, T* AV, T* Ark, int LDA, T* BU, T*
     return;
   }
   std::lock_guard<std::mutex> lk(g_ctx.mu);
-  hipStream_t s = g_ctx.stream;
+  hipStream_t s = single_stream();
   if (K > kCoreMaxK) {
     snprintf(msg, sizeof msg, "%s: stacked rank %d exceeds the supported maximum %d", name, K,
              kCoreMaxK);
@@ -842,7 +899,7 @@ static int run_gemm_chain(const std::vector<GemmProb<double>>& gp) {
   }
   char* d = nullptr;
   HCHECK(hipMalloc(&d, pk.buf.size() + 256));
-  hipStream_t s = g_ctx.stream;
+  hipStream_t s = single_stream();
   HCHECK(hipMemcpyAsync(d, pk.buf.data(), pk.buf.size(), hipMemcpyHostToDevice, s));
   for (size_t i = 0; i < gp.size(); ++i)
     launch_gemm<double>(reinterpret_cast<const GemmProb<double>*>(d + o_gp),
@@ -984,11 +1041,11 @@ int HCORE_hip_init(int device) {
 int HCORE_hip_finalize(void) {
   std::lock_guard<std::mutex> lk(g_ctx.mu);
   if (!g_ctx.ready) return HCORE_HIP_OK;
-  (void)hipStreamSynchronize(g_ctx.stream);
+  if (g_ctx.stream) (void)hipStreamSynchronize(g_ctx.stream);
   if (g_ctx.arena) (void)hipFree(g_ctx.arena);
   g_ctx.arena = nullptr;
   g_ctx.arena_cap = 0;
-  (void)hipStreamDestroy(g_ctx.stream);
+  if (g_ctx.stream) (void)hipStreamDestroy(g_ctx.stream);
   g_ctx.stream = nullptr;
   g_ctx.ready = false;
   return HCORE_HIP_OK;
@@ -1006,7 +1063,8 @@ int HCORE_hip_last_status(void) { return t_last_status; }
 // ---- plans -----------------------------------------------------------------------
 int HCORE_hip_plan_create(hcore_hip_plan** plan, int M, int precision) {
   if (!plan || M <= 0) return HCORE_HIP_ERR_ARG;
-  if (precision != HCORE_HIP_FP64 && precision != HCORE_HIP_FP32) return HCORE_HIP_ERR_ARG;
+  if (precision != HCORE_HIP_FP64 && precision != HCORE_HIP_FP32 && precision != HCORE_HIP_MIXED)
+    return HCORE_HIP_ERR_ARG;
   int st = ctx_ensure();
   if (st) return st;
   hcore_hip_plan* p = new hcore_hip_plan();
@@ -1137,7 +1195,7 @@ int HCORE_hip_plan_get_sigma(hcore_hip_plan* p, int u, double* out, int* K_out) 
   if (K_out) *K_out = K;
   if (!out) return HCORE_HIP_OK;
   HCHECK(hipDeviceSynchronize());
-  if (p->prec == HCORE_HIP_FP64) {
+  if (p->sigma_double) {
     HCHECK(hipMemcpy(out, reinterpret_cast<double*>(p->ws) + p->sigma_off[u], sizeof(double) * K,
                      hipMemcpyDeviceToHost));
   } else {
@@ -1185,9 +1243,7 @@ void HCORE_hip_plan_destroy(hcore_hip_plan* p) {
     if (L.stream) (void)hipStreamDestroy(L.stream);
     if (L.ev_done) (void)hipEventDestroy(L.ev_done);
   }
-  if (p->syrk_stream) (void)hipStreamDestroy(p->syrk_stream);
   if (p->ev_start) (void)hipEventDestroy(p->ev_start);
-  if (p->ev_syrk) (void)hipEventDestroy(p->ev_syrk);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   if (p->dmem) {

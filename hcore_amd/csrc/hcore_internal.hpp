@@ -103,6 +103,13 @@ struct WySolveProb {
   int K;
 };
 
+// ---- precision conversion of one K x K core (mixed-precision plans) ---------
+struct ConvProb {
+  const void* src;
+  void* dst;
+  int n;  // elements
+};
+
 // ---- reassembly: X = Q [Z; 0] written into the caller's factor ---------------
 template <typename T>
 struct ApplyProb {
@@ -168,6 +175,9 @@ void launch_apply(const ApplyProb<T>* probs, int n, int nb, hipStream_t s);
 constexpr int kWyMaxK = 256;
 template <typename T>
 void launch_wy_solve(const WySolveProb<T>* probs, int n, int kmax, int rmax, hipStream_t s);
+
+// batched element-wise conversion FP32 -> FP64 (to_double) or FP64 -> FP32
+void launch_convert(const ConvProb* probs, int n, int nmax, bool to_double, hipStream_t s);
 
 template <typename T>
 void prepare_kernels();

@@ -32,39 +32,110 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+// Lane exchanges without the LDS crossbar.  All of them require a full wave
+// (every lane active), which is how every caller below uses them.
+//   offsets 1, 2: DPP quad_perm; 4: DPP row_half_mirror (lane i <-> 7 - i in
+//   each group of 8, which flips bit 2 and keeps bits 3..5); 8: DPP row_ror:8;
+//   16, 32: v_permlane16_swap / v_permlane32_swap (gfx950).
+template <int OFF>
+__device__ __forceinline__ unsigned dpp_partner(unsigned v) {
+  static_assert(OFF == 1 || OFF == 2 || OFF == 4 || OFF == 8, "DPP partner offset");
+  if constexpr (OFF == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xf, 0xf, false);
+  else if constexpr (OFF == 2) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xf, 0xf, false);
+  else if constexpr (OFF == 4) return __builtin_amdgcn_mov_dpp(v, 0x141, 0xf, 0xf, false);
+  else return __builtin_amdgcn_mov_dpp(v, 0x128, 0xf, 0xf, false);
 }
 
-// Sum of a group of G consecutive lanes (G a power of two, groups aligned).
+// (x, y) -> in lanes whose bit OFF is clear: x + x(partner); where it is set:
+// y + y(partner), for OFF = 16 or 32: one permlane swap per 32-bit word.
+template <int OFF>
+__device__ __forceinline__ float swap_sum(float x, float y) {
+  static_assert(OFF == 16 || OFF == 32, "swap offset");
+  const unsigned xb = __float_as_uint(x), yb = __float_as_uint(y);
+  if constexpr (OFF == 32) {
+    const auto r = __builtin_amdgcn_permlane32_swap(xb, yb, false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  } else {
+    const auto r = __builtin_amdgcn_permlane16_swap(xb, yb, false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  }
+}
+template <int OFF>
+__device__ __forceinline__ double swap_sum(double x, double y) {
+  static_assert(OFF == 16 || OFF == 32, "swap offset");
+  const unsigned long long xb = __double_as_longlong(x), yb = __double_as_longlong(y);
+  const unsigned xl = (unsigned)xb, xh = (unsigned)(xb >> 32);
+  const unsigned yl = (unsigned)yb, yh = (unsigned)(yb >> 32);
+  unsigned a0, a1, b0, b1;
+  if constexpr (OFF == 32) {
+    const auto rl = __builtin_amdgcn_permlane32_swap(xl, yl, false, false);
+    const auto rh = __builtin_amdgcn_permlane32_swap(xh, yh, false, false);
+    a0 = rl[0]; b0 = rl[1]; a1 = rh[0]; b1 = rh[1];
+  } else {
+    const auto rl = __builtin_amdgcn_permlane16_swap(xl, yl, false, false);
+    const auto rh = __builtin_amdgcn_permlane16_swap(xh, yh, false, false);
+    a0 = rl[0]; b0 = rl[1]; a1 = rh[0]; b1 = rh[1];
+  }
+  const double xn = __longlong_as_double(((unsigned long long)a1 << 32) | a0);
+  const double yn = __longlong_as_double(((unsigned long long)b1 << 32) | b0);
+  return xn + yn;
+}
+
+// Value of the partner lane at offset OFF <= 8 (see dpp_partner).
+template <int OFF>
+__device__ __forceinline__ float lane_partner(float v) {
+  return __uint_as_float(dpp_partner<OFF>(__float_as_uint(v)));
+}
+template <int OFF>
+__device__ __forceinline__ double lane_partner(double v) {
+  const unsigned long long b = __double_as_longlong(v);
+  const unsigned lo = dpp_partner<OFF>((unsigned)b), hi = dpp_partner<OFF>((unsigned)(b >> 32));
+  return __longlong_as_double(((unsigned long long)hi << 32) | lo);
+}
+
+// v + v(partner at OFF), for any OFF = 1..32.
+template <int OFF, typename T>
+__device__ __forceinline__ T add_partner(T v) {
+  if constexpr (OFF >= 16) return swap_sum<OFF>(v, v);
+  else return v + lane_partner<OFF>(v);
+}
+
+template <int OFF, typename T>
+__device__ __forceinline__ T butterfly_sum(T v) {
+  if constexpr (OFF >= 1) return butterfly_sum<OFF / 2>(add_partner<OFF>(v));
+  else return v;
+}
+
 template <typename T>
-__device__ __forceinline__ T group_sum(T v, int g) {
-  for (int off = g >> 1; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+__device__ __forceinline__ T wave_sum(T v) {
+  return butterfly_sum<32>(v);
 }
 
 // Reduce N values (N a power of two <= 64) across the 64 lanes: each exchange
 // step hands half of the remaining values to the partner lane, so the cost is
-// about N + 6 shuffles instead of 6N.  On return v[0] holds the wave total of
+// about N + 6 exchanges instead of 6N.  On return v[0] holds the wave total of
 // value index reduce_scatter_index<N>(lane).  The halving is written as a
 // compile-time recursion so every array index is a constant (no scratch).
 template <int HALF, int OFF, int N, typename T>
 __device__ __forceinline__ void rs_level(T (&v)[N], int lane) {
   if constexpr (HALF >= 1) {
-    const bool upper = (lane & OFF) != 0;
+    if constexpr (OFF >= 16) {
+      // lanes with bit OFF set keep the upper half: the permlane swap of
+      // (v[i], v[i + HALF]) delivers both partners' halves at once
 #pragma unroll
-    for (int i = 0; i < HALF; ++i) {
-      const T send = upper ? v[i] : v[i + HALF];
-      const T keep = upper ? v[i + HALF] : v[i];
-      v[i] = keep + __shfl_xor(send, OFF, 64);
+      for (int i = 0; i < HALF; ++i) v[i] = swap_sum<OFF>(v[i], v[i + HALF]);
+    } else {
+      const bool upper = (lane & OFF) != 0;
+#pragma unroll
+      for (int i = 0; i < HALF; ++i) {
+        const T send = upper ? v[i] : v[i + HALF];
+        const T keep = upper ? v[i + HALF] : v[i];
+        v[i] = keep + lane_partner<OFF>(send);
+      }
     }
     rs_level<HALF / 2, OFF / 2>(v, lane);
   } else {
-#pragma unroll
-    for (int off = OFF; off >= 1; off >>= 1) v[0] += __shfl_xor(v[0], off, 64);
+    v[0] = butterfly_sum<OFF>(v[0]);
   }
 }
 template <int N, typename T>
@@ -588,11 +659,10 @@ __device__ __forceinline__ void qr_reg_col(T (&x)[KMAX], const QrProb<T>& P, int
     const T dotv = fma(scal, tot, rowj[t]);  // y_t^T v_g (t < J), v_g^T x_t (t > J)
     wv[t] = (t > J && j0 + t < K) ? tau * dotv : T(0);
     if (t < J) Sl[J][t] = dotv;
-    if (t == 0) {
+    if (t == 0) {  // no global store here: the next barrier would wait for it
       sc[0] = beta;
       sc[1] = scal;
       taus[J] = tau;
-      P.tau[g] = tau;
     }
   }
   __syncthreads();
@@ -633,6 +703,7 @@ __global__ __launch_bounds__(NT) void k_qr_reg(const QrProb<T>* __restrict__ pro
   for (int k = 0; k < KMAX; ++k)
     x[k] = (row < nb && k < pw) ? P.A[row + (size_t)(j0 + k) * nb] : T(0);
   qr_reg_col<0, T, NT, KMAX>(x, P, K, j0, row, lane, wid, part, rowj, wv, Sl, taus, sc);
+  if (row < pw) P.tau[j0 + row] = taus[row];
   if (row < nb && row >= j0) {
 #pragma unroll
     for (int k = 0; k < KMAX; ++k)
@@ -674,16 +745,16 @@ __global__ __launch_bounds__(NT) void k_qr_reg(const QrProb<T>* __restrict__ pro
 
 // Whole-stack QR in one workgroup (nb <= NT, any K): the panels of PB columns
 // are factored in registers by the column recursion above; after each panel
-// its compact-WY factor (LDS) updates the trailing columns, kTrailCols at a
-// time: X <- (I - Y T Y^T)^T X = X - Y T^T (Y^T X), with the reflectors kept
-// in the registers that held the panel.  Each thread only ever touches its
-// own row of the stack, so the panels and the trailing columns need no
+// its compact-WY factor (LDS) updates the trailing columns, four at a time
+// (three block barriers per step):
+// X <- (I - Y T Y^T)^T X = X - Y T^T (Y^T X), with the reflectors kept in
+// the registers that held the panel.  Each thread only ever touches its own
+// row of the stack, so the panels and the trailing columns need no
 // inter-thread ordering through global memory.
-constexpr int kTrailCols = 4;
 template <typename T, int NT, int PB>
 __global__ __launch_bounds__(NT) void k_qr_fused(const QrProb<T>* __restrict__ probs, int nb) {
   constexpr int NW = NT / 64;
-  constexpr int CB = kTrailCols;
+  constexpr int CB = 4;
   constexpr int CH = 16;
   static_assert(PB % CH == 0 && PB <= 64 && NT >= PB * CB && PB % kPB == 0, "fused QR shape");
   __shared__ T part[NW][PB];
@@ -707,6 +778,7 @@ __global__ __launch_bounds__(NT) void k_qr_fused(const QrProb<T>* __restrict__ p
     for (int k = 0; k < PB; ++k)
       x[k] = (row < nb && k < pw) ? P.A[row + (size_t)(j0 + k) * nb] : T(0);
     qr_reg_col<0, T, NT, PB>(x, P, K, j0, row, lane, wid, part, rowj, wv, Sl, taus, sc);
+    if (row < pw) P.tau[j0 + row] = taus[row];
     if (row < nb && row >= j0) {
 #pragma unroll
       for (int k = 0; k < PB; ++k)
@@ -834,9 +906,7 @@ __device__ __forceinline__ void block_sync() {
 // Sum over a group of G consecutive lanes (G a power of two <= 64).
 template <int G, typename T>
 __device__ __forceinline__ T gsum(T v) {
-#pragma unroll
-  for (int off = G / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  return butterfly_sum<G / 2>(v);
 }
 
 // Squared column norms of the ncol columns of A (K rows, leading dim ld):
@@ -1260,6 +1330,18 @@ __device__ __forceinline__ void wave_sums(T (&d)[NV], T* xs, int lane) {
   wave_sync();
 }
 
+// Hestenes rotation angle for squared norms app, aqq and inner product apq:
+// the smaller of the two angles (|t| <= 1).
+template <typename T>
+__device__ __forceinline__ bool jacobi_angle(T app, T aqq, T apq, T& cs, T& sn, T& t) {
+  const T zeta = (aqq - app) / (T(2) * apq);
+  if (fabs(zeta) > T(1e30)) t = T(0.5) / zeta;
+  else t = (zeta >= T(0) ? T(1) : T(-1)) / (fabs(zeta) + sqrt(fma(zeta, zeta, T(1))));
+  cs = T(1) / sqrt(fma(t, t, T(1)));
+  sn = cs * t;
+  return true;
+}
+
 // Rotation that orthogonalises two columns with squared norms app, aqq and
 // inner product apq (Hestenes): returns false when the pair is left alone
 // (converged, or a column below the truncation floor thr2).
@@ -1270,12 +1352,22 @@ __device__ __forceinline__ bool jacobi_params(T app, T aqq, T apq, T tol, T thr2
   sn = T(0);
   t = T(0);
   if (!(fmin(app, aqq) > thr2 && fabs(apq) > tol * sqrt(app * aqq))) return false;
-  const T zeta = (aqq - app) / (T(2) * apq);
-  if (fabs(zeta) > T(1e30)) t = T(0.5) / zeta;
-  else t = (zeta >= T(0) ? T(1) : T(-1)) / (fabs(zeta) + sqrt(fma(zeta, zeta, T(1))));
-  cs = T(1) / sqrt(fma(t, t, T(1)));
-  sn = cs * t;
-  return true;
+  return jacobi_angle(app, aqq, apq, cs, sn, t);
+}
+
+// Same, also folding the pair's relative off-diagonal |apq| / sqrt(app aqq)
+// (pairs above the floor) into the running maximum rho.
+template <typename T>
+__device__ __forceinline__ bool jacobi_params(T app, T aqq, T apq, T tol, T thr2, T& cs, T& sn,
+                                              T& t, float& rho) {
+  cs = T(1);
+  sn = T(0);
+  t = T(0);
+  if (!(fmin(app, aqq) > thr2)) return false;
+  const T nn = sqrt(app * aqq);
+  if (nn > T(0)) rho = fmaxf(rho, (float)fabs(apq) / (float)nn);
+  if (!(fabs(apq) > tol * nn)) return false;
+  return jacobi_angle(app, aqq, apq, cs, sn, t);
 }
 
 // Round-robin (circle method) pairing of n players, round t, pair q < n/2.
@@ -1306,10 +1398,12 @@ __device__ __forceinline__ T& jreg(T (&a)[B][RPL], T (&b)[B][RPL], int idx, int 
 }
 
 // One round of B disjoint pairs.  nrm: the wave's 2B squared column norms
-// (LDS), rot: 2B values of LDS scratch, xs: reduction exchange.
+// (LDS), rot: 2B values of LDS scratch.  The lanes that hold a pair's reduced
+// inner product compute its rotation and keep the largest relative
+// off-diagonal |a_pq| / sqrt(a_pp a_qq) they have seen in rho.
 template <bool CROSS, typename T, int B, int RPL>
 __device__ __forceinline__ void jacobi_round(T (&a)[B][RPL], T (&b)[B][RPL], int t, T* nrm, T* rot,
-                                             int lane, T tol, T thr2, int* rotflag) {
+                                             int lane, T tol, T thr2, int* rotflag, float& rho) {
   constexpr int NB = pow2_ceil(B);
   T d[NB];
 #pragma unroll
@@ -1329,7 +1423,7 @@ __device__ __forceinline__ void jacobi_round(T (&a)[B][RPL], T (&b)[B][RPL], int
     jacobi_pair<CROSS, B>(t, idx, u, v);
     const T app = nrm[u], aqq = nrm[v], apq = d[0];
     T cs, sn, tt;
-    if (jacobi_params(app, aqq, apq, tol, thr2, cs, sn, tt)) {
+    if (jacobi_params(app, aqq, apq, tol, thr2, cs, sn, tt, rho)) {
       nrm[u] = fma(-tt, apq, app);
       nrm[v] = fma(tt, apq, aqq);
       *rotflag = 1;
@@ -1447,7 +1541,7 @@ __global__ __launch_bounds__(64 * W) void k_core_reg(const CoreProb<T>* __restri
   int* posv = piv + KM;                            // KM: position of each column
   int* perm = posv + KM;                           // KM: sorted order
   int* rnk = perm + KM;                            // KM: sorted position of each column
-  int* flags = rnk + KM;  // [0] r, [1] nonfinite, [2..4] rotated (sweep mod 3)
+  int* flags = rnk + KM;  // [0] r, [1] nonfinite, [2..4] rotated, [5..7] max rho (sweep mod 3)
 
   const CoreProb<T> P = probs[blockIdx.x];
   const int K = P.K;
@@ -1456,6 +1550,7 @@ __global__ __launch_bounds__(64 * W) void k_core_reg(const CoreProb<T>* __restri
   if (tid == 0) {
     flags[1] = 0;
     flags[2] = 0;
+    flags[5] = 0;
     if (P.clk) P.clk[0] = (long long)wall_clock64();
   }
   for (int j = tid; j < KM; j += NT) posv[j] = -1;
@@ -1701,28 +1796,46 @@ __global__ __launch_bounds__(64 * W) void k_core_reg(const CoreProb<T>* __restri
     if (lane < 2 * B) nrm[lane] = xs[lane];
     wave_sync();
   }
+  // Sweeps until no pair rotates, or until a sweep started with every
+  // relative off-diagonal below sqrt(tol): one-sided Jacobi converges
+  // quadratically, so that sweep's rotations leave them near tol.
   const T tol = T(K) * eps_of<T>();
+  const float rho_stop = (float)sqrt(tol);
   int sweep = 0;
   if (k > 1) {
     for (; sweep < kMaxSweeps; ++sweep) {
       int* rf = &flags[2 + sweep % 3];
-      if (tid == 0) flags[2 + (sweep + 1) % 3] = 0;
+      unsigned* rm = reinterpret_cast<unsigned*>(&flags[5 + sweep % 3]);
+      float rho = 0.f;  // this lane's largest relative off-diagonal of the sweep
+      if (tid == 0) {
+        flags[2 + (sweep + 1) % 3] = 0;
+        flags[5 + (sweep + 1) % 3] = 0;
+      }
 #pragma unroll 1
       for (int o = 0; o < 2 * W - 1; ++o) {
         if (o == 0) {
 #pragma unroll
           for (int t = 0; t < B - 1; ++t)
-            jacobi_round<false>(a, b, t, nrm, rot, lane, tol, thr2, rf);
+            jacobi_round<false>(a, b, t, nrm, rot, lane, tol, thr2, rf, rho);
         }
 #pragma unroll
-        for (int t = 0; t < B; ++t) jacobi_round<true>(a, b, t, nrm, rot, lane, tol, thr2, rf);
+        for (int t = 0; t < B; ++t)
+          jacobi_round<true>(a, b, t, nrm, rot, lane, tol, thr2, rf, rho);
+        if (o == 2 * W - 2) {  // publish the sweep's maximum before the last exchange
+#pragma unroll
+          for (int off = 32; off >= 1; off >>= 1) rho = fmaxf(rho, __shfl_xor(rho, off, 64));
+          if (lane == 0) atomicMax(rm, __float_as_uint(rho));
+        }
         block_exchange<T, W, B, RPL>(a, b, big, nst, nrm, w, lane);
       }
-      if (!*rf) break;
+      if (!*rf || __uint_as_float(*rm) < rho_stop) {
+        ++sweep;
+        break;
+      }
     }
   }
   if (tid == 0) {
-    if (P.sweeps) *P.sweeps = sweep + 1;
+    if (P.sweeps) *P.sweeps = sweep;
     if (P.clk) P.clk[2] = (long long)wall_clock64();
   }
 
@@ -2101,6 +2214,28 @@ void launch_wy_solve(const WySolveProb<T>* probs, int n, int kmax, int rmax, hip
   const int ncb = (rmax + 63) / 64;
   const size_t sm = sizeof(T) * ((size_t)kmax + kPB) * 64;
   hipLaunchKernelGGL(k_wy_solve<T>, dim3(n * ncb), dim3(kThreads), sm, s, probs, ncb);
+}
+
+// ------------------------------------------------------------------------------
+// element-wise precision conversion, one workgroup per (problem, 4096 elements)
+// ------------------------------------------------------------------------------
+template <typename S, typename D>
+__global__ __launch_bounds__(kThreads) void k_convert(const ConvProb* __restrict__ probs, int nchunk) {
+  const ConvProb P = probs[blockIdx.x / nchunk];
+  const int e0 = (blockIdx.x % nchunk) * 4096;
+  const S* src = reinterpret_cast<const S*>(P.src);
+  D* dst = reinterpret_cast<D*>(P.dst);
+  const int e1 = min(P.n, e0 + 4096);
+  for (int e = e0 + threadIdx.x; e < e1; e += kThreads) dst[e] = (D)src[e];
+}
+
+void launch_convert(const ConvProb* probs, int n, int nmax, bool to_double, hipStream_t s) {
+  if (n <= 0 || nmax <= 0) return;
+  const int nchunk = (nmax + 4095) / 4096;
+  if (to_double)
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(n * nchunk), dim3(kThreads), 0, s, probs, nchunk);
+  else
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(n * nchunk), dim3(kThreads), 0, s, probs, nchunk);
 }
 
 template <typename T>

@@ -34,6 +34,8 @@ extern "C" {
 /* precision forms of the recompression */
 #define HCORE_HIP_FP64 0   /* everything in FP64 (the plain HCORE_dgemm) */
 #define HCORE_HIP_FP32 1   /* everything in FP32 (HCORE_sgemm) */
+#define HCORE_HIP_MIXED 2  /* FP32 tiles, products, QR and reassembly; the core
+                              SVD and the truncation test against acc in FP64 */
 
 /* error policy of the drop-in HCORE_* entry points */
 #define HCORE_HIP_ON_ERROR_EXIT 0    /* reference behaviour: message + exit(-1) */
@@ -52,7 +54,8 @@ int HCORE_hip_plan_create(hcore_hip_plan **plan, int M, int precision);
 
 /*
  * Record n independent HCORE_dgemm updates C_u <- beta*C_u + alpha*A_u*B_u^T.
- * AU..CV: host arrays of n device pointers (FP64, or FP32 for HCORE_HIP_FP32)
+ * AU..CV: host arrays of n device pointers (FP64, or FP32 for HCORE_HIP_FP32
+ * and HCORE_HIP_MIXED)
  * to column-major factors with leading dimensions LDA, LDB, LDC.  CUout/CVout
  * (may be NULL: in place) receive the new factors with leading dimension
  * LDCout.  Ark/Brk/Crk: host int ranks.  rk, maxrk, acc as in HCORE_dgemm.
@@ -108,8 +111,10 @@ int HCORE_hip_plan_flops(hcore_hip_plan *plan, unsigned long *update, unsigned l
 void HCORE_hip_plan_destroy(hcore_hip_plan *plan);
 
 /* Per-phase timing.  A plan of more than 64 updates splits them into up to
- * three lanes by stacked rank K (core beyond LDS, 97..LDS limit, <= 96), each
- * running its kernels on its own stream; the dsyrk updates run on a fourth.  When timing is enabled,
+ * three lanes by stacked rank K (> 128, 65..128, <= 64), the first on the
+ * caller's stream and the others on streams of their own (four hardware
+ * queues with the null stream); the dsyrk updates follow the last lane on
+ * its stream.  When timing is enabled,
  * HCORE_hip_plan_execute records a HIP event pair around every kernel launch
  * on the stream it runs on.  Phases: 0 stack, 1 F=AV^T BV, 2 G=BU F^T,
  * 3 geqrf, 4 T=R_A R_B^T, 5 core SVD (all K buckets of the lane, run

@@ -199,6 +199,51 @@ def test_trailing_step_batch_vs_oracle(hc, orc, grid):
     p.destroy()
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp32"])
+def test_reduced_precision_trailing_step(hc, orc, precision):
+    """The grid-13 trailing step (66 updates, three rank lanes) on FP32 tiles
+    at acc = 1e-5: the mixed-precision plan (FP32 products, QR and
+    reassembly; FP64 core SVD and truncation) and the all-FP32 plan (the
+    batched HCORE_sgemm).  Every update within 10*acc of the dense update of
+    the same FP32 tiles (evaluated in FP64) and within one of the oracle's
+    rank on those tiles."""
+    from hcore_amd.tiles import TrailingStep
+    acc = 1e-5
+    f32 = np.float32
+    step = TrailingStep(grid=13, nb=256, key="mixed-test", diag=False)
+    _say("case: reduced precision", precision, "grid 13 nb 256")
+    nb = step.nb
+    ra, rb, rc = step.ranks()
+    K = ra + rc
+    panel = {i: (colmajor(hc, u.astype(f32)), colmajor(hc, v.astype(f32)))
+             for i, (u, v) in step.panel.items()}
+    CUd = [colmajor(hc, c[0].astype(f32), k) for c, k in zip(step.C, K)]
+    CVd = [colmajor(hc, c[1].astype(f32), k) for c, k in zip(step.C, K)]
+    p = hc.Plan(nb, hc.MIXED if precision == "mixed" else hc.FP32)
+    p.add_dgemm([panel[i][0].ptr for i, j in step.pairs], [panel[i][1].ptr for i, j in step.pairs],
+                ra, nb, [panel[j][0].ptr for i, j in step.pairs],
+                [panel[j][1].ptr for i, j in step.pairs], rb, nb,
+                [t.ptr for t in CUd], [t.ptr for t in CVd], rc, nb, -1.0, 1.0, 0, 512, acc)
+    p.finalize()
+    p.execute()
+    hc.device_synchronize()
+    assert np.all(p.status() == 0)
+    ranks = p.ranks()
+    lanes = p.lanes()
+    assert len(set(lanes.tolist())) == 3
+    for u, (i, j) in enumerate(step.pairs):
+        AU, AV = (x.astype(f32).astype(np.float64) for x in step.panel[i])
+        BU, BV = (x.astype(f32).astype(np.float64) for x in step.panel[j])
+        CU, CV = (x.astype(f32).astype(np.float64) for x in step.C[u])
+        E = CU @ CV.T - (AU @ AV.T) @ (BU @ BV.T).T
+        r = int(ranks[u])
+        got = CUd[u].numpy(r).astype(np.float64) @ CVd[u].numpy(r).astype(np.float64).T
+        assert rel_fro(got, E) <= 10 * acc, (u, rel_fro(got, E))
+        st, _, _, ro, _, _ = orc.dgemm(-1.0, AU, AV, BU, BV, 1.0, CU, CV, maxrk=512, acc=acc)
+        assert abs(r - ro) <= 1, (u, r, ro)
+    p.destroy()
+
+
 @pytest.mark.parametrize("grid", [6, 13])
 def test_graph_replays_match_eager(hc, grid):
     """The plan captured in a hipGraph and replayed several times (out of
