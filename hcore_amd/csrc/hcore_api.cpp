@@ -144,6 +144,18 @@ static int core_kl() { return sizeof(T) == 8 ? g_ctx.core_kl_d : g_ctx.core_kl_f
 // ------------------------------------------------------------------------------
 // host-side descriptor packing
 // ------------------------------------------------------------------------------
+// Output tiles of one batched GEMM phase: 64 x 64 tiles, or 32 x 32 tiles
+// (one wavefront each) for problems with m or n <= 32, which would otherwise
+// compute mostly padding.
+struct TaskList {
+  std::vector<GemmTask> big, small;
+  size_t size() const { return big.size() + small.size(); }
+};
+struct TaskRef {
+  size_t ob = 0, os = 0;
+  int nb = 0, ns = 0;
+};
+
 struct Packer {
   std::vector<char> buf;
   template <typename X>
@@ -153,11 +165,22 @@ struct Packer {
     if (!v.empty()) memcpy(buf.data() + off, v.data(), v.size() * sizeof(X));
     return off;
   }
+  TaskRef add(const TaskList& tl) {
+    TaskRef r;
+    r.ob = add(tl.big);
+    r.nb = (int)tl.big.size();
+    r.os = add(tl.small);
+    r.ns = (int)tl.small.size();
+    return r;
+  }
 };
 
-static void add_tasks(std::vector<GemmTask>& tasks, int prob, int m, int n) {
-  for (int tj = 0; tj < (n + 63) / 64; ++tj)
-    for (int ti = 0; ti < (m + 63) / 64; ++ti) tasks.push_back({prob, ti, tj});
+static void add_tasks(TaskList& tl, int prob, int m, int n) {
+  const bool sm = m <= 32 || n <= 32;
+  const int t = sm ? 32 : 64;
+  auto& v = sm ? tl.small : tl.big;
+  for (int tj = 0; tj < (n + t - 1) / t; ++tj)
+    for (int ti = 0; ti < (m + t - 1) / t; ++ti) v.push_back({prob, ti, tj});
 }
 
 }  // namespace hcore
@@ -202,24 +225,25 @@ struct hcore_hip_plan {
   };
   struct Lane {
     int n = 0;
-    size_t o_stack = 0, o_tF = 0, o_tG = 0, o_tT = 0, o_tZ = 0, o_qr = 0, o_apply = 0;
-    int n_tF = 0, n_tG = 0, n_tT = 0, n_tZ = 0, n_qr = 0, n_apply = 0;
+    size_t o_stack = 0, o_qr = 0, o_apply = 0;
+    int n_qr = 0, n_apply = 0;
+    hcore::TaskRef tF, tG, tT, tZ, tB, tX;  // batched GEMM phases
     int stack_cols = 0;  // largest 2rc+ra of the lane
     int kmax = 0;        // largest stacked rank of the lane
     bool wy = false;     // reassembly by the compact-WY solve (kmax <= kWyMaxK)
-    size_t o_tB = 0, o_wy = 0, o_tX = 0, o_qrr = 0, o_qrp = 0;
+    size_t o_wy = 0, o_qrr = 0, o_qrp = 0;
     int n_qrr = 0, qrr_kmax = 0, n_qrp = 0;
     size_t o_cvin = 0, o_cvout = 0;  // mixed: FP32 <-> FP64 conversions around the core
     int n_cv = 0, cv_max = 0;
-    int n_tB = 0, n_wy = 0, n_tX = 0;
+    int n_wy = 0;
     std::vector<CoreLaunch> core;  // one launch per K bucket
     hipStream_t stream = nullptr;  // lanes > 0 (lane 0 uses the caller's stream)
     hipEvent_t ev_done = nullptr;
   };
   std::vector<Lane> lanes;
   std::vector<int> lane_of;  // lane index of each update (HCORE_hip_plan_get_lanes)
-  size_t o_gp = 0, o_sp = 0, o_tW = 0, o_tX = 0, o_tC = 0;
-  int n_tW = 0, n_tX = 0, n_tC = 0;
+  size_t o_gp = 0, o_sp = 0;
+  hcore::TaskRef tW, tXs, tC;  // dsyrk GEMM phases
   hipEvent_t ev_start = nullptr;
   int* d_sweeps = nullptr;
   long long* d_clk = nullptr;  // kCoreClk core stamps per update
@@ -369,7 +393,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
     std::vector<GemmProb<T>> gp;
     struct LaneDesc {
       std::vector<StackProb<T>> stack;
-      std::vector<GemmTask> tF, tG, tT, tZ, tB, tX;
+      TaskList tF, tG, tT, tZ, tB, tX;
       std::vector<WySolveProb<T>> wy;
       int kmax = 0;
       bool wy_lane = false;
@@ -507,7 +531,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
     }
     // syrk (hcore_dsyrk.c:103, 144, 165-181)
     std::vector<GemmProb<T>> sp;
-    std::vector<GemmTask> tW, tX, tC;
+    TaskList tW, tX, tC;
     for (int s = 0; s < p->ns; ++s) {
       const int K = p->sK[s];
       GemmProb<T> g{};
@@ -535,16 +559,16 @@ static int plan_finalize_t(hcore_hip_plan* p) {
       L.o_stack = pk.add(d.stack);
       L.stack_cols = 0;
       for (const auto& sp_ : d.stack) L.stack_cols = std::max(L.stack_cols, 2 * sp_.rc + sp_.ra);
-      L.o_tF = pk.add(d.tF); L.n_tF = (int)d.tF.size();
-      L.o_tG = pk.add(d.tG); L.n_tG = (int)d.tG.size();
-      L.o_tT = pk.add(d.tT); L.n_tT = (int)d.tT.size();
-      L.o_tZ = pk.add(d.tZ); L.n_tZ = (int)d.tZ.size();
+      L.tF = pk.add(d.tF);
+      L.tG = pk.add(d.tG);
+      L.tT = pk.add(d.tT);
+      L.tZ = pk.add(d.tZ);
       L.o_qr = pk.add(d.qr); L.n_qr = (int)d.qr.size();
       L.o_qrr = pk.add(d.qrr); L.n_qrr = (int)d.qrr.size(); L.qrr_kmax = d.qrr_kmax;
       L.o_qrp = pk.add(d.qrp); L.n_qrp = (int)d.qrp.size();
       L.o_apply = pk.add(d.ap); L.n_apply = (int)d.ap.size();
-      L.o_tB = pk.add(d.tB); L.n_tB = (int)d.tB.size();
-      L.o_tX = pk.add(d.tX); L.n_tX = (int)d.tX.size();
+      L.tB = pk.add(d.tB);
+      L.tX = pk.add(d.tX);
       L.o_wy = pk.add(d.wy); L.n_wy = (int)d.wy.size();
       L.kmax = d.kmax;
       L.wy = d.wy_lane;
@@ -564,9 +588,9 @@ static int plan_finalize_t(hcore_hip_plan* p) {
                        });
     }
     p->o_sp = pk.add(sp);
-    p->o_tW = pk.add(tW); p->n_tW = (int)tW.size();
-    p->o_tX = pk.add(tX); p->n_tX = (int)tX.size();
-    p->o_tC = pk.add(tC); p->n_tC = (int)tC.size();
+    p->tW = pk.add(tW);
+    p->tXs = pk.add(tX);
+    p->tC = pk.add(tC);
   };
   build(nullptr, nullptr, nullptr);  // dry run: descriptor size
   const size_t head_i = ((size_t)std::max(n, 1) * 3 * sizeof(int) + 255) & ~size_t(255);
@@ -663,13 +687,18 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
   // dsyrk updates (hcore_dsyrk.c:103, 144, 165-181): on the stream of the last
   // lane, after its recompressions, so that a plan uses at most three streams
   // (the caller's and two lanes) next to the null stream: four hardware queues
+  // one GEMM phase: its 64-tile and 32-tile launches
+  auto gemm = [&](const GemmProb<T>* probs, const TaskRef& r, hipStream_t st) {
+    launch_gemm<T>(probs, reinterpret_cast<const GemmTask*>(D(r.ob)), r.nb, 64, st);
+    launch_gemm<T>(probs, reinterpret_cast<const GemmTask*>(D(r.os)), r.ns, 32, st);
+  };
   auto syrk = [&](hipStream_t ss) {
     if (p->ns <= 0) return;
     const auto* sp = reinterpret_cast<const GemmProb<T>*>(D(p->o_sp));
     const int lt = hcore_hip_plan::kMaxLanes - 1;
-    phase(lt, 9, ss, true, [&] { launch_gemm<T>(sp, reinterpret_cast<const GemmTask*>(D(p->o_tW)), p->n_tW, ss); });
-    phase(lt, 10, ss, true, [&] { launch_gemm<T>(sp, reinterpret_cast<const GemmTask*>(D(p->o_tX)), p->n_tX, ss); });
-    phase(lt, 11, ss, true, [&] { launch_gemm<T>(sp, reinterpret_cast<const GemmTask*>(D(p->o_tC)), p->n_tC, ss); });
+    phase(lt, 9, ss, true, [&] { gemm(sp, p->tW, ss); });
+    phase(lt, 10, ss, true, [&] { gemm(sp, p->tXs, ss); });
+    phase(lt, 11, ss, true, [&] { gemm(sp, p->tC, ss); });
   };
   (void)hipEventRecord(p->ev_start, s);
   for (int l = 0; l < nl; ++l) {
@@ -680,14 +709,14 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
       launch_stack<T>(reinterpret_cast<const StackProb<T>*>(D(L.o_stack)), L.n, M, L.stack_cols,
                       T(p->alpha), T(p->beta), ls);
     });
-    phase(l, 1, ls, true, [&] { launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(L.o_tF)), L.n_tF, ls); });
-    phase(l, 2, ls, true, [&] { launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(L.o_tG)), L.n_tG, ls); });
+    phase(l, 1, ls, true, [&] { gemm(gp, L.tF, ls); });
+    phase(l, 2, ls, true, [&] { gemm(gp, L.tG, ls); });
     phase(l, 3, ls, true, [&] {
       launch_qr_fused<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qrp)), L.n_qrp, M, ls);
       launch_qr_reg<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qrr)), L.n_qrr, M, L.qrr_kmax, ls);
       launch_geqrf<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qr)), L.n_qr, M, ls);
     });
-    phase(l, 4, ls, true, [&] { launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(L.o_tT)), L.n_tT, ls); });
+    phase(l, 4, ls, true, [&] { gemm(gp, L.tT, ls); });
     // the lane's K buckets, largest first (each bucket a kernel variant with its
     // own LDS size); at most 4 streams per plan, one per hardware queue
     phase(l, 5, ls, !L.core.empty(), [&] {
@@ -702,15 +731,15 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
       }
       launch_convert(reinterpret_cast<const ConvProb*>(D(L.o_cvout)), L.n_cv, L.cv_max, false, ls);
     });
-    phase(l, 7, ls, true, [&] { launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(L.o_tZ)), L.n_tZ, ls); });
+    phase(l, 7, ls, true, [&] { gemm(gp, L.tZ, ls); });
     if (L.wy) {
       phase(l, 6, ls, true, [&] {
-        launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(L.o_tB)), L.n_tB, ls);
+        gemm(gp, L.tB, ls);
         launch_wy_solve<T>(reinterpret_cast<const WySolveProb<T>*>(D(L.o_wy)), L.n_wy, L.kmax,
                            L.kmax, ls);
       });
       phase(l, 8, ls, true, [&] {
-        launch_gemm<T>(gp, reinterpret_cast<const GemmTask*>(D(L.o_tX)), L.n_tX, ls);
+        gemm(gp, L.tX, ls);
       });
     } else {
       phase(l, 8, ls, true, [&] {
@@ -889,10 +918,10 @@ static void single_gemm(int M, T alpha, T* AU, T* AV, T* Ark, int LDA, T* BU, T*
 
 // Run a short list of dependent GEMMs on the context stream (dsyrk, dgemmcd).
 static int run_gemm_chain(const std::vector<GemmProb<double>>& gp) {
-  std::vector<std::vector<GemmTask>> tl(gp.size());
+  std::vector<TaskList> tl(gp.size());
   Packer pk;
   size_t o_gp = pk.add(gp);
-  std::vector<size_t> o_t(gp.size());
+  std::vector<TaskRef> o_t(gp.size());
   for (size_t i = 0; i < gp.size(); ++i) {
     add_tasks(tl[i], (int)i, gp[i].m, gp[i].n);
     o_t[i] = pk.add(tl[i]);
@@ -901,9 +930,11 @@ static int run_gemm_chain(const std::vector<GemmProb<double>>& gp) {
   HCHECK(hipMalloc(&d, pk.buf.size() + 256));
   hipStream_t s = single_stream();
   HCHECK(hipMemcpyAsync(d, pk.buf.data(), pk.buf.size(), hipMemcpyHostToDevice, s));
-  for (size_t i = 0; i < gp.size(); ++i)
-    launch_gemm<double>(reinterpret_cast<const GemmProb<double>*>(d + o_gp),
-                        reinterpret_cast<const GemmTask*>(d + o_t[i]), (int)tl[i].size(), s);
+  for (size_t i = 0; i < gp.size(); ++i) {
+    const auto* probs = reinterpret_cast<const GemmProb<double>*>(d + o_gp);
+    launch_gemm<double>(probs, reinterpret_cast<const GemmTask*>(d + o_t[i].ob), o_t[i].nb, 64, s);
+    launch_gemm<double>(probs, reinterpret_cast<const GemmTask*>(d + o_t[i].os), o_t[i].ns, 32, s);
+  }
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   hipFree(d);

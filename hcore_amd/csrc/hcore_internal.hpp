@@ -49,7 +49,7 @@ struct GemmProb {
 
 struct GemmTask {
   int prob;
-  int ti, tj;  // 64x64 tile coordinates
+  int ti, tj;  // tile coordinates (64 x 64, or 32 x 32 in the small-tile lists)
 };
 
 // ---- stacked factors: Us = [beta*CU | alpha*AU], Vs[:, :rc] = CV --------------
@@ -139,8 +139,10 @@ constexpr int kCoreLdsG = 4;  // group width of the largest LDS-resident core va
 int core_lds_max_k(size_t elem_size);
 
 // ---- launchers (hcore_kernels.hip) -------------------------------------------
+// tile: 64 (256 threads) or 32 (64 threads, for problems with m or n <= 32)
 template <typename T>
-void launch_gemm(const GemmProb<T>* probs, const GemmTask* tasks, int ntasks, hipStream_t s);
+void launch_gemm(const GemmProb<T>* probs, const GemmTask* tasks, int ntasks, int tile,
+                 hipStream_t s);
 constexpr int kStackCols = 4;  // stacked columns per workgroup of k_stack
 template <typename T>
 void launch_stack(const StackProb<T>* probs, int n, int nb, int maxcols, T alpha, T beta,

@@ -159,12 +159,8 @@ __device__ __forceinline__ float eps_of<float>() { return FLT_EPSILON * 0.5f; }
 // ------------------------------------------------------------------------------
 // batched GEMM on 64x64 tiles, 4x4 outputs per thread, k staged through LDS
 // ------------------------------------------------------------------------------
-constexpr int GT = 64;  // output tile edge
 constexpr int GK = 32;  // k chunk
 
-// Load the k chunk starting at k0 of op(A) (rows row0..) and op(B) (columns
-// col0..) into registers: 8 elements of each per thread, with consecutive
-// lanes on consecutive addresses for every transpose combination.
 template <typename T>
 __device__ __forceinline__ T gemm_masked(const T* X, int ld, int p, int q, bool upper, bool unit_lower) {
   if (upper && p > q) return T(0);
@@ -172,17 +168,21 @@ __device__ __forceinline__ T gemm_masked(const T* X, int ld, int p, int q, bool 
   return X[p + (size_t)q * ld];
 }
 
-template <typename T>
+// Load the k chunk starting at k0 of op(A) (rows row0..) and op(B) (columns
+// col0..) into registers: LD elements of each per thread, consecutive lanes
+// on consecutive addresses for every transpose combination.
+template <typename T, int TILE, int NT, int LD>
 __device__ __forceinline__ void gemm_fetch(const GemmProb<T>& P, int n, int row0, int col0, int k0,
-                                           bool ta, bool tb, bool ua, bool ub, T (&ra)[8],
-                                           T (&rb)[8]) {
+                                           bool ta, bool tb, bool ua, bool ub, T (&ra)[LD],
+                                           T (&rb)[LD]) {
   const bool la = (P.flags & GF_A_UNIT_LOWER) != 0, lb = (P.flags & GF_B_UNIT_LOWER) != 0;
   const int t = threadIdx.x;
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
+  for (int q = 0; q < LD; ++q) {
+    const int idx = t + NT * q;
     int i, kk;
-    if (!ta) { i = t & 63; kk = (t >> 6) + 4 * q; }
-    else     { kk = t & 31; i = (t >> 5) + 8 * q; }
+    if (!ta) { i = idx % TILE; kk = idx / TILE; }
+    else     { kk = idx % GK; i = idx / GK; }
     const int gi = row0 + i, gk = k0 + kk;
     T v = T(0);
     if (gi < P.m && gk < P.k) {
@@ -192,10 +192,11 @@ __device__ __forceinline__ void gemm_fetch(const GemmProb<T>& P, int n, int row0
     ra[q] = v;
   }
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
+  for (int q = 0; q < LD; ++q) {
+    const int idx = t + NT * q;
     int j, kk;
-    if (tb) { j = t & 63; kk = (t >> 6) + 4 * q; }
-    else    { kk = t & 31; j = (t >> 5) + 8 * q; }
+    if (tb) { j = idx % TILE; kk = idx / TILE; }
+    else    { kk = idx % GK; j = idx / GK; }
     const int gj = col0 + j, gk = k0 + kk;
     T v = T(0);
     if (gj < n && gk < P.k) {
@@ -206,25 +207,31 @@ __device__ __forceinline__ void gemm_fetch(const GemmProb<T>& P, int n, int row0
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_gemm(const GemmProb<T>* __restrict__ probs,
-                                                   const GemmTask* __restrict__ tasks) {
-  // +1 column of padding: the transposed fills write 32 consecutive k at one i
-  __shared__ T As[GK][GT + 1];
-  __shared__ T Bs[GK][GT + 1];
+// Batched GEMM on TILE x TILE output tiles (64: 256 threads; 32: one
+// wavefront), 4 x 4 outputs per thread, k staged through LDS in chunks of GK
+// with the next chunk's global loads in flight during the current chunk.
+template <typename T, int TILE>
+__global__ __launch_bounds__((TILE / 4) * (TILE / 4)) void k_gemm(
+    const GemmProb<T>* __restrict__ probs, const GemmTask* __restrict__ tasks) {
+  constexpr int TD = TILE / 4;       // threads per tile edge
+  constexpr int NT = TD * TD;        // threads
+  constexpr int LD = TILE * GK / NT; // elements of each operand loaded per thread per chunk
+  // +1 column of padding: the transposed fills write consecutive k at one i
+  __shared__ T As[GK][TILE + 1];
+  __shared__ T Bs[GK][TILE + 1];
   const GemmTask tk = tasks[blockIdx.x];
   const GemmProb<T> P = probs[tk.prob];
   if (P.skip && *P.skip) return;
   int n = P.n;
   if (P.n_dev) n = min(n, *P.n_dev);
-  const int row0 = tk.ti * GT, col0 = tk.tj * GT;
+  const int row0 = tk.ti * TILE, col0 = tk.tj * TILE;
   if (row0 >= P.m || col0 >= n) return;
   const int t = threadIdx.x;
-  const int tx = t & 15, ty = t >> 4;
+  const int tx = t % TD, ty = t / TD;
   const bool lzu = (P.flags & GF_LOWER_ZERO_UPPER) != 0;
-  if (lzu && row0 + GT - 1 < col0) {  // tile strictly above the diagonal: zero fill
-    for (int e = t; e < GT * GT; e += kThreads) {
-      const int i = row0 + (e & 63), j = col0 + (e >> 6);
+  if (lzu && row0 + TILE - 1 < col0) {  // tile strictly above the diagonal: zero fill
+    for (int e = t; e < TILE * TILE; e += NT) {
+      const int i = row0 + (e % TILE), j = col0 + (e / TILE);
       if (i < P.m && j < n) P.C[i + (size_t)j * P.ldc] = T(0);
     }
     return;
@@ -237,27 +244,26 @@ __global__ __launch_bounds__(kThreads) void k_gemm(const GemmProb<T>* __restrict
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
 
-  // software pipeline: the next chunk's global loads are in flight while the
-  // current chunk is multiplied out of LDS
-  T ra[8], rb[8];
-  gemm_fetch(P, n, row0, col0, 0, ta, tb, ua, ub, ra, rb);
+  T ra[LD], rb[LD];
+  gemm_fetch<T, TILE, NT, LD>(P, n, row0, col0, 0, ta, tb, ua, ub, ra, rb);
   for (int k0 = 0; k0 < P.k; k0 += GK) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      if (!ta) As[(t >> 6) + 4 * q][t & 63] = ra[q];
-      else     As[t & 31][(t >> 5) + 8 * q] = ra[q];
-      if (tb) Bs[(t >> 6) + 4 * q][t & 63] = rb[q];
-      else    Bs[t & 31][(t >> 5) + 8 * q] = rb[q];
+    for (int q = 0; q < LD; ++q) {
+      const int idx = t + NT * q;
+      if (!ta) As[idx / TILE][idx % TILE] = ra[q];
+      else     As[idx % GK][idx / GK] = ra[q];
+      if (tb) Bs[idx / TILE][idx % TILE] = rb[q];
+      else    Bs[idx % GK][idx / GK] = rb[q];
     }
     __syncthreads();
-    if (k0 + GK < P.k) gemm_fetch(P, n, row0, col0, k0 + GK, ta, tb, ua, ub, ra, rb);
+    if (k0 + GK < P.k) gemm_fetch<T, TILE, NT, LD>(P, n, row0, col0, k0 + GK, ta, tb, ua, ub, ra, rb);
 #pragma unroll 8
     for (int kk = 0; kk < GK; ++kk) {
       T av[4], bv[4];
 #pragma unroll
-      for (int a = 0; a < 4; ++a) av[a] = As[kk][tx + 16 * a];
+      for (int a = 0; a < 4; ++a) av[a] = As[kk][tx + TD * a];
 #pragma unroll
-      for (int b = 0; b < 4; ++b) bv[b] = Bs[kk][ty + 16 * b];
+      for (int b = 0; b < 4; ++b) bv[b] = Bs[kk][ty + TD * b];
 #pragma unroll
       for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -271,7 +277,7 @@ __global__ __launch_bounds__(kThreads) void k_gemm(const GemmProb<T>* __restrict
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-      const int i = row0 + tx + 16 * a, j = col0 + ty + 16 * b;
+      const int i = row0 + tx + TD * a, j = col0 + ty + TD * b;
       if (i < P.m && j < n) {
         T* c = P.C + i + (size_t)j * P.ldc;
         if (lzu && i < j) { *c = T(0); continue; }
@@ -287,9 +293,13 @@ __global__ __launch_bounds__(kThreads) void k_gemm(const GemmProb<T>* __restrict
 }
 
 template <typename T>
-void launch_gemm(const GemmProb<T>* probs, const GemmTask* tasks, int ntasks, hipStream_t s) {
+void launch_gemm(const GemmProb<T>* probs, const GemmTask* tasks, int ntasks, int tile,
+                 hipStream_t s) {
   if (ntasks <= 0) return;
-  hipLaunchKernelGGL(k_gemm<T>, dim3(ntasks), dim3(kThreads), 0, s, probs, tasks);
+  if (tile == 32)
+    hipLaunchKernelGGL((k_gemm<T, 32>), dim3(ntasks), dim3(64), 0, s, probs, tasks);
+  else
+    hipLaunchKernelGGL((k_gemm<T, 64>), dim3(ntasks), dim3(256), 0, s, probs, tasks);
 }
 
 // ------------------------------------------------------------------------------
@@ -2257,7 +2267,7 @@ void prepare_kernels() {
 
 // ---- explicit instantiations ---------------------------------------------------
 #define HCORE_INSTANTIATE(T)                                                                  \
-  template void launch_gemm<T>(const GemmProb<T>*, const GemmTask*, int, hipStream_t);        \
+  template void launch_gemm<T>(const GemmProb<T>*, const GemmTask*, int, int, hipStream_t);   \
   template void launch_stack<T>(const StackProb<T>*, int, int, int, T, T, hipStream_t);       \
   template void launch_geqrf<T>(const QrProb<T>*, int, int, hipStream_t);                     \
   template void launch_qr_reg<T>(const QrProb<T>*, int, int, int, hipStream_t);               \
