@@ -10,15 +10,15 @@ timed region; the recompressed factors are written out of place so every
 step performs identical work (the diagonal dsyrk updates run in place).
 
 A "step" is one execution of the whole batch: one plan execution, whose
-updates run in up to three lanes by stacked rank (each on its own stream)
-plus the diagonal dsyrk updates on a fourth, captured once into a hipGraph
-and replayed.  value = tile updates per second (2016 per step), timed over
+updates run in up to three lanes by stacked rank, each on its own stream
+(the diagonal dsyrk updates follow one of them), captured once into a
+hipGraph and replayed.  value = tile updates per second (2016 per step), timed over
 graph replays with HIP events on the launching stream.
 
 Also measured in the same run:
-  * per-kernel durations: the same step executed eagerly with a HIP event
-    pair around every launch, on the stream the kernel runs on -> the
-    roofline of the dominant kernel;
+  * per-kernel durations: the same step executed eagerly with the lanes one
+    after another and a HIP event pair around every launch -> the roofline of
+    the dominant kernel (the longest launch);
   * the CPU baseline: the oracle (the reference's LAPACK call sequence,
     scipy-openblas, one update per OpenMP thread) on the host cores.
 """
@@ -237,9 +237,10 @@ def main():
     updates = n + ns
     value = updates / (ms_per_step / 1e3)
 
-    # ---- per-kernel durations: the same step executed eagerly with a HIP event
-    # pair around every launch, on the stream each kernel runs on ------------------
-    plan.set_timing(True)
+    # ---- per-kernel durations: the same step executed eagerly, lanes one after
+    # another on the stream, with a HIP event pair around every launch, so that
+    # each duration is the launch's own (not shared with concurrent lanes) ------
+    plan.set_timing(2)
     ms0, c0 = plan.phase_times()
     q0, q1 = hc.Event(), hc.Event()
     q0.record(stream)
@@ -260,7 +261,7 @@ def main():
         row = " ".join("%s %.3f" % (PHASES[i], phase_ms[l, i]) for i in range(12) if c1[l, i] > c0[l, i])
         if row:
             log("  lane %-10s %s" % (lane_names[l], row))
-    log("graph %.3f ms/step (timed), eager with per-launch events %.3f ms/step" %
+    log("graph %.3f ms/step (timed); lanes serialized, per-launch events: %.3f ms/step" %
         (ms_per_step, eager_ms))
 
     # ---- dominant kernel roofline ------------------------------------------------
@@ -364,7 +365,7 @@ def main():
                                "acc=1e-8, maxrank=512",
                    "grid": args.grid, "nb": nb, "updates_per_step": updates,
                    "precision": "fp64", "execution": "hipGraph replay of the step",
-                   "eager_ms_per_step": eager_ms},
+                   "serialized_lanes_ms_per_step": eager_ms},
         "roofline": roof,
         "reduced_precision": mixed,
         "cpu_baseline": cpu,

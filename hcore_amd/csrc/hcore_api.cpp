@@ -257,6 +257,7 @@ struct hcore_hip_plan {
   static constexpr int kRing = 16;
   static constexpr int kMaxLanes = 4;  // 3 gemm lanes + the syrk stream
   bool timing = false;
+  bool serial = false;  // timing mode 2: lanes one after another on the caller's stream
   hipEvent_t ev[kRing][kMaxLanes][HCORE_HIP_NPHASES][2] = {};
   bool used[kRing][kMaxLanes][HCORE_HIP_NPHASES] = {};
   bool pending[kRing] = {};
@@ -684,9 +685,13 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
     }
   };
   const int nl = (int)p->lanes.size();
-  // dsyrk updates (hcore_dsyrk.c:103, 144, 165-181): on the stream of the last
-  // lane, after its recompressions, so that a plan uses at most three streams
-  // (the caller's and two lanes) next to the null stream: four hardware queues
+  // dsyrk updates (hcore_dsyrk.c:103, 144, 165-181): on the stream of one of
+  // the lanes, after its recompressions, so that a plan uses at most three
+  // streams (the caller's and two lanes) next to the null stream: four hardware
+  // queues.  With three lanes they follow the middle one (K 65..128), which
+  // finishes first; the bulk lane (K <= 64) carries the most work and the first
+  // lane the longest chain.
+  const int syrk_lane = nl >= 2 ? nl - 2 : 0;
   // one GEMM phase: its 64-tile and 32-tile launches
   auto gemm = [&](const GemmProb<T>* probs, const TaskRef& r, hipStream_t st) {
     launch_gemm<T>(probs, reinterpret_cast<const GemmTask*>(D(r.ob)), r.nb, 64, st);
@@ -703,8 +708,9 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
   (void)hipEventRecord(p->ev_start, s);
   for (int l = 0; l < nl; ++l) {
     auto& L = p->lanes[l];
-    hipStream_t ls = l == 0 ? s : L.stream;
-    if (l > 0) (void)hipStreamWaitEvent(ls, p->ev_start, 0);
+    const bool own = l > 0 && !(tm && p->serial);  // the lane runs on a stream of its own
+    hipStream_t ls = own ? L.stream : s;
+    if (own) (void)hipStreamWaitEvent(ls, p->ev_start, 0);
     phase(l, 0, ls, true, [&] {
       launch_stack<T>(reinterpret_cast<const StackProb<T>*>(D(L.o_stack)), L.n, M, L.stack_cols,
                       T(p->alpha), T(p->beta), ls);
@@ -746,11 +752,12 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
         launch_apply<T>(reinterpret_cast<const ApplyProb<T>*>(D(L.o_apply)), L.n_apply, M, ls);
       });
     }
-    if (l == nl - 1) syrk(ls);  // the dsyrk updates follow the last (bulk) lane
-    if (l > 0) (void)hipEventRecord(L.ev_done, ls);
+    if (l == syrk_lane) syrk(ls);
+    if (own) (void)hipEventRecord(L.ev_done, ls);
   }
   if (nl == 0) syrk(s);
-  for (int l = 1; l < nl; ++l) (void)hipStreamWaitEvent(s, p->lanes[l].ev_done, 0);
+  if (!(tm && p->serial))
+    for (int l = 1; l < nl; ++l) (void)hipStreamWaitEvent(s, p->lanes[l].ev_done, 0);
   if (tm) p->pending[k] = true;
   HCHECK(hipGetLastError());
   return HCORE_HIP_OK;
@@ -1292,6 +1299,7 @@ int HCORE_hip_plan_set_timing(hcore_hip_plan* p, int enable) {
         for (int i = 0; i < HCORE_HIP_NPHASES; ++i)
           for (int j = 0; j < 2; ++j) HCHECK(hipEventCreate(&p->ev[k][l][i][j]));
   p->timing = enable != 0;
+  p->serial = enable == 2;
   return HCORE_HIP_OK;
 }
 

@@ -113,8 +113,8 @@ void HCORE_hip_plan_destroy(hcore_hip_plan *plan);
 /* Per-phase timing.  A plan of more than 64 updates splits them into up to
  * three lanes by stacked rank K (> 128, 65..128, <= 64), the first on the
  * caller's stream and the others on streams of their own (four hardware
- * queues with the null stream); the dsyrk updates follow the last lane on
- * its stream.  When timing is enabled,
+ * queues with the null stream); the dsyrk updates follow the next-to-last
+ * lane on its stream.  When timing is enabled,
  * HCORE_hip_plan_execute records a HIP event pair around every kernel launch
  * on the stream it runs on.  Phases: 0 stack, 1 F=AV^T BV, 2 G=BU F^T,
  * 3 geqrf, 4 T=R_A R_B^T, 5 core SVD (all K buckets of the lane, run
@@ -124,6 +124,9 @@ void HCORE_hip_plan_destroy(hcore_hip_plan *plan);
  * lane 3 holds the syrk phases); times accumulate over executions. */
 #define HCORE_HIP_NPHASES 12
 #define HCORE_HIP_NLANES 4
+/* enable: 0 off, 1 on (lanes concurrent, as in untimed execution), 2 on with
+ * the lanes run one after another on the caller's stream, so that every
+ * launch's duration is its own rather than shared with the other lanes */
 int HCORE_hip_plan_set_timing(hcore_hip_plan *plan, int enable);
 int HCORE_hip_plan_phase_times(hcore_hip_plan *plan, double *ms, int *count);
 
