@@ -231,8 +231,8 @@ struct hcore_hip_plan {
     int stack_cols = 0;  // largest 2rc+ra of the lane
     int kmax = 0;        // largest stacked rank of the lane
     bool wy = false;     // reassembly by the compact-WY solve (kmax <= kWyMaxK)
-    size_t o_wy = 0, o_qrr = 0, o_qrp = 0;
-    int n_qrr = 0, qrr_kmax = 0, n_qrp = 0;
+    size_t o_wy = 0, o_qrr16 = 0, o_qrr = 0, o_qrp = 0;
+    int n_qrr16 = 0, n_qrr = 0, n_qrp = 0;  // register QR for K <= 16, 17..32; panels
     size_t o_cvin = 0, o_cvout = 0;  // mixed: FP32 <-> FP64 conversions around the core
     int n_cv = 0, cv_max = 0;
     int n_wy = 0;
@@ -398,8 +398,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
       std::vector<WySolveProb<T>> wy;
       int kmax = 0;
       bool wy_lane = false;
-      std::vector<QrProb<T>> qr, qrr, qrp;  // blocked / register single pass / register panels
-      int qrr_kmax = 0;
+      std::vector<QrProb<T>> qr, qrr16, qrr, qrp;  // blocked / register single pass / register panels
       std::vector<std::vector<CoreProb<T>>> core_b;
       std::vector<std::vector<CoreProb<double>>> core_b64;  // mixed: the FP64 core
       std::vector<ConvProb> cv_in, cv_out;
@@ -489,9 +488,9 @@ static int plan_finalize_t(hcore_hip_plan* p) {
         // QR of both stacks
         if (qreg) {  // one register-resident pass (also yields S)
           T* SA = W + o_S[u];
-          d.qrr.push_back({Us, W + o_tA[u], W + o_TA[u], SA, K});
-          d.qrr.push_back({Vs, W + o_tB[u], W + o_TB[u], SA + al((size_t)K * K), K});
-          d.qrr_kmax = std::max(d.qrr_kmax, K);
+          auto& q = K <= 16 ? d.qrr16 : d.qrr;
+          q.push_back({Us, W + o_tA[u], W + o_TA[u], SA, K});
+          q.push_back({Vs, W + o_tB[u], W + o_TB[u], SA + al((size_t)K * K), K});
         } else if (g_ctx.qr_reg && M <= 1024) {  // register panels + in-place trailing updates
           d.qrp.push_back({Us, W + o_tA[u], W + o_TA[u], nullptr, K});
           d.qrp.push_back({Vs, W + o_tB[u], W + o_TB[u], nullptr, K});
@@ -565,7 +564,8 @@ static int plan_finalize_t(hcore_hip_plan* p) {
       L.tT = pk.add(d.tT);
       L.tZ = pk.add(d.tZ);
       L.o_qr = pk.add(d.qr); L.n_qr = (int)d.qr.size();
-      L.o_qrr = pk.add(d.qrr); L.n_qrr = (int)d.qrr.size(); L.qrr_kmax = d.qrr_kmax;
+      L.o_qrr16 = pk.add(d.qrr16); L.n_qrr16 = (int)d.qrr16.size();
+      L.o_qrr = pk.add(d.qrr); L.n_qrr = (int)d.qrr.size();
       L.o_qrp = pk.add(d.qrp); L.n_qrp = (int)d.qrp.size();
       L.o_apply = pk.add(d.ap); L.n_apply = (int)d.ap.size();
       L.tB = pk.add(d.tB);
@@ -719,7 +719,8 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
     phase(l, 2, ls, true, [&] { gemm(gp, L.tG, ls); });
     phase(l, 3, ls, true, [&] {
       launch_qr_fused<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qrp)), L.n_qrp, M, ls);
-      launch_qr_reg<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qrr)), L.n_qrr, M, L.qrr_kmax, ls);
+      launch_qr_reg<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qrr)), L.n_qrr, M, 32, ls);
+      launch_qr_reg<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qrr16)), L.n_qrr16, M, 16, ls);
       launch_geqrf<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qr)), L.n_qr, M, ls);
     });
     phase(l, 4, ls, true, [&] { gemm(gp, L.tT, ls); });
