@@ -233,6 +233,7 @@ struct hcore_hip_plan {
     bool wy = false;     // reassembly by the compact-WY solve (kmax <= kWyMaxK)
     size_t o_wy = 0, o_qrr16 = 0, o_qrr = 0, o_qrp = 0;
     int n_qrr16 = 0, n_qrr = 0, n_qrp = 0;  // register QR for K <= 16, 17..32; panels
+    bool gram = false;                      // T phase with the stacks' Gram products
     size_t o_cvin = 0, o_cvout = 0;  // mixed: FP32 <-> FP64 conversions around the core
     int n_cv = 0, cv_max = 0;
     int n_wy = 0;
@@ -399,6 +400,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
       int kmax = 0;
       bool wy_lane = false;
       std::vector<QrProb<T>> qr, qrr16, qrr, qrp;  // blocked / register single pass / register panels
+      bool gram = false;  // the T phase holds k = nb Gram products
       std::vector<std::vector<CoreProb<T>>> core_b;
       std::vector<std::vector<CoreProb<double>>> core_b64;  // mixed: the FP64 core
       std::vector<ConvProb> cv_in, cv_out;
@@ -465,7 +467,11 @@ static int plan_finalize_t(hcore_hip_plan* p) {
             g.m = K; g.n = K; g.k = M; g.lda = M; g.ldb = M; g.ldc = K;
             g.alpha = T(1); g.beta = T(0);
             g.flags = GF_A_TRANS | GF_A_UNIT_LOWER | GF_B_UNIT_LOWER | GF_BETA_ZERO;
-            if (!qreg) { add_tasks(d.tT, (int)gp.size(), g.m, g.n); gp.push_back(g); }
+            if (!qreg) {
+              add_tasks(d.tT, (int)gp.size(), g.m, g.n);
+              gp.push_back(g);
+              d.gram = true;
+            }
             // B = Y1^T Z (Z = Zu with Zv's launch, Z = Zv after it)
             g = GemmProb<T>{};
             g.A = Ys[side]; g.B = Zs[side]; g.C = Ms[side];
@@ -564,6 +570,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
       L.tT = pk.add(d.tT);
       L.tZ = pk.add(d.tZ);
       L.o_qr = pk.add(d.qr); L.n_qr = (int)d.qr.size();
+      L.gram = d.gram;
       L.o_qrr16 = pk.add(d.qrr16); L.n_qrr16 = (int)d.qrr16.size();
       L.o_qrr = pk.add(d.qrr); L.n_qrr = (int)d.qrr.size();
       L.o_qrp = pk.add(d.qrp); L.n_qrp = (int)d.qrp.size();
@@ -693,10 +700,13 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
   // lane the longest chain.
   const int syrk_lane = nl >= 2 ? nl - 2 : 0;
   // one GEMM phase: its 64-tile and 32-tile launches
-  auto gemm = [&](const GemmProb<T>* probs, const TaskRef& r, hipStream_t st) {
-    launch_gemm<T>(probs, reinterpret_cast<const GemmTask*>(D(r.ob)), r.nb, 64, st);
-    launch_gemm<T>(probs, reinterpret_cast<const GemmTask*>(D(r.os)), r.ns, 32, st);
+  auto gemm = [&](const GemmProb<T>* probs, const TaskRef& r, hipStream_t st, bool split = false) {
+    launch_gemm<T>(probs, reinterpret_cast<const GemmTask*>(D(r.ob)), r.nb, 64, st, split);
+    launch_gemm<T>(probs, reinterpret_cast<const GemmTask*>(D(r.os)), r.ns, 32, st, split);
   };
+  // phases whose products reduce over the nb rows of the tiles (F = AV^T BV,
+  // the stacks' Gram matrices next to T) share each tile's k range
+  const bool split_k = M >= 256;
   auto syrk = [&](hipStream_t ss) {
     if (p->ns <= 0) return;
     const auto* sp = reinterpret_cast<const GemmProb<T>*>(D(p->o_sp));
@@ -715,7 +725,7 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
       launch_stack<T>(reinterpret_cast<const StackProb<T>*>(D(L.o_stack)), L.n, M, L.stack_cols,
                       T(p->alpha), T(p->beta), ls);
     });
-    phase(l, 1, ls, true, [&] { gemm(gp, L.tF, ls); });
+    phase(l, 1, ls, true, [&] { gemm(gp, L.tF, ls, split_k); });
     phase(l, 2, ls, true, [&] { gemm(gp, L.tG, ls); });
     phase(l, 3, ls, true, [&] {
       launch_qr_fused<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qrp)), L.n_qrp, M, ls);
@@ -723,7 +733,7 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
       launch_qr_reg<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qrr16)), L.n_qrr16, M, 16, ls);
       launch_geqrf<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qr)), L.n_qr, M, ls);
     });
-    phase(l, 4, ls, true, [&] { gemm(gp, L.tT, ls); });
+    phase(l, 4, ls, true, [&] { gemm(gp, L.tT, ls, split_k && L.gram); });
     // the lane's K buckets, largest first (each bucket a kernel variant with its
     // own LDS size); at most 4 streams per plan, one per hardware queue
     phase(l, 5, ls, !L.core.empty(), [&] {

@@ -139,10 +139,12 @@ constexpr int kCoreLdsG = 4;  // group width of the largest LDS-resident core va
 int core_lds_max_k(size_t elem_size);
 
 // ---- launchers (hcore_kernels.hip) -------------------------------------------
-// tile: 64 (256 threads) or 32 (64 threads, for problems with m or n <= 32)
+// tile: 64 (256 threads) or 32 (64 threads, for problems with m or n <= 32);
+// split: the k range of each tile is shared by several thread groups (for
+// k = nb reductions into small outputs)
 template <typename T>
 void launch_gemm(const GemmProb<T>* probs, const GemmTask* tasks, int ntasks, int tile,
-                 hipStream_t s);
+                 hipStream_t s, bool split = false);
 constexpr int kStackCols = 4;  // stacked columns per workgroup of k_stack
 template <typename T>
 void launch_stack(const StackProb<T>* probs, int n, int nb, int maxcols, T alpha, T beta,

@@ -171,18 +171,17 @@ __device__ __forceinline__ T gemm_masked(const T* X, int ld, int p, int q, bool 
 // Load the k chunk starting at k0 of op(A) (rows row0..) and op(B) (columns
 // col0..) into registers: LD elements of each per thread, consecutive lanes
 // on consecutive addresses for every transpose combination.
-template <typename T, int TILE, int NT, int LD>
+template <typename T, int TILE, int NT, int LD, int GKX = GK>
 __device__ __forceinline__ void gemm_fetch(const GemmProb<T>& P, int n, int row0, int col0, int k0,
                                            bool ta, bool tb, bool ua, bool ub, T (&ra)[LD],
-                                           T (&rb)[LD]) {
+                                           T (&rb)[LD], int t) {
   const bool la = (P.flags & GF_A_UNIT_LOWER) != 0, lb = (P.flags & GF_B_UNIT_LOWER) != 0;
-  const int t = threadIdx.x;
 #pragma unroll
   for (int q = 0; q < LD; ++q) {
     const int idx = t + NT * q;
     int i, kk;
     if (!ta) { i = idx % TILE; kk = idx / TILE; }
-    else     { kk = idx % GK; i = idx / GK; }
+    else     { kk = idx % GKX; i = idx / GKX; }
     const int gi = row0 + i, gk = k0 + kk;
     T v = T(0);
     if (gi < P.m && gk < P.k) {
@@ -196,7 +195,7 @@ __device__ __forceinline__ void gemm_fetch(const GemmProb<T>& P, int n, int row0
     const int idx = t + NT * q;
     int j, kk;
     if (tb) { j = idx % TILE; kk = idx / TILE; }
-    else    { kk = idx % GK; j = idx / GK; }
+    else    { kk = idx % GKX; j = idx / GKX; }
     const int gj = col0 + j, gk = k0 + kk;
     T v = T(0);
     if (gj < n && gk < P.k) {
@@ -205,6 +204,33 @@ __device__ __forceinline__ void gemm_fetch(const GemmProb<T>& P, int n, int row0
     }
     rb[q] = v;
   }
+}
+
+// C = alpha op(A) op(B) + beta C (or beta C0) for the 4 x 4 outputs of one
+// thread of a TD x TD thread tile.
+template <typename T, int TD>
+__device__ __forceinline__ void gemm_store(const GemmProb<T>& P, const T (&acc)[4][4], int row0,
+                                           int col0, int tx, int ty, int n) {
+  const bool lzu = (P.flags & GF_LOWER_ZERO_UPPER) != 0;
+  const bool bz = (P.flags & GF_BETA_ZERO) != 0;
+  const bool c0 = (P.flags & GF_C0) != 0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int i = row0 + tx + TD * a, j = col0 + ty + TD * b;
+      if (i < P.m && j < n) {
+        T* c = P.C + i + (size_t)j * P.ldc;
+        if (lzu && i < j) { *c = T(0); continue; }
+        T v = P.alpha * acc[a][b];
+        if (c0) {
+          if (i < P.m0) v = fma(P.beta, P.C0[i + (size_t)j * P.ldc0], v);
+        } else if (!bz) {
+          v = fma(P.beta, *c, v);
+        }
+        *c = v;
+      }
+    }
 }
 
 // Batched GEMM on TILE x TILE output tiles (64: 256 threads; 32: one
@@ -245,7 +271,7 @@ __global__ __launch_bounds__((TILE / 4) * (TILE / 4)) void k_gemm(
     for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
 
   T ra[LD], rb[LD];
-  gemm_fetch<T, TILE, NT, LD>(P, n, row0, col0, 0, ta, tb, ua, ub, ra, rb);
+  gemm_fetch<T, TILE, NT, LD>(P, n, row0, col0, 0, ta, tb, ua, ub, ra, rb, t);
   for (int k0 = 0; k0 < P.k; k0 += GK) {
 #pragma unroll
     for (int q = 0; q < LD; ++q) {
@@ -256,7 +282,7 @@ __global__ __launch_bounds__((TILE / 4) * (TILE / 4)) void k_gemm(
       else    Bs[idx % GK][idx / GK] = rb[q];
     }
     __syncthreads();
-    if (k0 + GK < P.k) gemm_fetch<T, TILE, NT, LD>(P, n, row0, col0, k0 + GK, ta, tb, ua, ub, ra, rb);
+    if (k0 + GK < P.k) gemm_fetch<T, TILE, NT, LD>(P, n, row0, col0, k0 + GK, ta, tb, ua, ub, ra, rb, t);
 #pragma unroll 8
     for (int kk = 0; kk < GK; ++kk) {
       T av[4], bv[4];
@@ -271,31 +297,120 @@ __global__ __launch_bounds__((TILE / 4) * (TILE / 4)) void k_gemm(
     }
     __syncthreads();
   }
-  const bool bz = (P.flags & GF_BETA_ZERO) != 0;
-  const bool c0 = (P.flags & GF_C0) != 0;
+  gemm_store<T, TD>(P, acc, row0, col0, tx, ty, n);
+}
+
+// Split-k variant for long reductions (k = nb) into small outputs: KS thread
+// groups of TD x TD threads share one output tile, group g taking the k
+// chunks g, g + KS, ... (GKS wide, staged through the group's own LDS
+// buffers); the groups' partial tiles are then summed in LDS in a fixed
+// pairwise order, so the result does not depend on timing.  Every group runs
+// the same number of chunk steps (chunks past k load zeros), so the block
+// barriers line up.
+template <typename T, int TILE, int KS>
+__global__ __launch_bounds__((TILE / 4) * (TILE / 4) * KS) void k_gemm_sk(
+    const GemmProb<T>* __restrict__ probs, const GemmTask* __restrict__ tasks) {
+  constexpr int TD = TILE / 4;
+  constexpr int NG = TD * TD;          // threads per group
+  constexpr int NT = NG * KS;
+  constexpr int GKS = 16;              // k chunk of one group
+  constexpr int LD = TILE * GKS / NG;  // elements of each operand per thread per chunk
+  constexpr int STG = 2 * GKS * (TILE + 1);
+  constexpr int RED = (KS / 2) * TILE * TILE;
+  constexpr int SM = KS * STG > RED ? KS * STG : RED;
+  static_assert((KS & (KS - 1)) == 0 && KS >= 2 && LD >= 1, "split-k shape");
+  __shared__ T sm[SM];
+  const GemmTask tk = tasks[blockIdx.x];
+  const GemmProb<T> P = probs[tk.prob];
+  if (P.skip && *P.skip) return;
+  int n = P.n;
+  if (P.n_dev) n = min(n, *P.n_dev);
+  const int row0 = tk.ti * TILE, col0 = tk.tj * TILE;
+  if (row0 >= P.m || col0 >= n) return;
+  const int grp = threadIdx.x / NG, t = threadIdx.x % NG;
+  const int tx = t % TD, ty = t / TD;
+  const bool lzu = (P.flags & GF_LOWER_ZERO_UPPER) != 0;
+  if (lzu && row0 + TILE - 1 < col0) {
+    for (int e = threadIdx.x; e < TILE * TILE; e += NT) {
+      const int i = row0 + (e % TILE), j = col0 + (e / TILE);
+      if (i < P.m && j < n) P.C[i + (size_t)j * P.ldc] = T(0);
+    }
+    return;
+  }
+  const bool ta = P.flags & GF_A_TRANS, tb = P.flags & GF_B_TRANS;
+  const bool ua = P.flags & GF_A_UPPER, ub = P.flags & GF_B_UPPER;
+  T (*As)[TILE + 1] = reinterpret_cast<T (*)[TILE + 1]>(sm + grp * STG);
+  T (*Bs)[TILE + 1] = reinterpret_cast<T (*)[TILE + 1]>(sm + grp * STG + GKS * (TILE + 1));
+  T acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int i = row0 + tx + TD * a, j = col0 + ty + TD * b;
-      if (i < P.m && j < n) {
-        T* c = P.C + i + (size_t)j * P.ldc;
-        if (lzu && i < j) { *c = T(0); continue; }
-        T v = P.alpha * acc[a][b];
-        if (c0) {
-          if (i < P.m0) v = fma(P.beta, P.C0[i + (size_t)j * P.ldc0], v);
-        } else if (!bz) {
-          v = fma(P.beta, *c, v);
-        }
-        *c = v;
-      }
+    for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
+  const int nsteps = (P.k + GKS * KS - 1) / (GKS * KS);
+  T ra[LD], rb[LD];
+  gemm_fetch<T, TILE, NG, LD, GKS>(P, n, row0, col0, grp * GKS, ta, tb, ua, ub, ra, rb, t);
+  for (int st = 0; st < nsteps; ++st) {
+#pragma unroll
+    for (int q = 0; q < LD; ++q) {
+      const int idx = t + NG * q;
+      if (!ta) As[idx / TILE][idx % TILE] = ra[q];
+      else     As[idx % GKS][idx / GKS] = ra[q];
+      if (tb) Bs[idx / TILE][idx % TILE] = rb[q];
+      else    Bs[idx % GKS][idx / GKS] = rb[q];
     }
+    __syncthreads();
+    if (st + 1 < nsteps)
+      gemm_fetch<T, TILE, NG, LD, GKS>(P, n, row0, col0, ((st + 1) * KS + grp) * GKS, ta, tb, ua,
+                                       ub, ra, rb, t);
+#pragma unroll 8
+    for (int kk = 0; kk < GKS; ++kk) {
+      T av[4], bv[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) av[a] = As[kk][tx + TD * a];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) bv[b] = Bs[kk][ty + TD * b];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = fma(av[a], bv[b], acc[a][b]);
+    }
+    __syncthreads();
+  }
+  // pairwise sum of the groups' tiles: at level h, group g (g % 2h == h) hands
+  // its tile to group g - h
+#pragma unroll
+  for (int h = KS / 2; h >= 1; h /= 2) {
+    if (grp >= h && grp < 2 * h) {
+      T* buf = sm + (size_t)(grp - h) * TILE * TILE;
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) buf[(tx + TD * a) + (ty + TD * b) * TILE] = acc[a][b];
+    }
+    __syncthreads();
+    if (grp < h) {
+      const T* buf = sm + (size_t)grp * TILE * TILE;
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] += buf[(tx + TD * a) + (ty + TD * b) * TILE];
+    }
+    __syncthreads();
+  }
+  if (grp == 0) gemm_store<T, TD>(P, acc, row0, col0, tx, ty, n);
 }
 
 template <typename T>
 void launch_gemm(const GemmProb<T>* probs, const GemmTask* tasks, int ntasks, int tile,
-                 hipStream_t s) {
+                 hipStream_t s, bool split) {
   if (ntasks <= 0) return;
+  // Only the 32-tiles are split (four groups of one wavefront): a split 64-tile
+  // workgroup needs a whole CU's wave slots and LDS at once and delays the
+  // other lanes' kernels more than it gains.
+  if (split && tile == 32) {
+    hipLaunchKernelGGL((k_gemm_sk<T, 32, 4>), dim3(ntasks), dim3(256), 0, s, probs, tasks);
+    return;
+  }
   if (tile == 32)
     hipLaunchKernelGGL((k_gemm<T, 32>), dim3(ntasks), dim3(64), 0, s, probs, tasks);
   else
@@ -2199,13 +2314,17 @@ void launch_apply(const ApplyProb<T>* probs, int n, int nb, hipStream_t s) {
 //   M_b = T_bb (B_b - S(b, >b) M_{>b}),  S = Y^T Y,  B = Y1^T Z (in M on entry),
 // which is M = T B for the forward compact-WY T (T^{-1} = diag(1/tau) +
 // striu(S)).  A workgroup holds 64 columns of M in LDS; wave w computes rows
-// 2w, 2w+1 of each panel, lane j column j.  The rows of S are wave-uniform
-// (scalar loads).
+// 2w, 2w+1 of each panel, lane j column j.  The panel's rows of S (contiguous
+// in the stored lower triangle) and its kPB x kPB factor are staged in LDS by
+// coalesced loads; the next panel's are fetched into registers while the
+// current one is solved.
 // ------------------------------------------------------------------------------
+constexpr int kWyLd = 65;  // row stride of M in LDS: column-wise fills spread over the banks
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_wy_solve(const WySolveProb<T>* __restrict__ probs,
                                                        int ncb) {
-  static_assert(kPB == 8 && kThreads == 256, "two rows of a panel per wavefront");
+  static_assert(kPB == 8 && kThreads == 256 && kWyMaxK <= kThreads,
+                "two rows of a panel per wavefront; one S element per thread and row");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const WySolveProb<T> P = probs[blockIdx.x / ncb];
   if (*P.status != ST_OK) return;
@@ -2213,50 +2332,85 @@ __global__ __launch_bounds__(kThreads) void k_wy_solve(const WySolveProb<T>* __r
   const int j0 = (blockIdx.x % ncb) * 64;
   if (j0 >= r) return;
   const int K = P.K;
-  T* Mb = reinterpret_cast<T*>(smem_raw);  // K x 64, row c at Mb[c * 64]
-  T* RH = Mb + (size_t)K * 64;              // kPB x 64 right-hand sides
-  const int lane = threadIdx.x & 63;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int col = j0 + lane;
-  const bool live = col < r;
-  for (int e = threadIdx.x; e < K * 64; e += kThreads) {
-    const int c = e >> 6, j = e & 63;
-    Mb[e] = j0 + j < r ? P.M[c + (size_t)(j0 + j) * K] : T(0);
+  const int nc = min(64, r - j0);
+  T* Mb = reinterpret_cast<T*>(smem_raw);  // K x 64, row c at Mb[c * kWyLd]
+  T* RH = Mb + (size_t)K * kWyLd;           // kPB x 64 right-hand sides
+  T* Sb = RH + kPB * 64;                    // kPB x kWyMaxK: S(hi + i, base + a) at Sb[a * kWyMaxK + i]
+  T* Tl = Sb + kPB * kWyMaxK;               // kPB x kPB panel factor
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  // columns j0 .. j0 + 63 of M, read along its columns (consecutive rows per thread)
+  for (int e = tid; e < K * 64; e += kThreads) {
+    const int c = e % K, j = e / K;
+    Mb[c * kWyLd + j] = j < nc ? P.M[c + (size_t)(j0 + j) * K] : T(0);
   }
-  __syncthreads();
   const int nblk = (K + kPB - 1) / kPB;
-  for (int b = nblk - 1; b >= 0; --b) {
-    const int base = b * kPB, hi = min(K, base + kPB);
+  T pre[kPB], pret = T(0);
+  auto fetch = [&](int b) {
+    const int base = b * kPB, hi = min(K, base + kPB), len = K - hi;
 #pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      const int c = base + 2 * w + e;
-      if (c < hi) {
-        const T* Sc = P.S + (size_t)c * K;  // S(c2, c) = S(c, c2), c2 > c
-        T acc0 = T(0), acc1 = T(0);
-        int c2 = hi;
-        for (; c2 + 1 < K; c2 += 2) {
-          acc0 = fma(Sc[c2], Mb[c2 * 64 + lane], acc0);
-          acc1 = fma(Sc[c2 + 1], Mb[(c2 + 1) * 64 + lane], acc1);
-        }
-        if (c2 < K) acc0 = fma(Sc[c2], Mb[c2 * 64 + lane], acc0);
-        RH[(2 * w + e) * 64 + lane] = Mb[c * 64 + lane] - (acc0 + acc1);
+    for (int a = 0; a < kPB; ++a)
+      pre[a] = (tid < len && base + a < hi) ? P.S[(size_t)(base + a) * K + hi + tid] : T(0);
+    if (tid < kPB * kPB) pret = P.Tp[(size_t)b * kPB * kPB + tid];
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int a = 0; a < kPB; ++a) Sb[a * kWyMaxK + tid] = pre[a];
+    if (tid < kPB * kPB) Tl[tid] = pret;
+  };
+  fetch(nblk - 1);
+  stash();
+  __syncthreads();
+  for (int b = nblk - 1; b >= 0; --b) {
+    const int base = b * kPB, hi = min(K, base + kPB), len = K - hi;
+    if (b > 0) fetch(b - 1);  // in flight while this panel is solved
+    {
+      const int a0 = 2 * w, a1 = 2 * w + 1;
+      const T* S0 = Sb + a0 * kWyMaxK;
+      const T* S1 = Sb + a1 * kWyMaxK;
+      T p0 = T(0), p1 = T(0), q0 = T(0), q1 = T(0);
+      int i = 0;
+      for (; i + 1 < len; i += 2) {
+        const T m0 = Mb[(hi + i) * kWyLd + lane], m1 = Mb[(hi + i + 1) * kWyLd + lane];
+        p0 = fma(S0[i], m0, p0);
+        q0 = fma(S1[i], m0, q0);
+        p1 = fma(S0[i + 1], m1, p1);
+        q1 = fma(S1[i + 1], m1, q1);
       }
+      if (i < len) {
+        const T m0 = Mb[(hi + i) * kWyLd + lane];
+        p0 = fma(S0[i], m0, p0);
+        q0 = fma(S1[i], m0, q0);
+      }
+      if (base + a0 < hi) RH[a0 * 64 + lane] = Mb[(base + a0) * kWyLd + lane] - (p0 + p1);
+      if (base + a1 < hi) RH[a1 * 64 + lane] = Mb[(base + a1) * kWyLd + lane] - (q0 + q1);
     }
     __syncthreads();
-    const T* Tb = P.Tp + (size_t)b * kPB * kPB;  // T_bb(a, c) = Tb[a + c * kPB]
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
       const int a = 2 * w + e, c = base + a;
       if (c < hi) {
         T v = T(0);
-        for (int cc = a; cc < hi - base; ++cc) v = fma(Tb[a + cc * kPB], RH[cc * 64 + lane], v);
-        Mb[c * 64 + lane] = v;
+        for (int cc = a; cc < hi - base; ++cc) v = fma(Tl[a + cc * kPB], RH[cc * 64 + lane], v);
+        Mb[c * kWyLd + lane] = v;
       }
     }
     __syncthreads();
+    if (b > 0) {
+      stash();
+      __syncthreads();
+    }
   }
-  if (live)
-    for (int c = 0; c < K; ++c) P.M[c + (size_t)col * K] = Mb[c * 64 + lane];
+  for (int e = tid; e < K * nc; e += kThreads) {
+    const int c = e % K, j = e / K;
+    P.M[c + (size_t)(j0 + j) * K] = Mb[c * kWyLd + j];
+  }
+}
+
+template <typename T>
+static size_t wy_smem(int kmax) {
+  return sizeof(T) * ((size_t)kmax * kWyLd + kPB * 64 + kPB * kWyMaxK + kPB * kPB);
 }
 
 template <typename T>
@@ -2264,14 +2418,12 @@ void launch_wy_solve(const WySolveProb<T>* probs, int n, int kmax, int rmax, hip
   static bool attr = false;
   if (!attr) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wy_solve<T>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)(sizeof(T) * (kWyMaxK + kPB) * 64));
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)wy_smem<T>(kWyMaxK));
     attr = true;
   }
   if (n <= 0 || rmax <= 0) return;
   const int ncb = (rmax + 63) / 64;
-  const size_t sm = sizeof(T) * ((size_t)kmax + kPB) * 64;
-  hipLaunchKernelGGL(k_wy_solve<T>, dim3(n * ncb), dim3(kThreads), sm, s, probs, ncb);
+  hipLaunchKernelGGL(k_wy_solve<T>, dim3(n * ncb), dim3(kThreads), wy_smem<T>(kmax), s, probs, ncb);
 }
 
 // ------------------------------------------------------------------------------
@@ -2315,7 +2467,8 @@ void prepare_kernels() {
 
 // ---- explicit instantiations ---------------------------------------------------
 #define HCORE_INSTANTIATE(T)                                                                  \
-  template void launch_gemm<T>(const GemmProb<T>*, const GemmTask*, int, int, hipStream_t);   \
+  template void launch_gemm<T>(const GemmProb<T>*, const GemmTask*, int, int, hipStream_t,   \
+                                bool);                                                        \
   template void launch_stack<T>(const StackProb<T>*, int, int, int, T, T, hipStream_t);       \
   template void launch_geqrf<T>(const QrProb<T>*, int, int, hipStream_t);                     \
   template void launch_qr_reg<T>(const QrProb<T>*, int, int, int, hipStream_t);               \
