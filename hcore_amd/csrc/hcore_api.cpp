@@ -148,12 +148,12 @@ static int core_kl() { return sizeof(T) == 8 ? g_ctx.core_kl_d : g_ctx.core_kl_f
 // (one wavefront each) for problems with m or n <= 32, which would otherwise
 // compute mostly padding.
 struct TaskList {
-  std::vector<GemmTask> big, small;
-  size_t size() const { return big.size() + small.size(); }
+  std::vector<GemmTask> big, small, tall;
+  size_t size() const { return big.size() + small.size() + tall.size(); }
 };
 struct TaskRef {
-  size_t ob = 0, os = 0;
-  int nb = 0, ns = 0;
+  size_t ob = 0, os = 0, ot = 0;
+  int nb = 0, ns = 0, nt = 0;
 };
 
 struct Packer {
@@ -171,16 +171,39 @@ struct Packer {
     r.nb = (int)tl.big.size();
     r.os = add(tl.small);
     r.ns = (int)tl.small.size();
+    r.ot = add(tl.tall);
+    r.nt = (int)tl.tall.size();
     return r;
   }
 };
 
-static void add_tasks(TaskList& tl, int prob, int m, int n) {
+// Output tiles of one product: tall-skinny blocks (kTallRows x kTallCols) for
+// products of the nb rows of a tile with a few dozen columns, else 64 x 64
+// tiles, or 32 x 32 ones when m or n <= 32.
+template <typename T>
+static void add_tasks(TaskList& tl, int prob, const GemmProb<T>& g) {
+  const int m = g.m, n = g.n;
+  const bool tall = !(g.flags & (GF_A_TRANS | GF_LOWER_ZERO_UPPER)) && m >= kTallMinM &&
+                    n <= kTallMaxN && g.k <= kTallMaxK;
+  if (tall) {
+    for (int tj = 0; tj < (n + kTallCols - 1) / kTallCols; ++tj)
+      for (int ti = 0; ti < (m + kTallRows - 1) / kTallRows; ++ti) tl.tall.push_back({prob, ti, tj});
+    return;
+  }
   const bool sm = m <= 32 || n <= 32;
   const int t = sm ? 32 : 64;
   auto& v = sm ? tl.small : tl.big;
   for (int tj = 0; tj < (n + t - 1) / t; ++tj)
     for (int ti = 0; ti < (m + t - 1) / t; ++ti) v.push_back({prob, ti, tj});
+}
+
+// The launches of one task list: 64-tiles, 32-tiles, tall-skinny blocks.
+template <typename T>
+static void launch_tasks(const GemmProb<T>* probs, const char* base, const TaskRef& r,
+                         hipStream_t s, bool split) {
+  launch_gemm<T>(probs, reinterpret_cast<const GemmTask*>(base + r.ob), r.nb, 64, s, split);
+  launch_gemm<T>(probs, reinterpret_cast<const GemmTask*>(base + r.os), r.ns, 32, s, split);
+  launch_gemm_tall<T>(probs, reinterpret_cast<const GemmTask*>(base + r.ot), r.nt, s);
 }
 
 }  // namespace hcore
@@ -427,27 +450,27 @@ static int plan_finalize_t(hcore_hip_plan* p) {
         g.A = (const T*)p->AV[u]; g.B = (const T*)p->BV[u]; g.C = W + o_F[u];
         g.m = ra; g.n = rb; g.k = M; g.lda = p->lda; g.ldb = p->ldb; g.ldc = ra;
         g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_BETA_ZERO;
-        add_tasks(d.tF, (int)gp.size(), g.m, g.n); gp.push_back(g);
+        add_tasks(d.tF, (int)gp.size(), g); gp.push_back(g);
         // G = BU F^T into Vs(:, rc:K)   (hcore_dgemm.c:199-205)
         g = GemmProb<T>{};
         g.A = (const T*)p->BU[u]; g.B = W + o_F[u]; g.C = Vs + (size_t)rc * M;
         g.m = M; g.n = ra; g.k = rb; g.lda = p->ldb; g.ldb = ra; g.ldc = M;
         g.alpha = T(1); g.beta = T(0); g.flags = GF_B_TRANS | GF_BETA_ZERO;
-        add_tasks(d.tG, (int)gp.size(), g.m, g.n); gp.push_back(g);
+        add_tasks(d.tG, (int)gp.size(), g); gp.push_back(g);
         // T = R_A R_B^T   (hcore_dgemm.c:384-386)
         g = GemmProb<T>{};
         g.A = Us; g.B = Vs; g.C = W + o_Tc[u];
         g.m = K; g.n = K; g.k = K; g.lda = M; g.ldb = M; g.ldc = K;
         g.alpha = T(1); g.beta = T(0);
         g.flags = GF_A_UPPER | GF_B_TRANS | GF_B_UPPER | GF_BETA_ZERO;
-        add_tasks(d.tT, (int)gp.size(), g.m, g.n); gp.push_back(g);
+        add_tasks(d.tT, (int)gp.size(), g); gp.push_back(g);
         // Zv = T^T Zu = V_r S_r
         g = GemmProb<T>{};
         g.A = W + o_Tc[u]; g.B = W + o_Zu[u]; g.C = W + o_Zv[u];
         g.m = K; g.n = K; g.k = K; g.lda = K; g.ldb = K; g.ldc = K;
         g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_BETA_ZERO;
         g.n_dev = d_rank + u; g.skip = d_status + u;
-        add_tasks(d.tZ, (int)gp.size(), g.m, g.n); gp.push_back(g);
+        add_tasks(d.tZ, (int)gp.size(), g); gp.push_back(g);
         // register-resident QR (it also yields the Gram S and the panel factors)
         const bool qreg = lane_wy[l] && g_ctx.qr_reg && K <= kQrRegMaxK && M <= 1024;
         if (lane_wy[l]) {
@@ -468,7 +491,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
             g.alpha = T(1); g.beta = T(0);
             g.flags = GF_A_TRANS | GF_A_UNIT_LOWER | GF_B_UNIT_LOWER | GF_BETA_ZERO;
             if (!qreg) {
-              add_tasks(d.tT, (int)gp.size(), g.m, g.n);
+              add_tasks(d.tT, (int)gp.size(), g);
               gp.push_back(g);
               d.gram = true;
             }
@@ -478,7 +501,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
             g.m = K; g.n = K; g.k = K; g.lda = M; g.ldb = K; g.ldc = K;
             g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_A_UNIT_LOWER | GF_BETA_ZERO;
             g.n_dev = d_rank + u; g.skip = d_status + u;
-            add_tasks(side == 0 ? d.tZ : d.tB, (int)gp.size(), g.m, g.n); gp.push_back(g);
+            add_tasks(side == 0 ? d.tZ : d.tB, (int)gp.size(), g); gp.push_back(g);
             // X = [Z; 0] - Y M into the caller's factor
             g = GemmProb<T>{};
             g.A = Ys[side]; g.B = Ms[side]; g.C = Xo[side];
@@ -486,7 +509,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
             g.alpha = T(-1); g.beta = T(1); g.flags = GF_A_UNIT_LOWER | GF_C0;
             g.C0 = Zs[side]; g.ldc0 = K; g.m0 = K;
             g.n_dev = d_rank + u; g.skip = d_status + u;
-            add_tasks(d.tX, (int)gp.size(), g.m, g.n); gp.push_back(g);
+            add_tasks(d.tX, (int)gp.size(), g); gp.push_back(g);
             d.wy.push_back({Ss[side], side == 0 ? W + o_TA[u] : W + o_TB[u], Ms[side], d_rank + u,
                             d_status + u, K});
           }
@@ -544,17 +567,17 @@ static int plan_finalize_t(hcore_hip_plan* p) {
       g.A = (const T*)p->sAV[s]; g.B = (const T*)p->sAV[s]; g.C = W + o_W[s];
       g.m = K; g.n = K; g.k = p->ldau; g.lda = p->ldau; g.ldb = p->ldau; g.ldc = K;
       g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_BETA_ZERO;
-      add_tasks(tW, (int)sp.size(), g.m, g.n); sp.push_back(g);
+      add_tasks(tW, (int)sp.size(), g); sp.push_back(g);
       g = GemmProb<T>{};
       g.A = (const T*)p->sAU[s]; g.B = W + o_W[s]; g.C = W + o_X[s];
       g.m = M; g.n = K; g.k = K; g.lda = p->ldau; g.ldb = K; g.ldc = M;
       g.alpha = T(1); g.beta = T(0); g.flags = GF_BETA_ZERO;
-      add_tasks(tX, (int)sp.size(), g.m, g.n); sp.push_back(g);
+      add_tasks(tX, (int)sp.size(), g); sp.push_back(g);
       g = GemmProb<T>{};
       g.A = W + o_X[s]; g.B = (const T*)p->sAU[s]; g.C = (T*)p->sCD[s];
       g.m = M; g.n = M; g.k = K; g.lda = M; g.ldb = p->ldau; g.ldc = p->ldcd;
       g.alpha = T(p->salpha); g.beta = T(1); g.flags = GF_B_TRANS | GF_LOWER_ZERO_UPPER;
-      add_tasks(tC, (int)sp.size(), g.m, g.n); sp.push_back(g);
+      add_tasks(tC, (int)sp.size(), g); sp.push_back(g);
     }
     p->o_gp = pk.add(gp);
     if (p->lanes.size() != ld.size()) p->lanes.resize(ld.size());
@@ -701,8 +724,7 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
   const int syrk_lane = nl >= 2 ? nl - 2 : 0;
   // one GEMM phase: its 64-tile and 32-tile launches
   auto gemm = [&](const GemmProb<T>* probs, const TaskRef& r, hipStream_t st, bool split = false) {
-    launch_gemm<T>(probs, reinterpret_cast<const GemmTask*>(D(r.ob)), r.nb, 64, st, split);
-    launch_gemm<T>(probs, reinterpret_cast<const GemmTask*>(D(r.os)), r.ns, 32, st, split);
+    launch_tasks<T>(probs, reinterpret_cast<const char*>(p->desc), r, st, split);
   };
   // phases whose products reduce over the nb rows of the tiles (F = AV^T BV,
   // the stacks' Gram matrices next to T) share each tile's k range
@@ -941,7 +963,7 @@ static int run_gemm_chain(const std::vector<GemmProb<double>>& gp) {
   size_t o_gp = pk.add(gp);
   std::vector<TaskRef> o_t(gp.size());
   for (size_t i = 0; i < gp.size(); ++i) {
-    add_tasks(tl[i], (int)i, gp[i].m, gp[i].n);
+    add_tasks(tl[i], (int)i, gp[i]);
     o_t[i] = pk.add(tl[i]);
   }
   char* d = nullptr;
@@ -950,8 +972,7 @@ static int run_gemm_chain(const std::vector<GemmProb<double>>& gp) {
   HCHECK(hipMemcpyAsync(d, pk.buf.data(), pk.buf.size(), hipMemcpyHostToDevice, s));
   for (size_t i = 0; i < gp.size(); ++i) {
     const auto* probs = reinterpret_cast<const GemmProb<double>*>(d + o_gp);
-    launch_gemm<double>(probs, reinterpret_cast<const GemmTask*>(d + o_t[i].ob), o_t[i].nb, 64, s);
-    launch_gemm<double>(probs, reinterpret_cast<const GemmTask*>(d + o_t[i].os), o_t[i].ns, 32, s);
+    launch_tasks<double>(probs, d, o_t[i], s, false);
   }
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(s);

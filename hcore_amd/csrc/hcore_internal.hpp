@@ -145,6 +145,12 @@ int core_lds_max_k(size_t elem_size);
 template <typename T>
 void launch_gemm(const GemmProb<T>* probs, const GemmTask* tasks, int ntasks, int tile,
                  hipStream_t s, bool split = false);
+// tall-skinny products: A not transposed, m >= kTallMinM, n <= kTallMaxN,
+// k <= kTallMaxK; tasks are (row block of kTallRows, column block of kTallCols)
+constexpr int kTallRows = 512, kTallCols = 32, kTallK = 64;
+constexpr int kTallMinM = 256, kTallMaxN = 128, kTallMaxK = 128;
+template <typename T>
+void launch_gemm_tall(const GemmProb<T>* probs, const GemmTask* tasks, int ntasks, hipStream_t s);
 constexpr int kStackCols = 4;  // stacked columns per workgroup of k_stack
 template <typename T>
 void launch_stack(const StackProb<T>* probs, int n, int nb, int maxcols, T alpha, T beta,

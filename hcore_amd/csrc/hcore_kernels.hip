@@ -400,6 +400,90 @@ __global__ __launch_bounds__((TILE / 4) * (TILE / 4) * KS) void k_gemm_sk(
   if (grp == 0) gemm_store<T, TD>(P, acc, row0, col0, tx, ty, n);
 }
 
+// Tall-skinny batched GEMM (m = the nb rows of a tile, n and k at most a few
+// dozen): C = alpha A op(B) + beta C (or beta C0), A not transposed.  One
+// workgroup covers kTallRows rows (two per thread) and kTallCols columns; the
+// k x kTallCols block of op(B) is staged in LDS in chunks of kTallK rows and
+// read as a broadcast, A is read straight from global memory, coalesced along
+// the rows.  The many tiny output tiles of such products as 2-D tiles would
+// be bound by workgroup dispatch, not by their bytes.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_tall(const GemmProb<T>* __restrict__ probs,
+                                                   const GemmTask* __restrict__ tasks) {
+  __shared__ T Bs[kTallK][kTallCols];
+  const GemmTask tk = tasks[blockIdx.x];
+  const GemmProb<T> P = probs[tk.prob];
+  if (P.skip && *P.skip) return;
+  int n = P.n;
+  if (P.n_dev) n = min(n, *P.n_dev);
+  const int row0 = tk.ti * kTallRows, col0 = tk.tj * kTallCols;
+  if (row0 >= P.m || col0 >= n) return;
+  const int nc = min(kTallCols, n - col0);
+  const int t = threadIdx.x;
+  const int r0 = row0 + t, r1 = row0 + t + 256;
+  const bool tb = (P.flags & GF_B_TRANS) != 0;
+  const bool ua = (P.flags & GF_A_UPPER) != 0, ub = (P.flags & GF_B_UPPER) != 0;
+  const bool la = (P.flags & GF_A_UNIT_LOWER) != 0, lb = (P.flags & GF_B_UNIT_LOWER) != 0;
+  T acc0[kTallCols], acc1[kTallCols];
+#pragma unroll
+  for (int j = 0; j < kTallCols; ++j) acc0[j] = acc1[j] = T(0);
+  for (int k0 = 0; k0 < P.k; k0 += kTallK) {
+    const int kc = min(kTallK, P.k - k0);
+    for (int e = t; e < kTallK * kTallCols; e += 256) {
+      int kk, j;
+      if (tb) { j = e % kTallCols; kk = e / kTallCols; }
+      else    { kk = e % kTallK; j = e / kTallK; }
+      T v = T(0);
+      if (kk < kc && j < nc) {
+        const int gk = k0 + kk, gj = col0 + j;
+        v = tb ? gemm_masked(P.B, P.ldb, gj, gk, ub, lb) : gemm_masked(P.B, P.ldb, gk, gj, ub, lb);
+      }
+      Bs[kk][j] = v;
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int kk = 0; kk < kc; ++kk) {
+      const int gk = k0 + kk;
+      const T a0 = r0 < P.m ? gemm_masked(P.A, P.lda, r0, gk, ua, la) : T(0);
+      const T a1 = r1 < P.m ? gemm_masked(P.A, P.lda, r1, gk, ua, la) : T(0);
+#pragma unroll
+      for (int j = 0; j < kTallCols; ++j) {
+        const T b = Bs[kk][j];
+        acc0[j] = fma(a0, b, acc0[j]);
+        acc1[j] = fma(a1, b, acc1[j]);
+      }
+    }
+    __syncthreads();
+  }
+  const bool bz = (P.flags & GF_BETA_ZERO) != 0;
+  const bool c0 = (P.flags & GF_C0) != 0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int i = h ? r1 : r0;
+    if (i >= P.m) continue;
+#pragma unroll
+    for (int j = 0; j < kTallCols; ++j) {
+      if (j < nc) {
+        const int gj = col0 + j;
+        T* c = P.C + i + (size_t)gj * P.ldc;
+        T v = P.alpha * (h ? acc1[j] : acc0[j]);
+        if (c0) {
+          if (i < P.m0) v = fma(P.beta, P.C0[i + (size_t)gj * P.ldc0], v);
+        } else if (!bz) {
+          v = fma(P.beta, *c, v);
+        }
+        *c = v;
+      }
+    }
+  }
+}
+
+template <typename T>
+void launch_gemm_tall(const GemmProb<T>* probs, const GemmTask* tasks, int ntasks, hipStream_t s) {
+  if (ntasks <= 0) return;
+  hipLaunchKernelGGL(k_gemm_tall<T>, dim3(ntasks), dim3(256), 0, s, probs, tasks);
+}
+
 template <typename T>
 void launch_gemm(const GemmProb<T>* probs, const GemmTask* tasks, int ntasks, int tile,
                  hipStream_t s, bool split) {
@@ -2469,6 +2553,7 @@ void prepare_kernels() {
 #define HCORE_INSTANTIATE(T)                                                                  \
   template void launch_gemm<T>(const GemmProb<T>*, const GemmTask*, int, int, hipStream_t,   \
                                 bool);                                                        \
+  template void launch_gemm_tall<T>(const GemmProb<T>*, const GemmTask*, int, hipStream_t);  \
   template void launch_stack<T>(const StackProb<T>*, int, int, int, T, T, hipStream_t);       \
   template void launch_geqrf<T>(const QrProb<T>*, int, int, hipStream_t);                     \
   template void launch_qr_reg<T>(const QrProb<T>*, int, int, int, hipStream_t);               \
