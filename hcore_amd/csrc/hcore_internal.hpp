@@ -157,13 +157,13 @@ void launch_stack(const StackProb<T>* probs, int n, int nb, int maxcols, T alpha
                   hipStream_t s);
 template <typename T>
 void launch_geqrf(const QrProb<T>* probs, int n, int nb, hipStream_t s);
-// register-resident Householder QR: one thread per row (nb <= 1024), K <= kQrRegMaxK
+// register-resident Householder QR in one pass (nb <= 1024, K <= kQrRegMaxK):
+// kmax <= 16 selects the 16-column kernel, else the 32-column one
 constexpr int kQrRegMaxK = 32;
 template <typename T>
 void launch_qr_reg(const QrProb<T>* probs, int n, int nb, int kmax, hipStream_t s);
-// whole-stack QR in one workgroup (nb <= 1024): kQrPanel-column panels
-// factored in registers, each applied to the trailing columns in place
-constexpr int kQrPanel = 32;
+// whole-stack QR in one workgroup (nb <= 1024): register panels of 16 columns
+// (32 at nb <= 256), each applied to the trailing columns in place
 template <typename T>
 void launch_qr_fused(const QrProb<T>* probs, int n, int nb, hipStream_t s);
 // core SVD variants by stacked rank K: one wavefront for K <= 64, one

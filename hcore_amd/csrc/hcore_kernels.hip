@@ -407,6 +407,8 @@ __global__ __launch_bounds__((TILE / 4) * (TILE / 4) * KS) void k_gemm_sk(
 // read as a broadcast, A is read straight from global memory, coalesced along
 // the rows.  The many tiny output tiles of such products as 2-D tiles would
 // be bound by workgroup dispatch, not by their bytes.
+constexpr int kTallQ = 8;  // k steps whose A loads are issued together
+static_assert(kTallK % kTallQ == 0, "k chunk of the tall GEMM");
 template <typename T>
 __global__ __launch_bounds__(256) void k_gemm_tall(const GemmProb<T>* __restrict__ probs,
                                                    const GemmTask* __restrict__ tasks) {
@@ -441,16 +443,24 @@ __global__ __launch_bounds__(256) void k_gemm_tall(const GemmProb<T>* __restrict
       Bs[kk][j] = v;
     }
     __syncthreads();
-#pragma unroll 2
-    for (int kk = 0; kk < kc; ++kk) {
-      const int gk = k0 + kk;
-      const T a0 = r0 < P.m ? gemm_masked(P.A, P.lda, r0, gk, ua, la) : T(0);
-      const T a1 = r1 < P.m ? gemm_masked(P.A, P.lda, r1, gk, ua, la) : T(0);
+    for (int kq = 0; kq < kc; kq += kTallQ) {
+      // kTallQ columns of A in flight at once
+      T a0[kTallQ], a1[kTallQ];
 #pragma unroll
-      for (int j = 0; j < kTallCols; ++j) {
-        const T b = Bs[kk][j];
-        acc0[j] = fma(a0, b, acc0[j]);
-        acc1[j] = fma(a1, b, acc1[j]);
+      for (int q = 0; q < kTallQ; ++q) {
+        const int gk = k0 + kq + q;
+        const bool in = kq + q < kc;
+        a0[q] = in && r0 < P.m ? gemm_masked(P.A, P.lda, r0, gk, ua, la) : T(0);
+        a1[q] = in && r1 < P.m ? gemm_masked(P.A, P.lda, r1, gk, ua, la) : T(0);
+      }
+#pragma unroll
+      for (int q = 0; q < kTallQ; ++q) {
+#pragma unroll
+        for (int j = 0; j < kTallCols; ++j) {
+          const T b = Bs[kq + q][j];  // zero past kc
+          acc0[j] = fma(a0[q], b, acc0[j]);
+          acc1[j] = fma(a1[q], b, acc1[j]);
+        }
       }
     }
     __syncthreads();
@@ -1108,18 +1118,10 @@ __global__ __launch_bounds__(NT) void k_qr_fused(const QrProb<T>* __restrict__ p
 // Shapes (threads x rows per thread) of the register QR: the stack's rows are
 // spread over NT threads, RPT rows each.  More rows per thread pay the
 // cross-lane reductions once per RPT rows; fewer keep more waves per stack.
-#ifndef HCORE_QR16_NT
-#define HCORE_QR16_NT 256
-#endif
-#ifndef HCORE_QR32_NT
-#define HCORE_QR32_NT 512
-#endif
-#ifndef HCORE_QRF_NT
-#define HCORE_QRF_NT 512
-#endif
-#ifndef HCORE_QRF_PB
-#define HCORE_QRF_PB 16
-#endif
+constexpr int kQr16Threads = 256;  // K <= 16: 4 rows per thread at nb = 1024 (two stacks per CU)
+constexpr int kQr32Threads = 512;  // K <= 32: 2 rows per thread at nb = 1024
+constexpr int kQrfThreads = 512;   // panels: 2 rows per thread at nb = 1024
+constexpr int kQrfPanel = 16;      // panel width of the fused QR (VGPR budget at 2 rows)
 
 template <typename T, int NT, int KMAX>
 static void launch_qr_reg_nt(const QrProb<T>* probs, int n, int nb, hipStream_t s) {
@@ -1132,15 +1134,15 @@ static void launch_qr_reg_nt(const QrProb<T>* probs, int n, int nb, hipStream_t 
 template <typename T>
 void launch_qr_reg(const QrProb<T>* probs, int n, int nb, int kmax, hipStream_t s) {
   if (n <= 0) return;
-  if (kmax <= 16) launch_qr_reg_nt<T, (HCORE_QR16_NT), 16>(probs, n, nb, s);
+  if (kmax <= 16) launch_qr_reg_nt<T, kQr16Threads, 16>(probs, n, nb, s);
   else if (nb <= 256) launch_qr_reg_nt<T, 256, 32>(probs, n, nb, s);
-  else launch_qr_reg_nt<T, (HCORE_QR32_NT), 32>(probs, n, nb, s);
+  else launch_qr_reg_nt<T, kQr32Threads, 32>(probs, n, nb, s);
 }
 
 template <typename T>
 void launch_qr_fused(const QrProb<T>* probs, int n, int nb, hipStream_t s) {
   if (n <= 0) return;
-  constexpr int NT = HCORE_QRF_NT, PB = HCORE_QRF_PB;
+  constexpr int NT = kQrfThreads, PB = kQrfPanel;
   const int rpt = (nb + NT - 1) / NT;
   // nb <= 2 NT (the plans route stacks of at most 1024 rows here)
   if (nb <= 256) hipLaunchKernelGGL((k_qr_fused<T, 256, 1, 32>), dim3(n), dim3(256), 0, s, probs, nb);
