@@ -523,20 +523,21 @@ __global__ __launch_bounds__(kThreads) void k_stack(const StackProb<T>* __restri
   const int c0 = blockIdx.y * kStackCols;
   if (c0 >= ncol) return;
   const int cn = min(kStackCols, ncol - c0);
-  const int total = cn * nb;
-#pragma unroll 4
-  for (int e = threadIdx.x; e < total; e += kThreads) {
-    const int cc = e / nb, row = e - cc * nb;
+  for (int cc = 0; cc < cn; ++cc) {  // uniform: one source column at a time
     const int col = c0 + cc;
+    const T* src;
+    T* dst;
+    T sc;
     if (col < P.rc) {
-      P.Us[row + (size_t)col * nb] = beta * P.CU[row + (size_t)col * P.ldc];
+      src = P.CU + (size_t)col * P.ldc; dst = P.Us + (size_t)col * nb; sc = beta;
     } else if (col < P.rc + P.ra) {
-      const int c = col - P.rc;
-      P.Us[row + (size_t)col * nb] = alpha * P.AU[row + (size_t)c * P.lda];
+      src = P.AU + (size_t)(col - P.rc) * P.lda; dst = P.Us + (size_t)col * nb; sc = alpha;
     } else {
       const int c = col - P.rc - P.ra;
-      P.Vs[row + (size_t)c * nb] = P.CV[row + (size_t)c * P.ldc];
+      src = P.CV + (size_t)c * P.ldc; dst = P.Vs + (size_t)c * nb; sc = T(1);
     }
+#pragma unroll 4
+    for (int row = threadIdx.x; row < nb; row += kThreads) dst[row] = sc * src[row];
   }
 }
 
