@@ -70,6 +70,12 @@ GPU_CASES = [  # (seed, M, ra, rb, rc, alpha, beta, rk, acc)
     (19, 1024, 80, 40, 107, -1.0, 1.0, 0, 1e-8),
     (20, 1024, 70, 60, 90, -1.0, 1.0, 0, 1e-8),   # K = 160: Jacobi outside LDS
     (21, 2048, 24, 24, 40, -1.0, 1.0, 0, 1e-8),
+    # register QR shapes with several rows per thread: K = 16 at 4 rows (nb 1024,
+    # and nb 700 with the last rows masked), K = 30 and 32 at 2 rows
+    (22, 1024, 8, 8, 8, -1.0, 1.0, 0, 1e-8),
+    (23, 700, 6, 9, 10, -1.0, 1.0, 0, 1e-8),
+    (24, 1024, 12, 20, 18, -1.0, 1.0, 0, 1e-8),
+    (25, 600, 20, 10, 12, 1.0, 1.0, 0, 1e-8),
 ]
 
 
