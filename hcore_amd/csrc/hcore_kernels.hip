@@ -1124,20 +1124,26 @@ constexpr int kQr32Threads = 512;  // K <= 32: 2 rows per thread at nb = 1024
 constexpr int kQrfThreads = 512;   // panels: 2 rows per thread at nb = 1024
 constexpr int kQrfPanel = 16;      // panel width of the fused QR (VGPR budget at 2 rows)
 
-template <typename T, int NT, int KMAX>
-static void launch_qr_reg_nt(const QrProb<T>* probs, int n, int nb, hipStream_t s) {
-  const int rpt = (nb + NT - 1) / NT;  // nb <= 4 NT
-  if (rpt <= 1) hipLaunchKernelGGL((k_qr_reg<T, NT, 1, KMAX>), dim3(n), dim3(NT), 0, s, probs, nb, 0);
-  else if (rpt <= 2) hipLaunchKernelGGL((k_qr_reg<T, NT, 2, KMAX>), dim3(n), dim3(NT), 0, s, probs, nb, 0);
-  else hipLaunchKernelGGL((k_qr_reg<T, NT, 4, KMAX>), dim3(n), dim3(NT), 0, s, probs, nb, 0);
+template <typename T, int NT, int RPT, int KMAX>
+static void launch_qr_reg_shape(const QrProb<T>* probs, int n, int nb, hipStream_t s) {
+  hipLaunchKernelGGL((k_qr_reg<T, NT, RPT, KMAX>), dim3(n), dim3(NT), 0, s, probs, nb, 0);
 }
 
+// Only the shapes below are instantiated (nb <= 1024): K <= 16 at 256
+// threads x 1, 2 or 4 rows; K <= 32 at 256 x 1, 512 x 1 or 512 x 2 rows.
 template <typename T>
 void launch_qr_reg(const QrProb<T>* probs, int n, int nb, int kmax, hipStream_t s) {
   if (n <= 0) return;
-  if (kmax <= 16) launch_qr_reg_nt<T, kQr16Threads, 16>(probs, n, nb, s);
-  else if (nb <= 256) launch_qr_reg_nt<T, 256, 32>(probs, n, nb, s);
-  else launch_qr_reg_nt<T, kQr32Threads, 32>(probs, n, nb, s);
+  static_assert(kQr16Threads == 256 && kQr32Threads == 512, "register QR shapes");
+  if (kmax <= 16) {
+    if (nb <= 256) launch_qr_reg_shape<T, 256, 1, 16>(probs, n, nb, s);
+    else if (nb <= 512) launch_qr_reg_shape<T, 256, 2, 16>(probs, n, nb, s);
+    else launch_qr_reg_shape<T, 256, 4, 16>(probs, n, nb, s);
+  } else {
+    if (nb <= 256) launch_qr_reg_shape<T, 256, 1, 32>(probs, n, nb, s);
+    else if (nb <= 512) launch_qr_reg_shape<T, 512, 1, 32>(probs, n, nb, s);
+    else launch_qr_reg_shape<T, 512, 2, 32>(probs, n, nb, s);
+  }
 }
 
 template <typename T>
