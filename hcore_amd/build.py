@@ -14,7 +14,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libhcore_amd.so")
-SOURCES = ["hcore_kernels.hip", "hcore_api.cpp"]
+SOURCES = ["k_qrr_f64.hip", "k_qrf_f64.hip", "k_qrr_f32.hip", "k_qrf_f32.hip", "k_core_reg.hip",
+           "k_core.hip", "k_qr.hip", "k_gemm.hip", "k_misc.hip", "hcore_api.cpp"]
 ARCH = "gfx950"
 
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
@@ -44,7 +45,7 @@ def _compile(src, verbose):
 
 def build(verbose=False):
     os.makedirs(LIBDIR, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
+    with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), SOURCES))
     cmd = [_hipcc(), "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", LIB + ".tmp"] + objs
     if verbose:

@@ -472,7 +472,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
         g.n_dev = d_rank + u; g.skip = d_status + u;
         add_tasks(d.tZ, (int)gp.size(), g); gp.push_back(g);
         // register-resident QR (it also yields the Gram S and the panel factors)
-        const bool qreg = lane_wy[l] && g_ctx.qr_reg && K <= kQrRegMaxK && M <= 1024;
+        const bool qreg = lane_wy[l] && g_ctx.qr_reg && K <= kQrRegMaxK && M <= kQrRegMaxNb;
         if (lane_wy[l]) {
           T* SA = W + o_S[u];
           T* SB = SA + al((size_t)K * K);
@@ -520,7 +520,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
           auto& q = K <= 16 ? d.qrr16 : d.qrr;
           q.push_back({Us, W + o_tA[u], W + o_TA[u], SA, K});
           q.push_back({Vs, W + o_tB[u], W + o_TB[u], SA + al((size_t)K * K), K});
-        } else if (g_ctx.qr_reg && M <= 1024) {  // register panels + in-place trailing updates
+        } else if (g_ctx.qr_reg && M <= kQrRegMaxNb) {  // register panels + in-place trailing updates
           d.qrp.push_back({Us, W + o_tA[u], W + o_TA[u], nullptr, K});
           d.qrp.push_back({Vs, W + o_tB[u], W + o_TB[u], nullptr, K});
         } else {  // blocked Householder QR with LDS panels
@@ -749,12 +749,17 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
     });
     phase(l, 1, ls, true, [&] { gemm(gp, L.tF, ls, split_k); });
     phase(l, 2, ls, true, [&] { gemm(gp, L.tG, ls); });
+    bool shapes_ok = true;
     phase(l, 3, ls, true, [&] {
-      launch_qr_fused<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qrp)), L.n_qrp, M, ls);
-      launch_qr_reg<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qrr)), L.n_qrr, M, 32, ls);
-      launch_qr_reg<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qrr16)), L.n_qrr16, M, 16, ls);
+      shapes_ok &= launch_qr_fused<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qrp)), L.n_qrp, M, ls);
+      shapes_ok &= launch_qr_reg<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qrr)), L.n_qrr, M, 32, ls);
+      shapes_ok &= launch_qr_reg<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qrr16)), L.n_qrr16, M, 16, ls);
       launch_geqrf<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qr)), L.n_qr, M, ls);
     });
+    if (!shapes_ok) {
+      fprintf(stderr, "hcore_amd: register QR routed a stack of %d rows beyond its shapes\n", M);
+      return HCORE_HIP_ERR_ARG;
+    }
     phase(l, 4, ls, true, [&] { gemm(gp, L.tT, ls, split_k && L.gram); });
     // the lane's K buckets, largest first (each bucket a kernel variant with its
     // own LDS size); at most 4 streams per plan, one per hardware queue

@@ -138,7 +138,7 @@ constexpr int kCoreLdsG = 4;  // group width of the largest LDS-resident core va
 // Largest K whose K x K FP64 core fits in LDS next to the Jacobi bookkeeping.
 int core_lds_max_k(size_t elem_size);
 
-// ---- launchers (hcore_kernels.hip) -------------------------------------------
+// ---- launchers (k_*.hip) ------------------------------------------------------
 // tile: 64 (256 threads) or 32 (64 threads, for problems with m or n <= 32);
 // split: the k range of each tile is shared by several thread groups (for
 // k = nb reductions into small outputs)
@@ -157,15 +157,18 @@ void launch_stack(const StackProb<T>* probs, int n, int nb, int maxcols, T alpha
                   hipStream_t s);
 template <typename T>
 void launch_geqrf(const QrProb<T>* probs, int n, int nb, hipStream_t s);
-// register-resident Householder QR in one pass (nb <= 1024, K <= kQrRegMaxK):
-// kmax <= 16 selects the 16-column kernel, else the 32-column one
+// register-resident Householder QR in one pass (nb <= kQrRegMaxNb, K <= kQrRegMaxK):
+// kmax <= 16 selects the 16-column kernel, else the 32-column one.  Returns
+// false (and launches nothing) for a shape beyond those limits.
 constexpr int kQrRegMaxK = 32;
+constexpr int kQrRegMaxNb = 1024;  // rows held by the largest register QR shape (512 x 2)
 template <typename T>
-void launch_qr_reg(const QrProb<T>* probs, int n, int nb, int kmax, hipStream_t s);
-// whole-stack QR in one workgroup (nb <= 1024): register panels of 16 columns
-// (32 at nb <= 256), each applied to the trailing columns in place
+bool launch_qr_reg(const QrProb<T>* probs, int n, int nb, int kmax, hipStream_t s);
+// whole-stack QR in one workgroup (nb <= kQrRegMaxNb): register panels of 16
+// columns (32 at nb <= 256), each applied to the trailing columns in place;
+// false for nb beyond the limit
 template <typename T>
-void launch_qr_fused(const QrProb<T>* probs, int n, int nb, hipStream_t s);
+bool launch_qr_fused(const QrProb<T>* probs, int n, int nb, hipStream_t s);
 // core SVD variants by stacked rank K: one wavefront for K <= 64, one
 // workgroup with the core in LDS up to the LDS limit, 1024 threads with the
 // core in global memory beyond it
@@ -175,7 +178,12 @@ enum : int {
   CORE_R64, CORE_R128, CORE_R192, CORE_NVARIANTS
 };
 int core_variant_for(int K, int lds_max_k, bool reg);
+// the register-resident variants (CORE_R64 .. CORE_R192); launch_core forwards to it
+template <typename T>
+void launch_core_reg(const CoreProb<T>* probs, int n, int variant, int rk, int maxrk, T acc,
+                     hipStream_t s);
 constexpr int kCoreMaxK = 512;
+constexpr int kMaxSweeps = 40;  // Jacobi sweep limit of every core variant
 template <typename T>
 void launch_core(const CoreProb<T>* probs, int n, int variant, int kmax, int rk, int maxrk,
                  T acc, hipStream_t s);

@@ -1,0 +1,6 @@
+// k_qrf_f32.hip — float instantiation of launch_qr_fused (kernels in qr_reg.hpp).
+#include "qr_reg.hpp"
+
+namespace hcore {
+template bool launch_qr_fused<float>(const QrProb<float>*, int, int, hipStream_t);
+}  // namespace hcore

@@ -1,0 +1,6 @@
+// k_qrr_f64.hip — double instantiation of launch_qr_reg (kernels in qr_reg.hpp).
+#include "qr_reg.hpp"
+
+namespace hcore {
+template bool launch_qr_reg<double>(const QrProb<double>*, int, int, int, hipStream_t);
+}  // namespace hcore
