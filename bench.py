@@ -35,6 +35,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12         # B/s
 FP64_PEAK = 78.6e12       # FLOP/s (MI355X dense FP64, matrix and vector)
+# execution unit of each phase's kernels (all FP64 work on the vector ALUs)
+ISSUED_ON = {}
 PHASES = ["stack", "F=AV^T BV", "G=BU F^T", "geqrf", "T=RA RB^T+Gram", "core_svd",
           "wy_solve", "Zv=T^T Zu", "reassembly", "syrk_W", "syrk_X", "syrk_CD"]
 
@@ -92,6 +94,7 @@ def reduced_precision_leg(step, stream, steps, warmup, acc, sample):
         stream.synchronize()
         assert np.all(plan.status() == 0)
         ranks = plan.ranks()
+        out.setdefault("_ranks", {})[name] = ranks.copy()
         worst = 0.0
         for u in sample:
             i, j = step.pairs[u]
@@ -109,6 +112,43 @@ def reduced_precision_leg(step, stream, steps, warmup, acc, sample):
         for t in CUd + CVd + OUd + OVd + [x for pr in panel.values() for x in pr]:
             t.free()
     out["mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    rk = out.pop("_ranks")
+    d = np.abs(rk["mixed"].astype(int) - rk["fp64"].astype(int))
+    out["rank_drift_mixed_vs_fp64"] = {"max": int(d.max()), "updates_differing": int(np.count_nonzero(d))}
+    out["meets_acc"] = {k: bool(out[k]["worst_rel_err"] <= 10 * acc) for k in ("fp64", "mixed")}
+    return out
+
+
+def single_tile_leg(reps):
+    """The nb=500, rank-32 single update (acc 1e-8, maxrank 250): latency of
+    one drop-in HCORE_dgemm call on the GPU (host pointers: staged through
+    the library; device pointers: in place) next to the oracle on the same
+    tile on one host core."""
+    import hcore_amd as hc
+    import oracle  # the CPU baseline leg: the oracle timed on one host core
+    from hcore_amd.tiles import single_tile_update
+    AU, AV, BU, BV, CU, CV = single_tile_update(nb=500, r=32, decay=8.0, key="single")
+    args = (-1.0, AU, AV, BU, BV, 1.0, CU, CV)
+    E = CU @ CV.T - (AU @ AV.T) @ (BU @ BV.T).T
+    out = {}
+    for name, dev in (("gpu_host_pointers_us", False), ("gpu_device_pointers_us", True)):
+        hc.dgemm(*args, maxrk=250, acc=1e-8, device=dev)
+        t = time.perf_counter()
+        for _ in range(reps):
+            U, V, r = hc.dgemm(*args, maxrk=250, acc=1e-8, device=dev)
+        out[name] = (time.perf_counter() - t) / reps * 1e6
+        out["rank"] = r
+        out["rel_err"] = float(np.linalg.norm(U @ V.T - E) / np.linalg.norm(E))
+    oracle.set_blas_threads(1)
+    oracle.dgemm(*args, maxrk=250, acc=1e-8)
+    t = time.perf_counter()
+    for _ in range(reps):
+        st, oU, oV, ro, _, _ = oracle.dgemm(*args, maxrk=250, acc=1e-8)
+    out["cpu_oracle_1core_us"] = (time.perf_counter() - t) / reps * 1e6
+    out["oracle_rank"] = ro
+    out["reps"] = reps
+    out["note"] = ("device-pointer time includes the host-side copies of the factors to and "
+                   "from device buffers made by the Python front end")
     return out
 
 
@@ -121,7 +161,8 @@ def main():
     ap.add_argument("--nb", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the reduced-precision leg")
-    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--single-reps", type=int, default=200)
     ap.add_argument("--check", type=int, default=24, help="updates checked against the oracle")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -285,7 +326,10 @@ def main():
               "G=BU F^T": int(np.sum(2 * nb * ral[sel] * rbl[sel]))}.get(dname, 0)
         ach = fl / (phase_ms[dl, dp] / 1e3)
         roof = {"bound": "mfma", "achieved": ach / 1e12, "peak": FP64_PEAK / 1e12,
-                "unit": "TFLOP/s", "frac": ach / FP64_PEAK, "traffic": None}
+                "unit": "TFLOP/s", "frac": ach / FP64_PEAK, "traffic": None,
+                "issued_on": ISSUED_ON.get(dname, "VALU"),
+                "peak_note": "MI355X FP64 peak, 78.6 TFLOP/s for both the matrix cores and "
+                             "the vector units"}
     roof["kernel"] = dname
     roof["lane"] = lane_names[dl]
     roof["avg_launch_ms"] = float(phase_ms[dl, dp])
@@ -305,17 +349,39 @@ def main():
     # 3e-8, so it is measured at the accuracy it can meet)
     mixed = None
     if not args.no_mixed:
-        mixed = reduced_precision_leg(step, stream, args.steps, args.warmup, 1e-6, sample)
-        log("acc 1e-6: fp64 %.3f ms (worst %.2e), mixed %.3f ms (worst %.2e), mixed/fp64 %.2fx" %
-            (mixed["fp64"]["ms_per_step"], mixed["fp64"]["worst_rel_err"],
-             mixed["mixed"]["ms_per_step"], mixed["mixed"]["worst_rel_err"],
-             mixed["mixed_over_fp64"]))
+        mixed = {}
+        for acc in (1e-8, 1e-6):
+            m = reduced_precision_leg(step, stream, args.steps, args.warmup, acc, sample)
+            mixed["acc_%g" % acc] = m
+            log("acc %g: fp64 %.3f ms (worst %.2e), mixed %.3f ms (worst %.2e), mixed/fp64 %.2fx, "
+                "rank drift %d" % (acc, m["fp64"]["ms_per_step"], m["fp64"]["worst_rel_err"],
+                                   m["mixed"]["ms_per_step"], m["mixed"]["worst_rel_err"],
+                                   m["mixed_over_fp64"], m["rank_drift_mixed_vs_fp64"]["max"]))
+    single = None
+    if not args.no_cpu:
+        single = single_tile_leg(args.single_reps)
+        log("single tile nb 500: gpu %.0f us (host ptrs) %.0f us (device ptrs), oracle %.0f us" %
+            (single["gpu_host_pointers_us"], single["gpu_device_pointers_us"],
+             single["cpu_oracle_1core_us"]))
 
     # ---- CPU baseline (oracle on the host cores) ------------------------------------
     cpu = None
     if not args.no_cpu:
         import oracle  # the CPU baseline leg: the oracle timed on the host cores
-        ncores = min(16, os.cpu_count() or 1)
+        # every core this process may run on, within the OpenMP thread budget the
+        # host sets (16 per GPU on a shared box)
+        avail = len(os.sched_getaffinity(0))
+        omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        ncores = min(avail, omp) if omp > 0 else avail
+        cpu_model = "unknown"
+        try:
+            with open("/proc/cpuinfo") as fh:
+                for line in fh:
+                    if line.startswith("model name"):
+                        cpu_model = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
         AU = [step.panel[i][0] for i, j in step.pairs]
         AV = [step.panel[i][1] for i, j in step.pairs]
         BU = [step.panel[j][0] for i, j in step.pairs]
@@ -344,7 +410,10 @@ def main():
                "sample": "the full step (%d gemm + %d syrk updates), median of %d runs after "
                          "1 warm-up; %s, 1 BLAS thread per update" %
                          (n, ns, args.cpu_reps, oracle.blas_config().split()[0] + " " +
-                          oracle.blas_config().split()[1])}
+                          oracle.blas_config().split()[1]),
+               "gflops": F_total / tmed / 1e9, "nproc": os.cpu_count(), "affinity_cores": avail,
+               "omp_num_threads": omp, "cpu_model": cpu_model,
+               "runs_s": [float(x) for x in times]}
         log("cpu baseline: %.3f s/step on %d threads" % (tmed, ncores))
 
     out = {
@@ -368,6 +437,7 @@ def main():
                    "serialized_lanes_ms_per_step": eager_ms},
         "roofline": roof,
         "reduced_precision": mixed,
+        "single_tile": single,
         "cpu_baseline": cpu,
         "gflops": F_total / (ms_per_step / 1e3) / 1e9,
         "step_roofline_frac": max(F_total / (ms_per_step / 1e3) / FP64_PEAK,

@@ -152,31 +152,45 @@ def _ptr(a):
 
 
 def dgemm(alpha, AU, AV, BU, BV, beta, CU, CV, rk=0, maxrk=None, acc=1e-8, dtype=np.float64,
-          flops=None):
-    """HCORE_dgemm (or HCORE_sgemm for dtype float32) on host numpy factors.
+          flops=None, ldc=None, device=False):
+    """HCORE_dgemm (or HCORE_sgemm for dtype float32) on numpy factors.
 
-    Returns (CU_new, CV_new, rank).  CU/CV are copied into buffers with room
-    for K = rank(C) + rank(A) columns, updated in place by the library.
+    Returns (CU_new, CV_new, rank).  CU/CV are copied into buffers of ldc
+    (default M) rows with room for K = rank(C) + rank(A) columns, updated in
+    place by the library.  device=True passes device copies of all six
+    factors (the library's device-pointer path) instead of host arrays.
     """
     M = AU.shape[0]
     ra, rb, rc = AU.shape[1], BU.shape[1], CU.shape[1]
     if maxrk is None:
         maxrk = M
+    if ldc is None:
+        ldc = M
     cap = max(ra + rc, 1)
     AU, AV, BU, BV = (_fortran(x, dtype) for x in (AU, AV, BU, BV))
-    CUw = np.zeros((M, cap), dtype=dtype, order="F")
-    CVw = np.zeros((M, cap), dtype=dtype, order="F")
-    CUw[:, :rc] = CU
-    CVw[:, :rc] = CV
+    CUw = np.zeros((ldc, cap), dtype=dtype, order="F")
+    CVw = np.zeros((ldc, cap), dtype=dtype, order="F")
+    CUw[:M, :rc] = CU
+    CVw[:M, :rc] = CV
     rks = np.array([ra, rb, rc], dtype=dtype)
     fc = flops if flops is not None else FlopCounter()
     L = lib()
     fn = L.HCORE_dgemm if dtype == np.float64 else L.HCORE_sgemm
-    fn(111, 112, M, M, alpha, _ptr(AU), _ptr(AV), _ptr(rks[0:1]), M, _ptr(BU), _ptr(BV),
-       _ptr(rks[1:2]), M, beta, _ptr(CUw), _ptr(CVw), _ptr(rks[2:3]), M, rk, maxrk, acc, None,
+    ptr_t = _fp if np.dtype(dtype) == np.float32 else _dp
+    if device:
+        bufs = [DeviceBuffer.from_numpy(x) for x in (AU, AV, BU, BV, CUw, CVw)]
+        P = [C.cast(C.c_void_p(b.ptr), ptr_t) for b in bufs]
+    else:
+        P = [_ptr(x) for x in (AU, AV, BU, BV, CUw, CVw)]
+    fn(111, 112, M, M, alpha, P[0], P[1], _ptr(rks[0:1]), M, P[2], P[3],
+       _ptr(rks[1:2]), M, beta, P[4], P[5], _ptr(rks[2:3]), ldc, rk, maxrk, acc, None,
        C.byref(fc))
     r = int(rks[2])
-    return CUw[:, :r].copy(order="F"), CVw[:, :r].copy(order="F"), r
+    if device:
+        CUw, CVw = bufs[4].numpy(), bufs[5].numpy()
+        for b in bufs:
+            b.free()
+    return CUw[:M, :r].copy(order="F"), CVw[:M, :r].copy(order="F"), r
 
 
 def dsyrk(alpha, AU, AV, CD, flops=None):
@@ -324,8 +338,10 @@ class Plan:
         self.n = 0
 
     def add_dgemm(self, AU, AV, ra, lda, BU, BV, rb, ldb, CU, CV, rc, ldc, alpha, beta, rk,
-                  maxrk, acc, CUout=None, CVout=None, ldco=0):
+                  maxrk, acc, CUout=None, CVout=None, ldco=None):
         n = len(AU)
+        if ldco is None:
+            ldco = self.M
         arrs = [_parr(x) for x in (AU, AV, BU, BV, CU, CV)]
         (ra_a, ra_p), (rb_a, rb_p), (rc_a, rc_p) = _iarr(ra), _iarr(rb), _iarr(rc)
         cuo = _parr(CUout) if CUout is not None else None

@@ -14,8 +14,10 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libhcore_amd.so")
-SOURCES = ["k_qrr_f64.hip", "k_qrf_f64.hip", "k_qrr_f32.hip", "k_qrf_f32.hip", "k_core_reg.hip",
-           "k_core.hip", "k_qr.hip", "k_gemm.hip", "k_misc.hip", "hcore_api.cpp"]
+# every translation unit of csrc/, slowest first (they compile in parallel)
+_FIRST = ["k_qrr_f64.hip", "k_qrf_f64.hip", "k_qrr_f32.hip", "k_qrf_f32.hip", "k_core_reg.hip"]
+SOURCES = _FIRST + sorted(f for f in os.listdir(CSRC)
+                          if f.endswith((".hip", ".cpp")) and f not in _FIRST)
 ARCH = "gfx950"
 
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,

@@ -72,14 +72,11 @@ struct Context {
   std::mutex mu;
   bool ready = false;
   int device = 0;
-  hipStream_t stream = nullptr;
   int policy = HCORE_HIP_ON_ERROR_EXIT;
   int core_kl_d = 0, core_kl_f = 0;
   bool core_reg = true;  // register-resident core SVD for 33 <= K <= 192
   bool qr_reg = true;    // register-resident Householder QR for K <= kQrRegMaxK
-  // grow-only device arena used by the single-call entry points
-  char* arena = nullptr;
-  size_t arena_cap = 0;
+  int generation = 0;    // bumped by HCORE_hip_finalize: per-thread state is stale
 };
 static Context g_ctx;
 static thread_local int t_last_status = HCORE_HIP_OK;
@@ -124,18 +121,86 @@ static int ctx_init_locked(int device) {
   return HCORE_HIP_OK;
 }
 
-// Stream of the single-tile entry points, created on first use: the box
-// exposes four hardware queues per process (HIP's default), and a batched
-// plan's lanes should not share them with an idle library stream.
-static hipStream_t single_stream() {
-  if (!g_ctx.stream) (void)hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking);
-  return g_ctx.stream;
-}
-
 static int ctx_ensure() {
   if (g_ctx.ready) return HCORE_HIP_OK;
   std::lock_guard<std::mutex> lk(g_ctx.mu);
   return ctx_init_locked(-1);
+}
+
+// Grow-only device buffer.  Growing drops the contents; callers grow it only
+// when no work of theirs on it is in flight.
+struct DevArena {
+  char* p = nullptr;
+  size_t cap = 0;
+  int reserve(size_t bytes) {
+    if (cap >= bytes) return HCORE_HIP_OK;
+    if (p) HCHECK(hipFree(p));
+    p = nullptr;
+    cap = 0;
+    const size_t mib = size_t(1) << 20;
+    const size_t c = ((std::max(bytes, cap * 2) + mib - 1) / mib) * mib;
+    HCHECK(hipMalloc(&p, c));
+    cap = c;
+    return HCORE_HIP_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+// Per-thread state of the single-tile entry points (HCORE_dgemm, HCORE_sgemm,
+// HCORE_dsyrk, HCORE_dgemmcd): a stream of its own, and grow-only device
+// buffers for the one-update plan (workspace and descriptors), the staging
+// of host operands and the GEMM chains, plus pinned host words for the rank
+// and status read-back.  The runtime above HCORE calls it from many worker
+// threads at once, one tile each; with per-thread state those calls share
+// no lock and overlap on the GPU, and a call allocates nothing once its
+// thread's buffers have grown to the tile size.
+struct ThreadCtx {
+  hipStream_t stream = nullptr;
+  DevArena plan, stage, chain;
+  std::vector<char> desc;  // host image of the last descriptors (alive until the stream syncs)
+  int* hres = nullptr;     // pinned: [0] rank, [1] status
+  int generation = -1;
+};
+static std::mutex g_tctx_mu;
+static std::vector<ThreadCtx*> g_tctx;  // every thread's state, released by HCORE_hip_finalize
+static thread_local ThreadCtx* t_ctx = nullptr;
+
+static ThreadCtx* thread_ctx() {
+  if (t_ctx && t_ctx->generation == g_ctx.generation) return t_ctx;
+  ThreadCtx* c = new ThreadCtx();
+  c->generation = g_ctx.generation;
+  if (hipSetDevice(g_ctx.device) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&c->hres), 4 * sizeof(int), hipHostMallocDefault) !=
+          hipSuccess) {
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return nullptr;
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_tctx_mu);
+    g_tctx.push_back(c);
+  }
+  t_ctx = c;
+  return c;
+}
+
+static void release_thread_ctxs() {
+  std::lock_guard<std::mutex> lk(g_tctx_mu);
+  for (ThreadCtx* c : g_tctx) {
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->plan.release();
+    c->stage.release();
+    c->chain.release();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->hres) (void)hipHostFree(c->hres);
+    delete c;
+  }
+  g_tctx.clear();
 }
 
 template <typename T>
@@ -190,7 +255,10 @@ static void add_tasks(TaskList& tl, int prob, const GemmProb<T>& g) {
       for (int ti = 0; ti < (m + kTallRows - 1) / kTallRows; ++ti) tl.tall.push_back({prob, ti, tj});
     return;
   }
-  const bool sm = m <= 32 || n <= 32;
+  // 32-tiles for thin outputs, and for long reductions (k = nb) into outputs
+  // of at most 128 x 128, whose phases share each tile's k range among four
+  // wavefronts (split-k)
+  const bool sm = m <= 32 || n <= 32 || (g.k >= 256 && m <= 128 && n <= 128);
   const int t = sm ? 32 : 64;
   auto& v = sm ? tl.small : tl.big;
   for (int tj = 0; tj < (n + t - 1) / t; ++tj)
@@ -215,7 +283,13 @@ struct hcore_hip_plan {
   int M = 0;
   int prec = HCORE_HIP_FP64;
   bool finalized = false;
-  bool use_arena = false;  // workspace from the context arena (single calls)
+  // single-call plans: device memory from a per-thread arena (or a fixed
+  // buffer), set up asynchronously on `fin_stream`, no events or lane streams
+  hcore::DevArena* arena = nullptr;
+  char* fixed = nullptr;          // fixed device buffer instead of an arena
+  size_t fixed_cap = 0;
+  hipStream_t fin_stream = nullptr;
+  std::vector<char>* desc_host = nullptr;  // keeps the uploaded descriptors alive
   // gemm group
   int ng = 0;
   std::vector<void*> AU, AV, BU, BV, CU, CV, CUo, CVo;
@@ -230,7 +304,7 @@ struct hcore_hip_plan {
   int ldau = 0, ldcd = 0;
   double salpha = -1;
   // device memory
-  char* dmem = nullptr;  // owned unless use_arena
+  char* dmem = nullptr;  // owned by the plan unless single-call
   size_t dbytes = 0;
   size_t ws_bytes = 0;
   char* desc = nullptr;
@@ -630,15 +704,14 @@ static int plan_finalize_t(hcore_hip_plan* p) {
   const size_t wsb = (ws_elems * es + 255) & ~size_t(255);
   const size_t total = head + wsb + pk.buf.size() + 256;
   char* base = nullptr;
-  if (p->use_arena) {
-    if (g_ctx.arena_cap < total) {
-      if (g_ctx.arena) HCHECK(hipFree(g_ctx.arena));
-      g_ctx.arena = nullptr;
-      size_t cap = std::max(total, g_ctx.arena_cap * 2);
-      HCHECK(hipMalloc(&g_ctx.arena, cap));
-      g_ctx.arena_cap = cap;
-    }
-    base = g_ctx.arena;
+  const bool single = p->arena || p->fixed;
+  if (p->fixed) {
+    if (total > p->fixed_cap) return HCORE_HIP_ERR_ARG;
+    base = p->fixed;
+  } else if (p->arena) {
+    int st = p->arena->reserve(total);
+    if (st) return st;
+    base = p->arena->p;
   } else {
     HCHECK(hipMalloc(&base, total));
     p->dmem = base;
@@ -654,6 +727,14 @@ static int plan_finalize_t(hcore_hip_plan* p) {
   p->ws = base + head;
   p->desc = base + head + wsb;
   build(p->ws, p->d_rank, p->d_status);
+  if (single) {  // asynchronous on the caller's stream; the host image outlives the copy
+    p->desc_host->swap(pk.buf);
+    HCHECK(hipMemcpyAsync(p->desc, p->desc_host->data(), p->desc_host->size(),
+                          hipMemcpyHostToDevice, p->fin_stream));
+    HCHECK(hipMemsetAsync(base, 0, head, p->fin_stream));
+    p->finalized = true;
+    return HCORE_HIP_OK;
+  }
   HCHECK(hipMemcpy(p->desc, pk.buf.data(), pk.buf.size(), hipMemcpyHostToDevice));
   HCHECK(hipMemset(base, 0, head));
   HCHECK(hipDeviceSynchronize());
@@ -733,11 +814,11 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
     if (p->ns <= 0) return;
     const auto* sp = reinterpret_cast<const GemmProb<T>*>(D(p->o_sp));
     const int lt = hcore_hip_plan::kMaxLanes - 1;
-    phase(lt, 9, ss, true, [&] { gemm(sp, p->tW, ss); });
+    phase(lt, 9, ss, true, [&] { gemm(sp, p->tW, ss, split_k); });
     phase(lt, 10, ss, true, [&] { gemm(sp, p->tXs, ss); });
     phase(lt, 11, ss, true, [&] { gemm(sp, p->tC, ss); });
   };
-  (void)hipEventRecord(p->ev_start, s);
+  if (p->ev_start) (void)hipEventRecord(p->ev_start, s);
   for (int l = 0; l < nl; ++l) {
     auto& L = p->lanes[l];
     const bool own = l > 0 && !(tm && p->serial);  // the lane runs on a stream of its own
@@ -823,33 +904,62 @@ static void fail(int status, const char* msg) {
 // ------------------------------------------------------------------------------
 // single-call recompression (HCORE_dgemm / HCORE_sgemm)
 // ------------------------------------------------------------------------------
+// Read one rank argument (a double/float on the host or on the device).
+template <typename T>
+static bool read_rank(const T* p, T* out) {
+  if (!is_device_ptr(p)) {
+    *out = *p;
+    return true;
+  }
+  return hipMemcpy(out, p, sizeof(T), hipMemcpyDeviceToHost) == hipSuccess;
+}
+
+// Bytes of the caller's `work` under use_scratch: the layout the reference
+// carves from it (hcore_dgemm.c:760-923; use_trmm = 0), clones of
+// M x 2 maxrk, tau, F, newU, newV, R_A, R_B, T, sigma and superb.  A device
+// `work` of that size holds this library's one-update workspace instead
+// (ranks that cannot be read here give 0: the thread's own arena is used).
+template <typename T>
+static size_t reference_work_bytes(int M, int maxrk, const T* Ark, const T* Crk, size_t elem) {
+  if (!Ark || !Crk || is_device_ptr(Ark) || is_device_ptr(Crk) || M <= 0 || maxrk <= 0) return 0;
+  const size_t m = M, mk = maxrk, K = (size_t)(int)Ark[0] + (size_t)(int)Crk[0];
+  const size_t rows = std::min(m, K);
+  const size_t n = 2 * m * 2 * mk + 2 * m + mk * mk + std::max(m, mk) * mk + m * mk +
+                   2 * rows * K + K * K + 2 * K;
+  return n * elem;
+}
+
+// Where a single call's device memory comes from: the calling thread's grow-only
+// arenas, or (scratch != nullptr) one fixed buffer that holds the plan and
+// nothing else (the caller's factors must then be device pointers).
+struct SingleMem {
+  ThreadCtx* tc = nullptr;
+  char* scratch = nullptr;
+  size_t scratch_cap = 0;
+  hipStream_t stream = nullptr;
+  std::vector<char>* desc = nullptr;
+  int* hres = nullptr;  // two ints the rank and status are read back into
+};
+
 template <typename T>
 static void single_gemm(int M, T alpha, T* AU, T* AV, T* Ark, int LDA, T* BU, T* BV, T* Brk,
                         int LDB, T beta, T* CU, T* CV, T* Crk, int LDC, int rk, int maxrk, T acc,
-                        flop_counter* flops, const char* name) {
+                        flop_counter* flops, const char* name, void* work = nullptr,
+                        size_t work_bytes = 0, SingleMem* fixed = nullptr) {
   t_last_status = HCORE_HIP_OK;
   char msg[512];
   // ranks may live on the host or the device
   T rks[3];
-  T* rp[3] = {Ark, Brk, Crk};
-  for (int i = 0; i < 3; ++i) {
-    if (is_device_ptr(rp[i])) {
-      if (hipMemcpy(&rks[i], rp[i], sizeof(T), hipMemcpyDeviceToHost) != hipSuccess) {
-        fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: cannot read rank");
-        return;
-      }
-    } else {
-      rks[i] = *rp[i];
-    }
-  }
+  if (!read_rank(Ark, &rks[0]) || !read_rank(Brk, &rks[1]) || !read_rank(Crk, &rks[2]))
+    return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: cannot read rank");
   const int ra = (int)rks[0], rb = (int)rks[1], rc = (int)rks[2];
   if (ra == 0 || rb == 0 || rc == 0) {
     snprintf(msg, sizeof msg, "%s: _Ark=%d _Brk=%d _Crk=%d. These rank values should not be zero.",
              name, ra, rb, rc);
     return fail(HCORE_HIP_ERR_ZERO_RANK, msg);
   }
-  if (ra < 0 || rb < 0 || rc < 0 || M <= 0) {
-    return fail(HCORE_HIP_ERR_ARG, "hcore_amd: negative rank or size");
+  if (ra < 0 || rb < 0 || rc < 0 || M <= 0 || LDA < M || LDB < M || LDC < M) {
+    return fail(HCORE_HIP_ERR_ARG, "hcore_amd: negative rank, size or leading dimension below M");
   }
   const int K = ra + rc;
   if (K > 2 * maxrk) {
@@ -857,43 +967,69 @@ static void single_gemm(int M, T alpha, T* AU, T* AV, T* Ark, int LDA, T* BU, T*
              name, K, ra, rc, maxrk, 2 * maxrk);
     return fail(HCORE_HIP_ERR_SUM_RANKS, msg);
   }
-  if (maxrk < K || K > M) {
+  if (maxrk < K) {
     snprintf(msg, sizeof msg, "%s: Increase maxrank. %d is not enough ld_newV:%d CU_ncols:%d",
              name, maxrk, maxrk, K);
     return fail(HCORE_HIP_ERR_K_GT_MAXRK, msg);
+  }
+  if (K > M) {  // the stacks would be wider than tall
+    snprintf(msg, sizeof msg,
+             "%s: stacked rank %d exceeds the tile size %d (a rank above M/2 is not low rank)",
+             name, K, M);
+    return fail(HCORE_HIP_ERR_K_GT_MAXRK, msg);
+  }
+  if (K > kCoreMaxK) {
+    snprintf(msg, sizeof msg, "%s: stacked rank %d exceeds the supported maximum %d", name, K,
+             kCoreMaxK);
+    return fail(HCORE_HIP_ERR_ARG, msg);
   }
   // argument checks above need no GPU; everything below runs on it
   if (ctx_ensure() != HCORE_HIP_OK) {
     fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: GPU initialisation failed");
     return;
   }
-  std::lock_guard<std::mutex> lk(g_ctx.mu);
-  hipStream_t s = single_stream();
-  if (K > kCoreMaxK) {
-    snprintf(msg, sizeof msg, "%s: stacked rank %d exceeds the supported maximum %d", name, K,
-             kCoreMaxK);
-    return fail(HCORE_HIP_ERR_ARG, msg);
+  SingleMem mem;
+  if (fixed) {
+    mem = *fixed;
+  } else {
+    mem.tc = thread_ctx();
+    if (!mem.tc) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: per-thread GPU state failed");
+    mem.stream = mem.tc->stream;
+    mem.desc = &mem.tc->desc;
+    mem.hres = mem.tc->hres;
   }
+  if (mem.tc && work_bytes && work && is_device_ptr(work)) {  // the caller's device workspace
+    mem.scratch = reinterpret_cast<char*>(work);
+    mem.scratch_cap = work_bytes;
+  }
+  hipStream_t s = mem.stream;
   const bool dev = is_device_ptr(CU) && is_device_ptr(CV) && is_device_ptr(AU) &&
                    is_device_ptr(AV) && is_device_ptr(BU) && is_device_ptr(BV);
+  if (!dev && !mem.tc)
+    return fail(HCORE_HIP_ERR_ARG, "hcore_amd: this entry point takes device pointers only");
   hcore_hip_plan p;
   p.M = M;
   p.prec = sizeof(T) == 8 ? HCORE_HIP_FP64 : HCORE_HIP_FP32;
-  p.use_arena = true;
   p.ng = 1;
   p.ra = {ra}; p.rb = {rb}; p.rc = {rc};
   p.alpha = alpha; p.beta = beta; p.rk = rk; p.maxrk = maxrk; p.acc = acc;
-  // staging buffers live in a separate block of the arena after the plan
+  p.fin_stream = s;
+  p.desc_host = mem.desc;
+  if (mem.scratch) {
+    p.fixed = mem.scratch;
+    p.fixed_cap = mem.scratch_cap;
+  } else {
+    p.arena = &mem.tc->plan;
+  }
+  // host operands are staged in the thread's staging arena (M rows, packed)
   T *dAU = AU, *dAV = AV, *dBU = BU, *dBV = BV, *dCU = CU, *dCV = CV;
   T *oCU = nullptr, *oCV = nullptr;
-  void* stage = nullptr;
   if (!dev) {
     const size_t nA = (size_t)M * ra, nB = (size_t)M * rb, nC = (size_t)M * K;
     const size_t bytes = sizeof(T) * (2 * nA + 2 * nB + 4 * nC) + 1024;
-    if (hipMalloc(&stage, bytes) != hipSuccess) {
+    if (mem.tc->stage.reserve(bytes) != HCORE_HIP_OK)
       return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: staging allocation failed");
-    }
-    T* q = (T*)stage;
+    T* q = reinterpret_cast<T*>(mem.tc->stage.p);
     dAU = q; q += nA; dAV = q; q += nA; dBU = q; q += nB; dBV = q; q += nB;
     dCU = q; q += nC; dCV = q; q += nC; oCU = q; q += nC; oCV = q;
     auto up = [&](T* d, const T* h, int ld, int cols) {
@@ -901,10 +1037,8 @@ static void single_gemm(int M, T alpha, T* AU, T* AV, T* Ark, int LDA, T* BU, T*
                               hipMemcpyHostToDevice, s);
     };
     if (up(dAU, AU, LDA, ra) || up(dAV, AV, LDA, ra) || up(dBU, BU, LDB, rb) ||
-        up(dBV, BV, LDB, rb) || up(dCU, CU, LDC, rc) || up(dCV, CV, LDC, rc)) {
-      hipFree(stage);
+        up(dBV, BV, LDB, rb) || up(dCU, CU, LDC, rc) || up(dCV, CV, LDC, rc))
       return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: host-to-device staging failed");
-    }
     p.lda = M; p.ldb = M; p.ldc = M; p.ldco = M;
     p.CUo = {oCU}; p.CVo = {oCV};
   } else {
@@ -913,26 +1047,36 @@ static void single_gemm(int M, T alpha, T* AU, T* AV, T* Ark, int LDA, T* BU, T*
   }
   p.AU = {dAU}; p.AV = {dAV}; p.BU = {dBU}; p.BV = {dBV}; p.CU = {dCU}; p.CV = {dCV};
   int st = plan_finalize_t<T>(&p);
+  if (st == HCORE_HIP_ERR_ARG && mem.scratch && mem.tc) {  // caller's work too small: own arena
+    p.fixed = nullptr;
+    p.fixed_cap = 0;
+    p.arena = &mem.tc->plan;
+    st = plan_finalize_t<T>(&p);
+  }
+  if (st == HCORE_HIP_ERR_ARG && mem.scratch) {
+    snprintf(msg, sizeof msg, "%s: M=%d, K=%d needs more than the %zu-byte workspace", name, M, K,
+             mem.scratch_cap);
+    return fail(HCORE_HIP_ERR_ARG, msg);
+  }
   if (st == HCORE_HIP_OK) st = plan_execute_t<T>(&p, s);
-  int r = 0, dst = 0;
+  int* hres = mem.hres;
   if (st == HCORE_HIP_OK) {
-    if (hipMemcpyAsync(&r, p.d_rank, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipMemcpyAsync(&dst, p.d_status, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+    if (hipMemcpyAsync(hres, p.d_rank, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(hres + 1, p.d_status, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
       st = HCORE_HIP_ERR_RUNTIME;
   }
   if (st != HCORE_HIP_OK) {
-    if (stage) hipFree(stage);
+    (void)hipStreamSynchronize(s);
     return fail(st, "hcore_amd: GPU execution failed");
   }
+  const int r = hres[0], dst = hres[1];
   if (dst == ST_R_GT_MAXRK) {
-    if (stage) hipFree(stage);
     snprintf(msg, sizeof msg, "%s: Rank after truncation is too big! finalrank:%d maxrank:%d", name,
              r, maxrk);
     return fail(HCORE_HIP_ERR_R_GT_MAXRK, msg);
   }
   if (dst != ST_OK) {
-    if (stage) hipFree(stage);
     snprintf(msg, sizeof msg, "%s: ERROR in the core SVD (non-finite values), K=%d", name, K);
     return fail(HCORE_HIP_ERR_NUMERIC, msg);
   }
@@ -942,12 +1086,13 @@ static void single_gemm(int M, T alpha, T* AU, T* AV, T* Ark, int LDA, T* BU, T*
     hipError_t e2 = hipMemcpy2DAsync(CV, sizeof(T) * LDC, oCV, sizeof(T) * M, sizeof(T) * M, r,
                                      hipMemcpyDeviceToHost, s);
     hipError_t e3 = hipStreamSynchronize(s);
-    hipFree(stage);
     if (e1 || e2 || e3) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: device-to-host copy failed");
   }
   if (is_device_ptr(Crk)) {
     T rv = (T)r;
-    (void)hipMemcpy(Crk, &rv, sizeof(T), hipMemcpyHostToDevice);
+    if (hipMemcpyAsync(Crk, &rv, sizeof(T), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: cannot write rank");
   } else {
     Crk[0] = (T)r;
   }
@@ -961,8 +1106,9 @@ static void single_gemm(int M, T alpha, T* AU, T* AV, T* Ark, int LDA, T* BU, T*
   }
 }
 
-// Run a short list of dependent GEMMs on the context stream (dsyrk, dgemmcd).
-static int run_gemm_chain(const std::vector<GemmProb<double>>& gp) {
+// Run a short list of dependent GEMMs on the calling thread's stream (dsyrk,
+// dgemmcd); the descriptors go to the thread's chain arena.
+static int run_gemm_chain(ThreadCtx* tc, const std::vector<GemmProb<double>>& gp) {
   std::vector<TaskList> tl(gp.size());
   Packer pk;
   size_t o_gp = pk.add(gp);
@@ -971,41 +1117,86 @@ static int run_gemm_chain(const std::vector<GemmProb<double>>& gp) {
     add_tasks(tl[i], (int)i, gp[i]);
     o_t[i] = pk.add(tl[i]);
   }
-  char* d = nullptr;
-  HCHECK(hipMalloc(&d, pk.buf.size() + 256));
-  hipStream_t s = single_stream();
-  HCHECK(hipMemcpyAsync(d, pk.buf.data(), pk.buf.size(), hipMemcpyHostToDevice, s));
+  int st = tc->chain.reserve(pk.buf.size() + 256);
+  if (st) return st;
+  char* d = tc->chain.p;
+  hipStream_t s = tc->stream;
+  tc->desc.swap(pk.buf);
+  HCHECK(hipMemcpyAsync(d, tc->desc.data(), tc->desc.size(), hipMemcpyHostToDevice, s));
   for (size_t i = 0; i < gp.size(); ++i) {
     const auto* probs = reinterpret_cast<const GemmProb<double>*>(d + o_gp);
-    launch_tasks<double>(probs, d, o_t[i], s, false);
+    launch_tasks<double>(probs, d, o_t[i], s, true);
   }
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  hipFree(d);
   if (e != hipSuccess) return hip_fail(e, "gemm chain");
   return HCORE_HIP_OK;
 }
 
 // Device copies of host matrices (rows x cols, leading dimension ld) for the
-// dense-output routines; `dev` true means the caller's pointers are used.
+// dense-output routines, carved from the thread's staging arena; `dev` true
+// means the caller's pointers are used as they are.
 struct Staged {
-  std::vector<void*> bufs;
-  ~Staged() {
-    for (void* b : bufs) hipFree(b);
+  struct Req {
+    const double* h;
+    int rows, ld, cols;
+    bool upload;
+  };
+  std::vector<Req> req;
+  int add(const double* h, int rows, int ld, int cols, bool upload) {
+    req.push_back({h, rows, ld, std::max(cols, 1), upload});
+    return (int)req.size() - 1;
   }
-  double* get(const double* h, int rows, int ld, int cols, bool dev, bool upload) {
-    if (dev) return const_cast<double*>(h);
-    double* d = nullptr;
-    if (hipMalloc(&d, sizeof(double) * (size_t)rows * std::max(cols, 1)) != hipSuccess)
-      return nullptr;
-    bufs.push_back(d);
-    if (upload && cols > 0 &&
-        hipMemcpy2D(d, sizeof(double) * rows, h, sizeof(double) * ld, sizeof(double) * rows, cols,
-                    hipMemcpyHostToDevice) != hipSuccess)
-      return nullptr;
-    return d;
+  // stage every request (dev: only the scratch ones); out[i] receives the
+  // device address of request i
+  int place(ThreadCtx* tc, bool dev, std::vector<double*>& out) {
+    size_t total = 0;
+    for (const Req& r : req)
+      if (!(dev && r.h)) total += (((size_t)r.rows * r.cols * sizeof(double)) + 255) & ~size_t(255);
+    int st = tc->stage.reserve(total + 256);
+    if (st) return st;
+    char* q = tc->stage.p;
+    out.assign(req.size(), nullptr);
+    for (size_t i = 0; i < req.size(); ++i) {
+      const Req& r = req[i];
+      if (dev && r.h) {
+        out[i] = const_cast<double*>(r.h);
+        continue;
+      }
+      out[i] = reinterpret_cast<double*>(q);
+      q += (((size_t)r.rows * r.cols * sizeof(double)) + 255) & ~size_t(255);
+      if (r.upload &&
+          hipMemcpy2DAsync(out[i], sizeof(double) * r.rows, r.h, sizeof(double) * r.ld,
+                           sizeof(double) * r.rows, r.cols, hipMemcpyHostToDevice,
+                           tc->stream) != hipSuccess)
+        return HCORE_HIP_ERR_RUNTIME;
+    }
+    return HCORE_HIP_OK;
   }
 };
+
+// One HCORE_dgemm on device pointers whose one-update workspace, descriptors
+// and status words go to a fixed device buffer, on stream s: nothing is
+// allocated on the device.  Fails (policy, status) when the update does not
+// fit in ws_bytes.
+void dgemm_fixed_workspace(int M, double alpha, double* AU, double* AV, double* Ark, int LDA,
+                           double* BU, double* BV, double* Brk, int LDB, double beta, double* CU,
+                           double* CV, double* Crk, int LDC, int rk, int maxrk, double acc,
+                           flop_counter* flops, const char* name, char* ws, size_t ws_bytes,
+                           hipStream_t s) {
+  SingleMem mem;
+  mem.scratch = ws;
+  mem.scratch_cap = ws_bytes;
+  mem.stream = s;
+  std::vector<char> desc;  // host image of the descriptors, alive until the call returns
+  int hres[2] = {0, 0};
+  mem.desc = &desc;
+  mem.hres = hres;
+  single_gemm<double>(M, alpha, AU, AV, Ark, LDA, BU, BV, Brk, LDB, beta, CU, CV, Crk, LDC, rk,
+                      maxrk, acc, flops, name, nullptr, 0, &mem);
+}
+
+void report_failure(int status, const char* msg) { fail(status, msg); }
 
 }  // namespace hcore
 
@@ -1020,18 +1211,20 @@ void HCORE_dgemm(HCORE_enum transA, int transB, int M, int N, double alpha, doub
                  double* AV, double* Ark, int LDA, double* BU, double* BV, double* Brk, int LDB,
                  double beta, double* CU, double* CV, double* Crk, int LDC, int rk, int maxrk,
                  double acc, double* work, flop_counter* flops) {
-  (void)transA; (void)transB; (void)N; (void)work;
+  (void)transA; (void)transB; (void)N;
   single_gemm<double>(M, alpha, AU, AV, Ark, LDA, BU, BV, Brk, LDB, beta, CU, CV, Crk, LDC, rk,
-                      maxrk, acc, flops, "HCORE_dgemm");
+                      maxrk, acc, flops, "HCORE_dgemm", use_scratch ? work : nullptr,
+                      reference_work_bytes(M, maxrk, Ark, Crk, sizeof(double)));
 }
 
 void HCORE_sgemm(HCORE_enum transA, HCORE_enum transB, int M, int N, float alpha, float* AU,
                  float* AV, float* Ark, int LDA, float* BU, float* BV, float* Brk, int LDB,
                  float beta, float* CU, float* CV, float* Crk, int LDC, int rk, int maxrk,
                  float acc, float* work, flop_counter* flops) {
-  (void)transA; (void)transB; (void)N; (void)work;
+  (void)transA; (void)transB; (void)N;
   single_gemm<float>(M, alpha, AU, AV, Ark, LDA, BU, BV, Brk, LDB, beta, CU, CV, Crk, LDC, rk,
-                     maxrk, acc, flops, "HCORE_sgemm");
+                     maxrk, acc, flops, "HCORE_sgemm", use_scratch ? work : nullptr,
+                     reference_work_bytes(M, maxrk, Ark, Crk, sizeof(float)));
 }
 
 void HCORE_dsyrk(HCORE_enum uplo, HCORE_enum trans, int M, int K, double alpha, const double* AU,
@@ -1040,26 +1233,32 @@ void HCORE_dsyrk(HCORE_enum uplo, HCORE_enum trans, int M, int K, double alpha, 
   (void)uplo; (void)trans; (void)LDAV; (void)beta; (void)work;
   t_last_status = HCORE_HIP_OK;
   if (ctx_ensure() != HCORE_HIP_OK) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: GPU init failed");
-  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  ThreadCtx* tc = thread_ctx();
+  if (!tc) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: per-thread GPU state failed");
+  if (M <= 0 || K <= 0 || LDAU < M || LDCD < M)
+    return fail(HCORE_HIP_ERR_ARG, "hcore_amd: dsyrk with an empty tile or leading dimension below M");
   const int N = LDAU;  // rows of AV, as in the reference (hcore_dsyrk.c:57)
   const bool dev = is_device_ptr(AU) && is_device_ptr(AV) && is_device_ptr(CD);
-  Staged st;
-  double* dAU = st.get(AU, LDAU, LDAU, K, dev, true);
-  double* dAV = st.get(AV, LDAU, LDAU, K, dev, true);
-  double* dCD = st.get(CD, LDCD, LDCD, M, dev, true);
-  double* dW = st.get(nullptr, std::max(K, 1), 0, K, false, false);
-  double* dX = st.get(nullptr, M, 0, K, false, false);
-  if (!dAU || !dAV || !dCD || !dW || !dX) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: staging failed");
+  Staged sg;
+  const int iAU = sg.add(AU, LDAU, LDAU, K, true), iAV = sg.add(AV, LDAU, LDAU, K, true);
+  const int iCD = sg.add(CD, LDCD, LDCD, M, true);
+  const int iW = sg.add(nullptr, std::max(K, 1), 0, K, false), iX = sg.add(nullptr, M, 0, K, false);
+  std::vector<double*> d;
+  if (sg.place(tc, dev, d) != HCORE_HIP_OK)
+    return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: staging failed");
   std::vector<GemmProb<double>> gp(3);
-  gp[0] = GemmProb<double>{dAV, dAV, dW, K, K, N, LDAU, LDAU, K, 1.0, 0.0,
+  gp[0] = GemmProb<double>{d[iAV], d[iAV], d[iW], K, K, N, LDAU, LDAU, K, 1.0, 0.0,
                            GF_A_TRANS | GF_BETA_ZERO, nullptr, nullptr};
-  gp[1] = GemmProb<double>{dAU, dW, dX, M, K, K, LDAU, K, M, 1.0, 0.0, GF_BETA_ZERO, nullptr,
-                           nullptr};
-  gp[2] = GemmProb<double>{dX, dAU, dCD, M, M, K, M, LDAU, LDCD, alpha, 1.0,
+  gp[1] = GemmProb<double>{d[iAU], d[iW], d[iX], M, K, K, LDAU, K, M, 1.0, 0.0, GF_BETA_ZERO,
+                           nullptr, nullptr};
+  gp[2] = GemmProb<double>{d[iX], d[iAU], d[iCD], M, M, K, M, LDAU, LDCD, alpha, 1.0,
                            GF_B_TRANS | GF_LOWER_ZERO_UPPER, nullptr, nullptr};
-  if (run_gemm_chain(gp) != HCORE_HIP_OK) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: dsyrk failed");
-  if (!dev && hipMemcpy2D(CD, sizeof(double) * LDCD, dCD, sizeof(double) * LDCD,
-                          sizeof(double) * LDCD, M, hipMemcpyDeviceToHost) != hipSuccess)
+  if (run_gemm_chain(tc, gp) != HCORE_HIP_OK)
+    return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: dsyrk failed");
+  if (!dev && (hipMemcpy2DAsync(CD, sizeof(double) * LDCD, d[iCD], sizeof(double) * LDCD,
+                                sizeof(double) * LDCD, M, hipMemcpyDeviceToHost,
+                                tc->stream) != hipSuccess ||
+               hipStreamSynchronize(tc->stream) != hipSuccess))
     return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: dsyrk copy-back failed");
   if (flops)
     flops->syrk += flop_counts('m', K, K, N, 0) + flop_counts('m', M, K, K, 0) +
@@ -1074,36 +1273,41 @@ void HCORE_dgemmcd(HCORE_enum transA, HCORE_enum transB, int M, double alpha, do
   (void)maxrank; (void)work; (void)flops;
   t_last_status = HCORE_HIP_OK;
   if (ctx_ensure() != HCORE_HIP_OK) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: GPU init failed");
-  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  ThreadCtx* tc = thread_ctx();
+  if (!tc) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: per-thread GPU state failed");
+  if (M <= 0 || Arank <= 0 || Brank <= 0)
+    return fail(HCORE_HIP_ERR_ARG, "hcore_amd: dgemmcd with an empty tile or a rank of 0");
   const bool dev = is_device_ptr(Au) && is_device_ptr(Av) && is_device_ptr(Bu) &&
                    is_device_ptr(Bv) && is_device_ptr(CD);
-  Staged st;
   // every leading dimension is M (hcore_dgemmcd.c:66-100)
-  double* dAu = st.get(Au, M, M, Arank, dev, true);
-  double* dAv = st.get(Av, M, M, Arank, dev, true);
-  double* dBu = st.get(Bu, M, M, Brank, dev, true);
-  double* dBv = st.get(Bv, M, M, Brank, dev, true);
-  double* dCD = st.get(CD, M, M, M, dev, true);
-  double* dW = st.get(nullptr, std::max(Arank, 1), 0, Brank, false, false);
-  double* dX = st.get(nullptr, M, 0, std::max(Arank, Brank), false, false);
-  if (!dAu || !dAv || !dBu || !dBv || !dCD || !dW || !dX)
+  Staged sg;
+  const int iAu = sg.add(Au, M, M, Arank, true), iAv = sg.add(Av, M, M, Arank, true);
+  const int iBu = sg.add(Bu, M, M, Brank, true), iBv = sg.add(Bv, M, M, Brank, true);
+  const int iCD = sg.add(CD, M, M, M, true);
+  const int iW = sg.add(nullptr, std::max(Arank, 1), 0, Brank, false);
+  const int iX = sg.add(nullptr, M, 0, std::max(Arank, Brank), false);
+  std::vector<double*> d;
+  if (sg.place(tc, dev, d) != HCORE_HIP_OK)
     return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: staging failed");
   std::vector<GemmProb<double>> gp(3);
-  gp[0] = GemmProb<double>{dAv, dBv, dW, Arank, Brank, M, M, M, Arank, 1.0, 0.0,
+  gp[0] = GemmProb<double>{d[iAv], d[iBv], d[iW], Arank, Brank, M, M, M, Arank, 1.0, 0.0,
                            GF_A_TRANS | GF_BETA_ZERO, nullptr, nullptr};
   if (Arank > Brank) {  // X = Au W (M x Brank); CD -= X Bu^T
-    gp[1] = GemmProb<double>{dAu, dW, dX, M, Brank, Arank, M, Arank, M, 1.0, 0.0, GF_BETA_ZERO,
+    gp[1] = GemmProb<double>{d[iAu], d[iW], d[iX], M, Brank, Arank, M, Arank, M, 1.0, 0.0,
+                             GF_BETA_ZERO, nullptr, nullptr};
+    gp[2] = GemmProb<double>{d[iX], d[iBu], d[iCD], M, M, Brank, M, M, M, -1.0, 1.0, GF_B_TRANS,
                              nullptr, nullptr};
-    gp[2] = GemmProb<double>{dX, dBu, dCD, M, M, Brank, M, M, M, -1.0, 1.0, GF_B_TRANS, nullptr,
-                             nullptr};
   } else {  // X = W Bu^T (Arank x M); CD -= Au X
-    gp[1] = GemmProb<double>{dW, dBu, dX, Arank, M, Brank, Arank, M, Arank, 1.0, 0.0,
+    gp[1] = GemmProb<double>{d[iW], d[iBu], d[iX], Arank, M, Brank, Arank, M, Arank, 1.0, 0.0,
                              GF_B_TRANS | GF_BETA_ZERO, nullptr, nullptr};
-    gp[2] = GemmProb<double>{dAu, dX, dCD, M, M, Arank, M, Arank, M, -1.0, 1.0, 0, nullptr,
-                             nullptr};
+    gp[2] = GemmProb<double>{d[iAu], d[iX], d[iCD], M, M, Arank, M, Arank, M, -1.0, 1.0, 0,
+                             nullptr, nullptr};
   }
-  if (run_gemm_chain(gp) != HCORE_HIP_OK) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: dgemmcd failed");
-  if (!dev && hipMemcpy(CD, dCD, sizeof(double) * (size_t)M * M, hipMemcpyDeviceToHost) != hipSuccess)
+  if (run_gemm_chain(tc, gp) != HCORE_HIP_OK)
+    return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: dgemmcd failed");
+  if (!dev && (hipMemcpyAsync(CD, d[iCD], sizeof(double) * (size_t)M * M, hipMemcpyDeviceToHost,
+                              tc->stream) != hipSuccess ||
+               hipStreamSynchronize(tc->stream) != hipSuccess))
     return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: dgemmcd copy-back failed");
 }
 
@@ -1116,12 +1320,8 @@ int HCORE_hip_init(int device) {
 int HCORE_hip_finalize(void) {
   std::lock_guard<std::mutex> lk(g_ctx.mu);
   if (!g_ctx.ready) return HCORE_HIP_OK;
-  if (g_ctx.stream) (void)hipStreamSynchronize(g_ctx.stream);
-  if (g_ctx.arena) (void)hipFree(g_ctx.arena);
-  g_ctx.arena = nullptr;
-  g_ctx.arena_cap = 0;
-  if (g_ctx.stream) (void)hipStreamDestroy(g_ctx.stream);
-  g_ctx.stream = nullptr;
+  release_thread_ctxs();
+  ++g_ctx.generation;
   g_ctx.ready = false;
   return HCORE_HIP_OK;
 }
@@ -1156,7 +1356,16 @@ int HCORE_hip_plan_add_dgemm(hcore_hip_plan* p, int n, void* const* AU, void* co
                              int LDCout, double alpha, double beta, int rk, int maxrk,
                              double acc) {
   if (!p || p->finalized || p->ng > 0 || n < 0) return HCORE_HIP_ERR_ARG;
+  // outputs: both in place (leading dimension LDC) or both out of place
+  // (leading dimension LDCout), every entry given
+  if ((CUout == nullptr) != (CVout == nullptr)) return HCORE_HIP_ERR_ARG;
+  const int M = p->M;
+  if (LDA < M || LDB < M || LDC < M || (CUout && LDCout < M)) return HCORE_HIP_ERR_ARG;
+  if (n > 0 && (!AU || !AV || !BU || !BV || !CU || !CV || !Ark || !Brk || !Crk))
+    return HCORE_HIP_ERR_ARG;
   for (int u = 0; u < n; ++u) {
+    if (!AU[u] || !AV[u] || !BU[u] || !BV[u] || !CU[u] || !CV[u]) return HCORE_HIP_ERR_ARG;
+    if (CUout && (!CUout[u] || !CVout[u])) return HCORE_HIP_ERR_ARG;
     const int K = Ark[u] + Crk[u];
     if (Ark[u] <= 0 || Brk[u] <= 0 || Crk[u] <= 0) return HCORE_HIP_ERR_ZERO_RANK;
     if (K > 2 * maxrk) return HCORE_HIP_ERR_SUM_RANKS;
@@ -1171,7 +1380,7 @@ int HCORE_hip_plan_add_dgemm(hcore_hip_plan* p, int n, void* const* AU, void* co
   if (CUout) p->CUo.assign(CUout, CUout + n);
   if (CVout) p->CVo.assign(CVout, CVout + n);
   p->ra.assign(Ark, Ark + n); p->rb.assign(Brk, Brk + n); p->rc.assign(Crk, Crk + n);
-  p->lda = LDA; p->ldb = LDB; p->ldc = LDC; p->ldco = (CUout || CVout) ? LDCout : LDC;
+  p->lda = LDA; p->ldb = LDB; p->ldc = LDC; p->ldco = CUout ? LDCout : LDC;
   p->alpha = alpha; p->beta = beta; p->rk = rk; p->maxrk = maxrk; p->acc = acc;
   return HCORE_HIP_OK;
 }
@@ -1179,6 +1388,12 @@ int HCORE_hip_plan_add_dgemm(hcore_hip_plan* p, int n, void* const* AU, void* co
 int HCORE_hip_plan_add_dsyrk(hcore_hip_plan* p, int n, void* const* AU, void* const* AV,
                              const int* K, int LDAU, void* const* CD, int LDCD, double alpha) {
   if (!p || p->finalized || p->ns > 0 || n < 0) return HCORE_HIP_ERR_ARG;
+  if (LDAU < p->M || LDCD < p->M) return HCORE_HIP_ERR_ARG;
+  if (n > 0 && (!AU || !AV || !K || !CD)) return HCORE_HIP_ERR_ARG;
+  for (int u = 0; u < n; ++u) {
+    if (!AU[u] || !AV[u] || !CD[u]) return HCORE_HIP_ERR_ARG;
+    if (K[u] <= 0 || K[u] > LDAU) return HCORE_HIP_ERR_ARG;
+  }
   p->ns = n;
   p->sAU.assign(AU, AU + n); p->sAV.assign(AV, AV + n); p->sCD.assign(CD, CD + n);
   p->sK.assign(K, K + n);
@@ -1188,7 +1403,6 @@ int HCORE_hip_plan_add_dsyrk(hcore_hip_plan* p, int n, void* const* AU, void* co
 
 int HCORE_hip_plan_finalize(hcore_hip_plan* p) {
   if (!p || p->finalized) return HCORE_HIP_ERR_ARG;
-  std::lock_guard<std::mutex> lk(g_ctx.mu);
   return p->prec == HCORE_HIP_FP64 ? plan_finalize_t<double>(p) : plan_finalize_t<float>(p);
 }
 

@@ -9,6 +9,8 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "flop_util_structs.h"
+
 namespace hcore {
 
 // ---- status codes written per update by the device --------------------------
@@ -147,7 +149,7 @@ void launch_gemm(const GemmProb<T>* probs, const GemmTask* tasks, int ntasks, in
                  hipStream_t s, bool split = false);
 // tall-skinny products: A not transposed, m >= kTallMinM, n <= kTallMaxN,
 // k <= kTallMaxK; tasks are (row block of kTallRows, column block of kTallCols)
-constexpr int kTallRows = 512, kTallCols = 32, kTallK = 64;
+constexpr int kTallRows = 256, kTallCols = 32, kTallK = 64;
 constexpr int kTallMinM = 256, kTallMaxN = 128, kTallMaxK = 128;
 template <typename T>
 void launch_gemm_tall(const GemmProb<T>* probs, const GemmTask* tasks, int ntasks, hipStream_t s);
@@ -199,6 +201,19 @@ void launch_convert(const ConvProb* probs, int n, int nmax, bool to_double, hipS
 
 template <typename T>
 void prepare_kernels();
+
+// ---- single-update entry with a caller-provided workspace (hcore_api.cpp) ----
+// One HCORE_dgemm on device pointers whose one-update workspace, descriptors
+// and status words go to the fixed device buffer ws (ws_bytes), on stream s:
+// nothing is allocated on the device.  Fails through the library's error
+// policy when the update does not fit.
+void dgemm_fixed_workspace(int M, double alpha, double* AU, double* AV, double* Ark, int LDA,
+                           double* BU, double* BV, double* Brk, int LDB, double beta, double* CU,
+                           double* CV, double* Crk, int LDC, int rk, int maxrk, double acc,
+                           flop_counter* flops, const char* name, char* ws, size_t ws_bytes,
+                           hipStream_t s);
+// Print msg, record status for HCORE_hip_last_status and apply the error policy.
+void report_failure(int status, const char* msg);
 
 // Size in elements of the compact-WY storage for a K-column stack.
 inline int tp_elems(int K) { return ((K + kPB - 1) / kPB) * kPB * kPB; }

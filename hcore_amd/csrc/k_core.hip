@@ -9,6 +9,7 @@
 
 #include "hcore_internal.hpp"
 #include "hcore_wave.hpp"
+#include "core_complete.hpp"
 
 namespace hcore {
 
@@ -256,7 +257,8 @@ __device__ void r_transpose(T* A, int K, int ld, int k, const T* rdiag) {
 // left singular vectors U = P W / sigma (rows permuted back by piv).
 template <typename T, int NT>
 __device__ void core_epilogue(const CoreProb<T>& P, const T* W, int ld, int k, const int* piv, T* sig,
-                              int* perm, int* flags, int rk, int maxrk, T acc) {
+                              int* perm, int* flags, int rk, int maxrk, T acc, T* nrm, T* rdiag,
+                              T* red, int* gi) {
   const int K = P.K;
   const int tid = threadIdx.x;
   if (tid == 0) flags[1] = K;  // flags[2] (non-finite input) was set at load time
@@ -303,12 +305,20 @@ __device__ void core_epilogue(const CoreProb<T>& P, const T* W, int ld, int k, c
     *P.rank = r;
     *P.status = r > maxrk ? ST_R_GT_MAXRK : ST_OK;
   }
+  if (tid == 0) flags[0] = 0;
+  block_sync<NT>();
   for (int e = tid; e < r * K; e += NT) {
     const int i = e / K, row = e - i * K;
     const int j = perm[i];
     const T sj = sig[j];
-    P.Zu[piv[row] + (size_t)i * K] = (sj > T(0) && j < k) ? W[row + (size_t)j * ld] / sj : T(0);
+    const bool ok = sj > T(0) && j < k;
+    P.Zu[piv[row] + (size_t)i * K] = ok ? W[row + (size_t)j * ld] / sj : T(0);
+    if (!ok) flags[0] = 1;
   }
+  block_sync<NT>();
+  // zero singular values: complete U to an orthonormal set (scratch: the
+  // pivoted-QR norms and diagonal, free by now)
+  if (flags[0]) complete_basis<T, NT>(P.Zu, K, r, nrm, rdiag, red, gi);
 }
 
 __host__ __device__ constexpr size_t core_head_bytes(size_t elem, int kmax) {
@@ -376,7 +386,7 @@ __global__ __launch_bounds__(NT) void k_core(const CoreProb<T>* __restrict__ pro
     }
   }
   if (tid == 0 && P.sweeps) *P.sweeps = sweep + 1;
-  core_epilogue<T, NT>(P, A, ld, k, piv, sig, perm, flags, rk, maxrk, acc);
+  core_epilogue<T, NT>(P, A, ld, k, piv, sig, perm, flags, rk, maxrk, acc, nrm, rdiag, red, gi);
 }
 
 int core_lds_max_k(size_t elem) {

@@ -10,6 +10,7 @@
 
 #include "hcore_internal.hpp"
 #include "hcore_wave.hpp"
+#include "core_complete.hpp"
 
 namespace hcore {
 // ------------------------------------------------------------------------------
@@ -625,6 +626,8 @@ __global__ __launch_bounds__(64 * W) void k_core_reg(const CoreProb<T>* __restri
     *P.rank = r;
     *P.status = r > maxrk ? ST_R_GT_MAXRK : ST_OK;
   }
+  if (tid == 0) flags[2] = 0;
+  __syncthreads();
 #pragma unroll
   for (int q = 0; q < 2 * B; ++q) {
     const int c = q < B ? cA + q : cB + q - B;
@@ -632,7 +635,9 @@ __global__ __launch_bounds__(64 * W) void k_core_reg(const CoreProb<T>* __restri
       const int i = rnk[c];
       if (i < r) {
         const T s = sig[c];
-        const T inv = (s > T(0) && c < k) ? T(1) / s : T(0);
+        const bool ok = s > T(0) && c < k;
+        const T inv = ok ? T(1) / s : T(0);
+        if (!ok) flags[2] = 1;
 #pragma unroll
         for (int ii = 0; ii < RPL; ++ii) {
           const int row = lane + 64 * ii;
@@ -644,6 +649,10 @@ __global__ __launch_bounds__(64 * W) void k_core_reg(const CoreProb<T>* __restri
       }
     }
   }
+  __syncthreads();
+  // zero singular values: complete U to an orthonormal set (scratch: the
+  // block-exchange area and the reflector buffer, free by now)
+  if (flags[2]) complete_basis<T, NT>(P.Zu, K, r, big, vbuf, big + KM, cand_i);
   if (tid == 0 && P.clk) P.clk[3] = (long long)wall_clock64();
 }
 

@@ -13,9 +13,38 @@
 namespace hcore {
 
 // ------------------------------------------------------------------------------
-// batched GEMM on 64x64 tiles, 4x4 outputs per thread, k staged through LDS
+// batched GEMM on the matrix cores.  Every product of the path (the k = nb
+// reductions F = AV^T BV and Y^T Y, the small cores T = R_A R_B^T and T^T Zu,
+// the tall products G = BU F^T and X = [Z; 0] - Y M, and dsyrk's W, X and CD)
+// runs through v_mfma_f64_16x16x4_f64 (FP32 plans: v_mfma_f32_16x16x4_f32):
+// one wavefront owns a 32 x 32 (or 16-row-block x 32) piece of the output as
+// 16 x 16 accumulator blocks; operands are staged through LDS (2-D tiles) or
+// read straight from HBM with coalesced 16-row segments (tall products).
 // ------------------------------------------------------------------------------
-constexpr int GK = 32;  // k chunk
+constexpr int GK = 32;  // k chunk staged in LDS
+
+// One 16 x 16 x 4 matrix-core step, D = A B + C, per precision.  Operand
+// layout (lane l): A(m = l % 16, k = l / 16), B(k = l / 16, n = l % 16);
+// result D(m = 4 (l / 16) + v, n = l % 16) in element v of the accumulator.
+// The kernels below feed op(B) as A and op(A) as B, so that lane l holds
+// C(i = l % 16, j = 4 (l / 16) + v): 16 consecutive rows of 4 columns, a
+// coalesced column-major store.
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  using V = double __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ V mac(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma<float> {
+  using V = float __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ V mac(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
 
 template <typename T>
 __device__ __forceinline__ T gemm_masked(const T* X, int ld, int p, int q, bool upper, bool unit_lower) {
@@ -62,45 +91,85 @@ __device__ __forceinline__ void gemm_fetch(const GemmProb<T>& P, int n, int row0
   }
 }
 
-// C = alpha op(A) op(B) + beta C (or beta C0) for the 4 x 4 outputs of one
-// thread of a TD x TD thread tile.
-template <typename T, int TD>
-__device__ __forceinline__ void gemm_store(const GemmProb<T>& P, const T (&acc)[4][4], int row0,
-                                           int col0, int tx, int ty, int n) {
-  const bool lzu = (P.flags & GF_LOWER_ZERO_UPPER) != 0;
-  const bool bz = (P.flags & GF_BETA_ZERO) != 0;
-  const bool c0 = (P.flags & GF_C0) != 0;
+// Registers -> LDS: As[kk][i] = op(A)(i, k0 + kk), Bs[kk][j] = op(B)(k0 + kk, j).
+template <typename T, int TILE, int NT, int LD, int GKX, int LDT>
+__device__ __forceinline__ void gemm_stage(T (*As)[LDT], T (*Bs)[LDT], const T (&ra)[LD],
+                                           const T (&rb)[LD], bool ta, bool tb, int t) {
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int i = row0 + tx + TD * a, j = col0 + ty + TD * b;
-      if (i < P.m && j < n) {
-        T* c = P.C + i + (size_t)j * P.ldc;
-        if (lzu && i < j) { *c = T(0); continue; }
-        T v = P.alpha * acc[a][b];
-        if (c0) {
-          if (i < P.m0) v = fma(P.beta, P.C0[i + (size_t)j * P.ldc0], v);
-        } else if (!bz) {
-          v = fma(P.beta, *c, v);
-        }
-        *c = v;
-      }
-    }
+  for (int q = 0; q < LD; ++q) {
+    const int idx = t + NT * q;
+    if (!ta) As[idx / TILE][idx % TILE] = ra[q];
+    else     As[idx % GKX][idx / GKX] = ra[q];
+    if (tb) Bs[idx / TILE][idx % TILE] = rb[q];
+    else    Bs[idx % GKX][idx / GKX] = rb[q];
+  }
 }
 
-// Batched GEMM on TILE x TILE output tiles (64: 256 threads; 32: one
-// wavefront), 4 x 4 outputs per thread, k staged through LDS in chunks of GK
-// with the next chunk's global loads in flight during the current chunk.
+// C(i, j) = alpha * acc + beta * C (or beta * C0 rows < m0, 0 below), with the
+// strict upper triangle zeroed under GF_LOWER_ZERO_UPPER.
+template <typename T>
+__device__ __forceinline__ void gemm_store1(const GemmProb<T>& P, int i, int j, int n, T acc) {
+  if (i >= P.m || j >= n) return;
+  T* c = P.C + i + (size_t)j * P.ldc;
+  if ((P.flags & GF_LOWER_ZERO_UPPER) && i < j) {
+    *c = T(0);
+    return;
+  }
+  T v = P.alpha * acc;
+  if (P.flags & GF_C0) {
+    if (i < P.m0) v = fma(P.beta, P.C0[i + (size_t)j * P.ldc0], v);
+  } else if (!(P.flags & GF_BETA_ZERO)) {
+    v = fma(P.beta, *c, v);
+  }
+  *c = v;
+}
+
+// The 2 x 2 accumulator blocks of one wavefront's 32 x 32 piece at (r0, c0).
+template <typename T>
+__device__ __forceinline__ void gemm_store_wave(const GemmProb<T>& P, const typename Mfma<T>::V (&acc)[2][2],
+                                                int r0, int c0, int n, int lane) {
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        gemm_store1(P, r0 + 16 * rb + (lane & 15), c0 + 16 * cb + 4 * (lane >> 4) + v, n,
+                    acc[rb][cb][v]);
+}
+
+// GKX/4 matrix-core steps over the staged chunk for the wavefront's 32 x 32
+// piece (rows r0w.., columns c0w.. of the tile).
+template <typename T, int GKX, int LDT>
+__device__ __forceinline__ void gemm_mfma_chunk(const T (*As)[LDT], const T (*Bs)[LDT],
+                                                typename Mfma<T>::V (&acc)[2][2], int r0w, int c0w,
+                                                int lane) {
+  const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int kk = 0; kk < GKX; kk += 4) {
+    const T a0 = As[kk + lk][r0w + li], a1 = As[kk + lk][r0w + 16 + li];
+    const T b0 = Bs[kk + lk][c0w + li], b1 = Bs[kk + lk][c0w + 16 + li];
+    acc[0][0] = Mfma<T>::mac(b0, a0, acc[0][0]);
+    acc[0][1] = Mfma<T>::mac(b1, a0, acc[0][1]);
+    acc[1][0] = Mfma<T>::mac(b0, a1, acc[1][0]);
+    acc[1][1] = Mfma<T>::mac(b1, a1, acc[1][1]);
+  }
+}
+
+// Batched GEMM on TILE x TILE output tiles (64: four wavefronts of 32 x 32;
+// 32: one wavefront), k staged through LDS in chunks (GK, or GK/2 for the
+// one-wavefront tiles) with the next chunk's global loads in flight during the
+// current chunk's matrix-core steps.
 template <typename T, int TILE>
 __global__ __launch_bounds__((TILE / 4) * (TILE / 4)) void k_gemm(
     const GemmProb<T>* __restrict__ probs, const GemmTask* __restrict__ tasks) {
-  constexpr int TD = TILE / 4;       // threads per tile edge
-  constexpr int NT = TD * TD;        // threads
-  constexpr int LD = TILE * GK / NT; // elements of each operand loaded per thread per chunk
+  using V = typename Mfma<T>::V;
+  constexpr int NT = (TILE / 4) * (TILE / 4);  // 256 or 64 threads
+  constexpr int GKT = TILE == 64 ? GK : GK / 2;  // k chunk: 8 elements per thread per operand
+  constexpr int LD = TILE * GKT / NT;            // elements of each operand per thread per chunk
   // +1 column of padding: the transposed fills write consecutive k at one i
-  __shared__ T As[GK][TILE + 1];
-  __shared__ T Bs[GK][TILE + 1];
+  __shared__ T As[GKT][TILE + 1];
+  __shared__ T Bs[GKT][TILE + 1];
   const GemmTask tk = tasks[blockIdx.x];
   const GemmProb<T> P = probs[tk.prob];
   if (P.skip && *P.skip) return;
@@ -108,8 +177,7 @@ __global__ __launch_bounds__((TILE / 4) * (TILE / 4)) void k_gemm(
   if (P.n_dev) n = min(n, *P.n_dev);
   const int row0 = tk.ti * TILE, col0 = tk.tj * TILE;
   if (row0 >= P.m || col0 >= n) return;
-  const int t = threadIdx.x;
-  const int tx = t % TD, ty = t / TD;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const bool lzu = (P.flags & GF_LOWER_ZERO_UPPER) != 0;
   if (lzu && row0 + TILE - 1 < col0) {  // tile strictly above the diagonal: zero fill
     for (int e = t; e < TILE * TILE; e += NT) {
@@ -120,54 +188,38 @@ __global__ __launch_bounds__((TILE / 4) * (TILE / 4)) void k_gemm(
   }
   const bool ta = P.flags & GF_A_TRANS, tb = P.flags & GF_B_TRANS;
   const bool ua = P.flags & GF_A_UPPER, ub = P.flags & GF_B_UPPER;
-  T acc[4][4];
+  const int r0w = TILE == 64 ? 32 * (w & 1) : 0, c0w = TILE == 64 ? 32 * (w >> 1) : 0;
+  V acc[2][2];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
+    for (int b = 0; b < 2; ++b) acc[a][b] = V{T(0), T(0), T(0), T(0)};
 
   T ra[LD], rb[LD];
-  gemm_fetch<T, TILE, NT, LD>(P, n, row0, col0, 0, ta, tb, ua, ub, ra, rb, t);
-  for (int k0 = 0; k0 < P.k; k0 += GK) {
-#pragma unroll
-    for (int q = 0; q < LD; ++q) {
-      const int idx = t + NT * q;
-      if (!ta) As[idx / TILE][idx % TILE] = ra[q];
-      else     As[idx % GK][idx / GK] = ra[q];
-      if (tb) Bs[idx / TILE][idx % TILE] = rb[q];
-      else    Bs[idx % GK][idx / GK] = rb[q];
-    }
+  gemm_fetch<T, TILE, NT, LD, GKT>(P, n, row0, col0, 0, ta, tb, ua, ub, ra, rb, t);
+  for (int k0 = 0; k0 < P.k; k0 += GKT) {
+    gemm_stage<T, TILE, NT, LD, GKT, TILE + 1>(As, Bs, ra, rb, ta, tb, t);
     __syncthreads();
-    if (k0 + GK < P.k) gemm_fetch<T, TILE, NT, LD>(P, n, row0, col0, k0 + GK, ta, tb, ua, ub, ra, rb, t);
-#pragma unroll 8
-    for (int kk = 0; kk < GK; ++kk) {
-      T av[4], bv[4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) av[a] = As[kk][tx + TD * a];
-#pragma unroll
-      for (int b = 0; b < 4; ++b) bv[b] = Bs[kk][ty + TD * b];
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = fma(av[a], bv[b], acc[a][b]);
-    }
+    if (k0 + GKT < P.k)
+      gemm_fetch<T, TILE, NT, LD, GKT>(P, n, row0, col0, k0 + GKT, ta, tb, ua, ub, ra, rb, t);
+    gemm_mfma_chunk<T, GKT, TILE + 1>(As, Bs, acc, r0w, c0w, lane);
     __syncthreads();
   }
-  gemm_store<T, TD>(P, acc, row0, col0, tx, ty, n);
+  gemm_store_wave<T>(P, acc, row0 + r0w, col0 + c0w, n, lane);
 }
 
-// Split-k variant for long reductions (k = nb) into small outputs: KS thread
-// groups of TD x TD threads share one output tile, group g taking the k
-// chunks g, g + KS, ... (GKS wide, staged through the group's own LDS
-// buffers); the groups' partial tiles are then summed in LDS in a fixed
-// pairwise order, so the result does not depend on timing.  Every group runs
-// the same number of chunk steps (chunks past k load zeros), so the block
-// barriers line up.
+// Split-k variant for long reductions (k = nb) into small outputs: KS
+// wavefronts share one 32 x 32 output tile, wavefront g taking the k chunks
+// g, g + KS, ... (GKS wide, staged through its own LDS buffers); the partial
+// tiles are then summed in LDS in a fixed pairwise order, so the result does
+// not depend on timing.  Every wavefront runs the same number of chunk steps
+// (chunks past k load zeros), so the block barriers line up.
 template <typename T, int TILE, int KS>
-__global__ __launch_bounds__((TILE / 4) * (TILE / 4) * KS) void k_gemm_sk(
-    const GemmProb<T>* __restrict__ probs, const GemmTask* __restrict__ tasks) {
-  constexpr int TD = TILE / 4;
-  constexpr int NG = TD * TD;          // threads per group
+__global__ __launch_bounds__(64 * KS) void k_gemm_sk(const GemmProb<T>* __restrict__ probs,
+                                                     const GemmTask* __restrict__ tasks) {
+  using V = typename Mfma<T>::V;
+  static_assert(TILE == 32, "split-k tiles are one wavefront's 32 x 32 piece");
+  constexpr int NG = 64;               // threads per group (one wavefront)
   constexpr int NT = NG * KS;
   constexpr int GKS = 16;              // k chunk of one group
   constexpr int LD = TILE * GKS / NG;  // elements of each operand per thread per chunk
@@ -183,8 +235,7 @@ __global__ __launch_bounds__((TILE / 4) * (TILE / 4) * KS) void k_gemm_sk(
   if (P.n_dev) n = min(n, *P.n_dev);
   const int row0 = tk.ti * TILE, col0 = tk.tj * TILE;
   if (row0 >= P.m || col0 >= n) return;
-  const int grp = threadIdx.x / NG, t = threadIdx.x % NG;
-  const int tx = t % TD, ty = t / TD;
+  const int grp = threadIdx.x / NG, t = threadIdx.x % NG, lane = t;
   const bool lzu = (P.flags & GF_LOWER_ZERO_UPPER) != 0;
   if (lzu && row0 + TILE - 1 < col0) {
     for (int e = threadIdx.x; e < TILE * TILE; e += NT) {
@@ -197,78 +248,71 @@ __global__ __launch_bounds__((TILE / 4) * (TILE / 4) * KS) void k_gemm_sk(
   const bool ua = P.flags & GF_A_UPPER, ub = P.flags & GF_B_UPPER;
   T (*As)[TILE + 1] = reinterpret_cast<T (*)[TILE + 1]>(sm + grp * STG);
   T (*Bs)[TILE + 1] = reinterpret_cast<T (*)[TILE + 1]>(sm + grp * STG + GKS * (TILE + 1));
-  T acc[4][4];
+  V acc[2][2];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
+    for (int b = 0; b < 2; ++b) acc[a][b] = V{T(0), T(0), T(0), T(0)};
   const int nsteps = (P.k + GKS * KS - 1) / (GKS * KS);
   T ra[LD], rb[LD];
   gemm_fetch<T, TILE, NG, LD, GKS>(P, n, row0, col0, grp * GKS, ta, tb, ua, ub, ra, rb, t);
   for (int st = 0; st < nsteps; ++st) {
-#pragma unroll
-    for (int q = 0; q < LD; ++q) {
-      const int idx = t + NG * q;
-      if (!ta) As[idx / TILE][idx % TILE] = ra[q];
-      else     As[idx % GKS][idx / GKS] = ra[q];
-      if (tb) Bs[idx / TILE][idx % TILE] = rb[q];
-      else    Bs[idx % GKS][idx / GKS] = rb[q];
-    }
+    gemm_stage<T, TILE, NG, LD, GKS, TILE + 1>(As, Bs, ra, rb, ta, tb, t);
     __syncthreads();
     if (st + 1 < nsteps)
       gemm_fetch<T, TILE, NG, LD, GKS>(P, n, row0, col0, ((st + 1) * KS + grp) * GKS, ta, tb, ua,
                                        ub, ra, rb, t);
-#pragma unroll 8
-    for (int kk = 0; kk < GKS; ++kk) {
-      T av[4], bv[4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) av[a] = As[kk][tx + TD * a];
-#pragma unroll
-      for (int b = 0; b < 4; ++b) bv[b] = Bs[kk][ty + TD * b];
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = fma(av[a], bv[b], acc[a][b]);
-    }
+    gemm_mfma_chunk<T, GKS, TILE + 1>(As, Bs, acc, 0, 0, lane);
     __syncthreads();
   }
-  // pairwise sum of the groups' tiles: at level h, group g (g % 2h == h) hands
+  // pairwise sum of the groups' tiles: at level h, group g (h <= g < 2h) hands
   // its tile to group g - h
+  const int li = lane & 15, lk = lane >> 4;
 #pragma unroll
   for (int h = KS / 2; h >= 1; h /= 2) {
     if (grp >= h && grp < 2 * h) {
       T* buf = sm + (size_t)(grp - h) * TILE * TILE;
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) buf[(tx + TD * a) + (ty + TD * b) * TILE] = acc[a][b];
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            buf[(16 * a + li) + (16 * b + 4 * lk + v) * TILE] = acc[a][b][v];
     }
     __syncthreads();
     if (grp < h) {
       const T* buf = sm + (size_t)grp * TILE * TILE;
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] += buf[(tx + TD * a) + (ty + TD * b) * TILE];
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[a][b][v] += buf[(16 * a + li) + (16 * b + 4 * lk + v) * TILE];
     }
     __syncthreads();
   }
-  if (grp == 0) gemm_store<T, TD>(P, acc, row0, col0, tx, ty, n);
+  if (grp == 0) gemm_store_wave<T>(P, acc, row0, col0, n, lane);
 }
 
 // Tall-skinny batched GEMM (m = the nb rows of a tile, n and k at most a few
 // dozen): C = alpha A op(B) + beta C (or beta C0), A not transposed.  One
-// workgroup covers kTallRows rows (two per thread) and kTallCols columns; the
-// k x kTallCols block of op(B) is staged in LDS in chunks of kTallK rows and
-// read as a broadcast, A is read straight from global memory, coalesced along
-// the rows.  The many tiny output tiles of such products as 2-D tiles would
-// be bound by workgroup dispatch, not by their bytes.
-constexpr int kTallQ = 8;  // k steps whose A loads are issued together
-static_assert(kTallK % kTallQ == 0, "k chunk of the tall GEMM");
+// workgroup covers kTallRows rows and kTallCols columns: wavefront w a quarter
+// of the rows as 16-row blocks by two 16-column blocks of accumulators.  The
+// k x kTallCols block of op(B) is staged in LDS in chunks of kTallK rows; A
+// goes from HBM straight into the matrix-core operand (lane l: row l % 16 of
+// the block, column k + l / 16: four coalesced 128-byte segments per load),
+// the next k step's loads in flight during the current step's matrix-core
+// work.  The many tiny output tiles of such
+// products as 2-D tiles would be bound by workgroup dispatch, not by bytes.
 template <typename T>
 __global__ __launch_bounds__(256) void k_gemm_tall(const GemmProb<T>* __restrict__ probs,
                                                    const GemmTask* __restrict__ tasks) {
-  __shared__ T Bs[kTallK][kTallCols];
+  using V = typename Mfma<T>::V;
+  constexpr int RB = kTallRows / 4 / 16;  // 16-row blocks per wavefront
+  constexpr int CBK = kTallCols / 16;     // 16-column blocks
+  static_assert(RB * 64 == kTallRows && CBK * 16 == kTallCols && kTallK % 4 == 0, "tall shape");
+  __shared__ T Bs[kTallK][kTallCols + 1];
   const GemmTask tk = tasks[blockIdx.x];
   const GemmProb<T> P = probs[tk.prob];
   if (P.skip && *P.skip) return;
@@ -277,14 +321,24 @@ __global__ __launch_bounds__(256) void k_gemm_tall(const GemmProb<T>* __restrict
   const int row0 = tk.ti * kTallRows, col0 = tk.tj * kTallCols;
   if (row0 >= P.m || col0 >= n) return;
   const int nc = min(kTallCols, n - col0);
-  const int t = threadIdx.x;
-  const int r0 = row0 + t, r1 = row0 + t + 256;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int rw = row0 + w * (kTallRows / 4);  // first row of this wavefront
   const bool tb = (P.flags & GF_B_TRANS) != 0;
   const bool ua = (P.flags & GF_A_UPPER) != 0, ub = (P.flags & GF_B_UPPER) != 0;
   const bool la = (P.flags & GF_A_UNIT_LOWER) != 0, lb = (P.flags & GF_B_UNIT_LOWER) != 0;
-  T acc0[kTallCols], acc1[kTallCols];
+  V acc[RB][CBK];
 #pragma unroll
-  for (int j = 0; j < kTallCols; ++j) acc0[j] = acc1[j] = T(0);
+  for (int a = 0; a < RB; ++a)
+#pragma unroll
+    for (int b = 0; b < CBK; ++b) acc[a][b] = V{T(0), T(0), T(0), T(0)};
+  auto load_a = [&](int gk, T (&av)[RB]) {
+#pragma unroll
+    for (int a = 0; a < RB; ++a) {
+      const int row = rw + 16 * a + li;
+      av[a] = (gk < P.k && row < P.m) ? gemm_masked(P.A, P.lda, row, gk, ua, la) : T(0);
+    }
+  };
   for (int k0 = 0; k0 < P.k; k0 += kTallK) {
     const int kc = min(kTallK, P.k - k0);
     for (int e = t; e < kTallK * kTallCols; e += 256) {
@@ -299,49 +353,29 @@ __global__ __launch_bounds__(256) void k_gemm_tall(const GemmProb<T>* __restrict
       Bs[kk][j] = v;
     }
     __syncthreads();
-    for (int kq = 0; kq < kc; kq += kTallQ) {
-      // kTallQ columns of A in flight at once
-      T a0[kTallQ], a1[kTallQ];
+    T av[RB], an[RB];
+    load_a(k0 + lk, av);
+    for (int kq = 0; kq < kc; kq += 4) {
+      if (kq + 4 < kc) load_a(k0 + kq + 4 + lk, an);  // next step in flight
+      T bv[CBK];
 #pragma unroll
-      for (int q = 0; q < kTallQ; ++q) {
-        const int gk = k0 + kq + q;
-        const bool in = kq + q < kc;
-        a0[q] = in && r0 < P.m ? gemm_masked(P.A, P.lda, r0, gk, ua, la) : T(0);
-        a1[q] = in && r1 < P.m ? gemm_masked(P.A, P.lda, r1, gk, ua, la) : T(0);
-      }
+      for (int b = 0; b < CBK; ++b) bv[b] = Bs[kq + lk][16 * b + li];  // zero past kc
 #pragma unroll
-      for (int q = 0; q < kTallQ; ++q) {
+      for (int a = 0; a < RB; ++a)
 #pragma unroll
-        for (int j = 0; j < kTallCols; ++j) {
-          const T b = Bs[kq + q][j];  // zero past kc
-          acc0[j] = fma(a0[q], b, acc0[j]);
-          acc1[j] = fma(a1[q], b, acc1[j]);
-        }
-      }
+        for (int b = 0; b < CBK; ++b) acc[a][b] = Mfma<T>::mac(bv[b], av[a], acc[a][b]);
+#pragma unroll
+      for (int a = 0; a < RB; ++a) av[a] = an[a];
     }
     __syncthreads();
   }
-  const bool bz = (P.flags & GF_BETA_ZERO) != 0;
-  const bool c0 = (P.flags & GF_C0) != 0;
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int i = h ? r1 : r0;
-    if (i >= P.m) continue;
+  for (int a = 0; a < RB; ++a)
 #pragma unroll
-    for (int j = 0; j < kTallCols; ++j) {
-      if (j < nc) {
-        const int gj = col0 + j;
-        T* c = P.C + i + (size_t)gj * P.ldc;
-        T v = P.alpha * (h ? acc1[j] : acc0[j]);
-        if (c0) {
-          if (i < P.m0) v = fma(P.beta, P.C0[i + (size_t)gj * P.ldc0], v);
-        } else if (!bz) {
-          v = fma(P.beta, *c, v);
-        }
-        *c = v;
-      }
-    }
-  }
+    for (int b = 0; b < CBK; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        gemm_store1(P, rw + 16 * a + li, col0 + 16 * b + 4 * lk + v, n, acc[a][b][v]);
 }
 
 template <typename T>
@@ -354,9 +388,9 @@ template <typename T>
 void launch_gemm(const GemmProb<T>* probs, const GemmTask* tasks, int ntasks, int tile,
                  hipStream_t s, bool split) {
   if (ntasks <= 0) return;
-  // Only the 32-tiles are split (four groups of one wavefront): a split 64-tile
-  // workgroup needs a whole CU's wave slots and LDS at once and delays the
-  // other lanes' kernels more than it gains.
+  // Only the 32-tiles are split (four wavefronts): a split 64-tile workgroup
+  // needs a whole CU's wave slots and LDS at once and delays the other lanes'
+  // kernels more than it gains.
   if (split && tile == 32) {
     hipLaunchKernelGGL((k_gemm_sk<T, 32, 4>), dim3(ntasks), dim3(256), 0, s, probs, tasks);
     return;

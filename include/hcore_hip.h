@@ -56,9 +56,12 @@ int HCORE_hip_plan_create(hcore_hip_plan **plan, int M, int precision);
  * Record n independent HCORE_dgemm updates C_u <- beta*C_u + alpha*A_u*B_u^T.
  * AU..CV: host arrays of n device pointers (FP64, or FP32 for HCORE_HIP_FP32
  * and HCORE_HIP_MIXED)
- * to column-major factors with leading dimensions LDA, LDB, LDC.  CUout/CVout
- * (may be NULL: in place) receive the new factors with leading dimension
- * LDCout.  Ark/Brk/Crk: host int ranks.  rk, maxrk, acc as in HCORE_dgemm.
+ * to column-major factors with leading dimensions LDA, LDB, LDC (each >= M).
+ * CUout/CVout: both NULL (the new factors overwrite CU/CV, leading dimension
+ * LDC) or both host arrays of n device pointers that receive the new factors
+ * with leading dimension LDCout (>= M).  Ark/Brk/Crk: host int ranks.  rk,
+ * maxrk, acc as in HCORE_dgemm.  HCORE_HIP_ERR_ARG for a NULL pointer, a
+ * leading dimension below M or only one of CUout/CVout.
  * May be called once per plan.
  */
 int HCORE_hip_plan_add_dgemm(hcore_hip_plan *plan, int n,
@@ -71,7 +74,8 @@ int HCORE_hip_plan_add_dgemm(hcore_hip_plan *plan, int n,
 /*
  * Record n HCORE_dsyrk diagonal updates CD_u <- alpha*AU_u (AV_u^T AV_u) AU_u^T
  * + CD_u, strict upper triangle of CD_u set to 0.  Host arrays of device
- * pointers; K: host int ranks.  May be called once per plan.
+ * pointers; K: host int ranks, 1 <= K_u <= LDAU; LDAU and LDCD >= M
+ * (HCORE_HIP_ERR_ARG otherwise).  May be called once per plan.
  */
 int HCORE_hip_plan_add_dsyrk(hcore_hip_plan *plan, int n,
                              void *const *AU, void *const *AV, const int *K, int LDAU,

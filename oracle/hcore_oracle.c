@@ -25,6 +25,11 @@
  *                     testing/testing_sgemm.c:119 (the reference tests' inputs)
  *   orc_dgemm_batch / orc_dsyrk_batch  the CPU baseline: one update per
  *                     OpenMP thread, single-threaded BLAS per call.
+ *   orc_dpotrf        src/compute/hcore_dpotrf.c:23-34 (uplo as the intended
+ *                     triangle: HCORE 121/'U' upper, 122/'L' lower)
+ *   orc_dtrsm         src/compute/hcore_dtrsm.c:24-48
+ *   orc_dgemmbdcd     src/compute/hcore_dgemmbdcd.c:35-122
+ *   orc_duncompress   src/compute/hcore_duncompress.c:33-84
  *
  * Parity pins (tests/test_oracle.py): the reference's own assertions
  * (testing_dgemm.c:377-378, testing_sgemm.c:378-379, testing_dsyrk.c:181,
@@ -79,6 +84,12 @@ extern void scipy_dlatms_(const lint *m, const lint *n, const char *dist, lint *
                           const double *dmax, const lint *kl, const lint *ku,
                           const char *pack, double *a, const lint *lda, double *work,
                           lint *info, size_t, size_t, size_t);
+extern void scipy_dpotrf_(const char *uplo, const lint *n, double *a, const lint *lda,
+                          lint *info, size_t);
+extern void scipy_dtrsm_(const char *side, const char *uplo, const char *transa,
+                         const char *diag, const lint *m, const lint *n, const double *alpha,
+                         const double *a, const lint *lda, double *b, const lint *ldb, size_t,
+                         size_t, size_t, size_t);
 extern void scipy_openblas_set_num_threads(int);
 extern char *scipy_openblas_get_config(void);
 
@@ -435,4 +446,69 @@ int orc_max_threads(void)
 #else
     return 1;
 #endif
+}
+
+/* ---- dense neighbours of the recompression: potrf, trsm, the dense-B
+ * update and decompression.  HCORE_enum values equal the CBLAS ones
+ * (hcore.h:30-44): 111/112/113 trans, 121/122 uplo, 131/132 diag,
+ * 141/142 side. */
+static char orc_uplo_char(int uplo) {
+    /* the reference hands its enum to LAPACKE as the uplo character
+     * (hcore_dpotrf.c:29-32), which LAPACK rejects; the restatement follows
+     * the intended triangle */
+    return (uplo == 121 || uplo == 'U' || uplo == 'u') ? 'U' : 'L';
+}
+
+/* hcore_dpotrf.c:23-34: Cholesky of one dense tile; flops->potrf += 'c'. */
+int orc_dpotrf(int uplo, int N, double *A, int LDA, unsigned long *flops)
+{
+    char u = orc_uplo_char(uplo);
+    lint n = N, lda = LDA, info = 0;
+    scipy_dpotrf_(&u, &n, A, &lda, &info, 1);
+    if (flops) *flops += orc_flop_counts('c', N, 0, 0, 0);
+    return (int)info;
+}
+
+/* hcore_dtrsm.c:24-48: op(A) X = alpha B (left) or X op(A) = alpha B
+ * (right), B is M x Brank; flops->trsm += 't' with side 1 (left) or 2. */
+int orc_dtrsm(int side, int uplo, int transA, int diag, int M, int Brank, double alpha,
+              const double *A, int LDA, double *B, int LDB, unsigned long *flops)
+{
+    char sd = side == 141 ? 'L' : 'R';
+    char up = uplo == 121 ? 'U' : 'L';
+    char tr = transA == 111 ? 'N' : 'T';
+    char dg = diag == 132 ? 'U' : 'N';
+    lint m = M, n = Brank, lda = LDA, ldb = LDB;
+    if (side != 141 && side != 142) return -1;
+    scipy_dtrsm_(&sd, &up, &tr, &dg, &m, &n, &alpha, A, &lda, B, &ldb, 1, 1, 1, 1);
+    if (flops) *flops += orc_flop_counts('t', M, Brank, side == 141 ? 1 : 2, 0);
+    return 0;
+}
+
+/* hcore_dgemmbdcd.c:35-122: AVBD = AV^T BD (Ark x N), then
+ * CD = alpha AU AVBD + beta CD; with transA == HCORE_Trans the inner size
+ * of the first product is M and the rows of the second LDA, otherwise LDB
+ * and M (the reference's own two branches).  flops->update += both 'm'. */
+void orc_dgemmbdcd(int transA, int M, int N, double alpha, const double *AU, const double *AV,
+                   int Ark, int LDA, const double *BD, int LDB, double beta, double *CD,
+                   int LDC, unsigned long *flops)
+{
+    double *AVBD = malloc(sizeof(double) * (size_t)(Ark > 0 ? Ark : 1) * (N > 0 ? N : 1));
+    const int k1 = transA == 112 ? M : LDB;
+    const int m2 = transA == 112 ? LDA : M;
+    orc_dgemm_gemm('T', 'N', Ark, N, k1, 1.0, AV, LDA, BD, LDB, 0.0, AVBD, Ark);
+    orc_dgemm_gemm('N', 'N', m2, N, Ark, alpha, AU, LDA, AVBD, Ark, beta, CD, LDC);
+    if (flops)
+        *flops += orc_flop_counts('m', Ark, N, k1, 0) + orc_flop_counts('m', m2, N, Ark, 0);
+    free(AVBD);
+}
+
+/* hcore_duncompress.c:33-84: CD = AU BV^T (M x N, inner size Ark); alpha
+ * and beta are not used and the two ranks must be equal. */
+int orc_duncompress(int M, int N, const double *AU, int Ark, int LDA, const double *BV, int Brk,
+                    int LDB, double *CD, int LDC)
+{
+    if (Ark != Brk) return -1;
+    orc_dgemm_gemm('N', 'T', M, N, Ark, 1.0, AU, LDA, BV, LDB, 0.0, CD, LDC);
+    return 0;
 }

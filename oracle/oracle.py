@@ -55,6 +55,15 @@ def lib():
         L.orc_dsyrk_batch.restype = C.c_int
         L.orc_dsyrk_batch.argtypes = [C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_void_p,
                                       _ip, C.c_int, C.c_void_p, C.c_int, C.c_int, _ulp]
+        i, d = C.c_int, C.c_double
+        L.orc_dpotrf.restype = i
+        L.orc_dpotrf.argtypes = [i, i, _dp, i, _ulp]
+        L.orc_dtrsm.restype = i
+        L.orc_dtrsm.argtypes = [i, i, i, i, i, i, d, _dp, i, _dp, i, _ulp]
+        L.orc_dgemmbdcd.restype = None
+        L.orc_dgemmbdcd.argtypes = [i, i, i, d, _dp, _dp, i, i, _dp, i, d, _dp, i, _ulp]
+        L.orc_duncompress.restype = i
+        L.orc_duncompress.argtypes = [i, i, _dp, i, i, _dp, i, i, _dp, i]
         L.orc_blas_config.restype = C.c_char_p
         L.orc_max_threads.restype = C.c_int
         L.orc_set_blas_threads.argtypes = [C.c_int]
@@ -193,5 +202,51 @@ def dsyrk_batch(alpha, AU, AV, CD, nthreads=0):
     return fl.value
 
 
+def set_blas_threads(n):
+    """Threads of the oracle's BLAS inside one call."""
+    lib().orc_set_blas_threads(int(n))
+
+
 def max_threads():
     return int(lib().orc_max_threads())
+
+
+def dpotrf(uplo, A):
+    """Oracle HCORE_dpotrf on a copy of A: returns (factor, info, flops_potrf)."""
+    Aw = np.array(A, dtype=np.float64, order="F", copy=True)
+    fl = C.c_ulong(0)
+    info = lib().orc_dpotrf(uplo, Aw.shape[0], _ptr(Aw), Aw.shape[0], C.byref(fl))
+    return Aw, int(info), fl.value
+
+
+def dtrsm(side, uplo, trans, diag, alpha, A, B):
+    """Oracle HCORE_dtrsm on a copy of B (M x Brank): returns (X, flops_trsm)."""
+    A = _f64(A)
+    Bw = np.array(B, dtype=np.float64, order="F", copy=True)
+    fl = C.c_ulong(0)
+    st = lib().orc_dtrsm(side, uplo, trans, diag, Bw.shape[0], Bw.shape[1], alpha, _ptr(A),
+                         A.shape[0], _ptr(Bw), Bw.shape[0], C.byref(fl))
+    assert st == 0
+    return Bw, fl.value
+
+
+def dgemmbdcd(transA, alpha, AU, AV, BD, beta, CD):
+    """Oracle HCORE_dgemmbdcd (ld = M throughout): returns (CD_new, flops_update)."""
+    AU, AV, BD = _f64(AU), _f64(AV), _f64(BD)
+    CDw = np.array(CD, dtype=np.float64, order="F", copy=True)
+    M, N = CDw.shape
+    fl = C.c_ulong(0)
+    lib().orc_dgemmbdcd(transA, M, N, alpha, _ptr(AU), _ptr(AV), AU.shape[1], M, _ptr(BD), M,
+                        beta, _ptr(CDw), M, C.byref(fl))
+    return CDw, fl.value
+
+
+def duncompress(AU, BV):
+    """Oracle HCORE_duncompress: AU BV^T."""
+    AU, BV = _f64(AU), _f64(BV)
+    M, N = AU.shape[0], BV.shape[0]
+    CD = np.zeros((M, N), order="F")
+    st = lib().orc_duncompress(M, N, _ptr(AU), AU.shape[1], M, _ptr(BV), BV.shape[1], N,
+                               _ptr(CD), M)
+    assert st == 0
+    return CD

@@ -229,3 +229,75 @@ def test_oracle_dsyrk_dgemmcd_closed_forms(orc):
     Bu, Bv = rng.standard_normal((M, 5)), rng.standard_normal((M, 5))
     out = orc.dgemmcd(AU, AV, Bu, Bv, CD)
     assert rel_fro(out, CD - (AU @ AV.T) @ (Bu @ Bv.T).T) < 1e-13
+
+
+# ---- dense neighbours: potrf, trsm, dgemmbdcd, duncompress -------------------
+def _spd(n, seed):
+    rng = np.random.default_rng(seed)
+    G = rng.standard_normal((n, n))
+    return np.asfortranarray(G @ G.T + n * np.eye(n))
+
+
+@pytest.mark.parametrize("uplo", [121, 122])
+def test_oracle_dpotrf_pins(orc, uplo):
+    """The factor reproduces A (L L^T or U^T U) to 1e-13 and equals numpy's
+    Cholesky (the unique factor with a positive diagonal); the other
+    triangle is untouched; a matrix that is not positive definite reports
+    the order of the failing minor; flops->potrf gets the 'c' count."""
+    A = _spd(70, uplo)
+    F, info, fl = orc.dpotrf(uplo, A)
+    assert info == 0
+    L = np.linalg.cholesky(A)
+    if uplo == 122:
+        assert np.max(np.abs(np.tril(F) - L)) <= 1e-12 * np.abs(L).max()
+        assert np.array_equal(np.triu(F, 1), np.triu(A, 1))
+    else:
+        assert np.max(np.abs(np.triu(F) - L.T)) <= 1e-12 * np.abs(L).max()
+        assert np.array_equal(np.tril(F, -1), np.tril(A, -1))
+    assert fl == orc.flop_counts('c', 70)
+    B = A.copy(order="F")
+    B[40, 40] = -1e6
+    _, info, _ = orc.dpotrf(uplo, B)
+    assert info == 41
+
+
+@pytest.mark.parametrize("side", [141, 142])
+@pytest.mark.parametrize("uplo", [121, 122])
+@pytest.mark.parametrize("trans", [111, 112])
+@pytest.mark.parametrize("diag", [131, 132])
+def test_oracle_dtrsm_pins(orc, side, uplo, trans, diag):
+    """X solves op(A) X = alpha B (left) or X op(A) = alpha B (right) for the
+    stored triangle of A (unit diagonal when diag = Unit): residual at the
+    rounding level; flops->trsm gets the 't' count of the side."""
+    rng = np.random.default_rng(side + uplo + trans + diag)
+    M, N = 37, 11
+    n = M if side == 141 else N
+    A = np.asfortranarray(rng.standard_normal((n, n)) / n + 4 * np.eye(n))
+    B = np.asfortranarray(rng.standard_normal((M, N)))
+    X, fl = orc.dtrsm(side, uplo, trans, diag, 0.5, A, B)
+    T = np.triu(A) if uplo == 121 else np.tril(A)
+    if diag == 132:
+        T = T - np.diag(np.diag(T)) + np.eye(n)
+    opT = T.T if trans == 112 else T
+    R = opT @ X if side == 141 else X @ opT
+    assert np.max(np.abs(R - 0.5 * B)) <= 1e-13 * np.abs(B).max()
+    # numpy's general solve with the explicit triangle (zeros elsewhere)
+    ref = np.linalg.solve(opT, 0.5 * B) if side == 141 else np.linalg.solve(opT.T, 0.5 * B.T).T
+    assert np.max(np.abs(X - ref)) <= 1e-12 * np.abs(ref).max()
+    assert fl == orc.flop_counts('t', M, N, 1 if side == 141 else 2)
+
+
+@pytest.mark.parametrize("transA", [111, 112])
+def test_oracle_dgemmbdcd_uncompress_pins(orc, transA):
+    """dgemmbdcd: CD = alpha AU AV^T BD + beta CD (closed form, both of the
+    reference's branches at ld = M); duncompress: AU BV^T."""
+    rng = np.random.default_rng(transA)
+    M, N, r = 40, 40, 7
+    AU, AV = rng.standard_normal((M, r)), rng.standard_normal((M, r))
+    BD, CD = rng.standard_normal((M, N)), rng.standard_normal((M, N))
+    out, fl = orc.dgemmbdcd(transA, -1.0, AU, AV, BD, 0.5, CD)
+    ref = -1.0 * AU @ (AV.T @ BD) + 0.5 * CD
+    assert np.max(np.abs(out - ref)) <= 1e-13 * np.abs(ref).max()
+    assert fl == orc.flop_counts('m', r, N, M) + orc.flop_counts('m', M, N, r)
+    BV = rng.standard_normal((N, r))
+    assert np.max(np.abs(orc.duncompress(AU, BV) - AU @ BV.T)) <= 1e-13 * np.abs(AU @ BV.T).max()
