@@ -70,6 +70,17 @@ def lib():
         L.HCORE_dgemmcd.argtypes = [i, i, i, d, _dp, _dp, i, i, _dp, _dp, i, i, d, _dp, i, i,
                                     _dp, fl]
         L.HCORE_dgemmcd.restype = None
+        L.HCORE_dpotrf.argtypes = [i, i, _dp, i, _ip, fl]
+        L.HCORE_dpotrf.restype = None
+        L.HCORE_dtrsm.argtypes = [i, i, i, i, i, i, d, _dp, i, _dp, i, fl]
+        L.HCORE_dtrsm.restype = None
+        L.HCORE_dgemmbdcd.argtypes = [i, i, i, i, d, _dp, _dp, _dp, i, _dp, i, d, _dp, i, _dp, fl]
+        L.HCORE_dgemmbdcd.restype = None
+        L.HCORE_duncompress.argtypes = [i, i, i, i, d, _dp, _dp, i, _dp, _dp, i, d, _dp, i]
+        L.HCORE_duncompress.restype = None
+        for fn in ("HCORE_dgemm_qr_svd", "HCORE_dgemm_qr_svd_b_dense", "HCORE_dgemm_ormqr"):
+            getattr(L, fn).restype = None
+        L.HCORE_hip_gemm.argtypes = [i, i, i, i, i, i, d, _vp, i, _vp, i, d, _vp, i, i, _vp]
         L.flop_counts.argtypes = [C.c_char, C.c_ulong, C.c_ulong, C.c_ulong, C.c_ulong]
         L.flop_counts.restype = C.c_ulong
         L.HCORE_hip_init.argtypes = [i]
@@ -213,6 +224,127 @@ def dgemmcd(Au, Av, Bu, Bv, CD):
     lib().HCORE_dgemmcd(111, 112, M, -1.0, _ptr(Au), _ptr(Av), Au.shape[1], M, _ptr(Bu),
                         _ptr(Bv), Bu.shape[1], M, 1.0, _ptr(CDw), M, M, None, C.byref(fc))
     return CDw
+
+
+def dpotrf(uplo, A, flops=None):
+    """HCORE_dpotrf on a host copy of A; returns (factor, info)."""
+    Aw = np.array(A, dtype=np.float64, order="F", copy=True)
+    info = C.c_int(0)
+    fc = flops if flops is not None else FlopCounter()
+    lib().HCORE_dpotrf(uplo, Aw.shape[0], _ptr(Aw), Aw.shape[0], C.byref(info), C.byref(fc))
+    return Aw, info.value
+
+
+def dtrsm(side, uplo, trans, diag, alpha, A, B, flops=None):
+    """HCORE_dtrsm on host copies; returns X (B is M x Brank)."""
+    A = _fortran(A, np.float64)
+    Bw = np.array(B, dtype=np.float64, order="F", copy=True)
+    fc = flops if flops is not None else FlopCounter()
+    lib().HCORE_dtrsm(side, uplo, trans, diag, Bw.shape[0], Bw.shape[1], alpha, _ptr(A),
+                      A.shape[0], _ptr(Bw), Bw.shape[0], C.byref(fc))
+    return Bw
+
+
+def dgemmbdcd(transA, alpha, AU, AV, BD, beta, CD, flops=None):
+    """HCORE_dgemmbdcd on host copies (ld = M throughout); returns CD_new."""
+    AU, AV, BD = (_fortran(x, np.float64) for x in (AU, AV, BD))
+    CDw = np.array(CD, dtype=np.float64, order="F", copy=True)
+    M, N = CDw.shape
+    rk = np.array([AU.shape[1]], dtype=np.float64)
+    fc = flops if flops is not None else FlopCounter()
+    lib().HCORE_dgemmbdcd(transA, 111, M, N, alpha, _ptr(AU), _ptr(AV), _ptr(rk), M, _ptr(BD),
+                          M, beta, _ptr(CDw), M, None, C.byref(fc))
+    return CDw
+
+
+def duncompress(AU, BV):
+    """HCORE_duncompress on host arrays; returns AU BV^T."""
+    AU, BV = _fortran(AU, np.float64), _fortran(BV, np.float64)
+    M, N = AU.shape[0], BV.shape[0]
+    CD = np.zeros((M, N), order="F")
+    ra = np.array([AU.shape[1]], dtype=np.float64)
+    rb = np.array([BV.shape[1]], dtype=np.float64)
+    lib().HCORE_duncompress(111, 112, M, N, 1.0, _ptr(AU), _ptr(ra), M, _ptr(BV), _ptr(rb), N,
+                            0.0, _ptr(CD), M)
+    return CD
+
+
+def dgemm_split(alpha, AU, AV, BU, BV, beta, CU, CV, rk=0, maxrk=None, acc=1e-8,
+                storage_maxrank=None, B_dense=None, flops=None, between=None):
+    """HCORE_dgemm_qr_svd (or _b_dense with a dense B) then HCORE_dgemm_ormqr
+    on host numpy factors.  Returns (CU_new, CV_new, rank); `between(r)` is
+    called with the revealed rank between the two phases."""
+    M = AU.shape[0]
+    ra, rc = AU.shape[1], CU.shape[1]
+    maxrk = M if maxrk is None else maxrk
+    smr = (ra + rc) if storage_maxrank is None else storage_maxrank
+    L = lib()
+    f64 = lambda x: _fortran(x, np.float64)
+    AU, AV = f64(AU), f64(AV)
+    CUw = np.zeros((M, ra + rc), order="F")
+    CVw = np.zeros((M, ra + rc), order="F")
+    CUw[:, :rc], CVw[:, :rc] = CU, CV
+    iA, iC = np.array([ra], np.int32), np.array([rc], np.int32)
+    ip = lambda a: a.ctypes.data_as(_ip)
+    fc = flops if flops is not None else FlopCounter()
+    # the out-parameters in the reference's order: work_new, _CU, _CV, CU_ncols,
+    # new_UVrk, newU, ld_newU, qrtauA, CV_ncols, newV, ld_newV, qrtauB,
+    # use_CUV_clone, CUclone, ld_CUclone, _CU_save, CVclone, ld_CVclone, _CV_save
+    keep = {}
+    names = ["work_new", "_CU", "_CV", "CU_ncols", "new_UVrk", "newU", "ld_newU", "qrtauA",
+             "CV_ncols", "newV", "ld_newV", "qrtauB", "use_CUV_clone", "CUclone", "ld_CUclone",
+             "_CU_save", "CVclone", "ld_CVclone", "_CV_save"]
+    for nm in names:
+        keep[nm] = C.c_int(0) if nm in ("CU_ncols", "new_UVrk", "ld_newU", "CV_ncols",
+                                         "ld_newV", "use_CUV_clone", "ld_CUclone",
+                                         "ld_CVclone") else C.c_void_p()
+    refs = [C.byref(keep[nm]) for nm in names]
+    if B_dense is None:
+        BU, BV = f64(BU), f64(BV)
+        iB = np.array([BU.shape[1]], np.int32)
+        L.HCORE_dgemm_qr_svd(111, 112, M, M, C.c_double(alpha), _ptr(AU), _ptr(AV), ip(iA), M,
+                             _ptr(BU), _ptr(BV), ip(iB), M, C.c_double(beta), _ptr(CUw), _ptr(CVw),
+                             ip(iC), M, rk, smr, maxrk, C.c_double(acc), None, C.byref(fc), *refs)
+    else:
+        Bd = f64(B_dense)
+        L.HCORE_dgemm_qr_svd_b_dense(111, 112, M, M, C.c_double(alpha), _ptr(AU), _ptr(AV),
+                                     ip(iA), M, _ptr(Bd), M, C.c_double(beta), _ptr(CUw),
+                                     _ptr(CVw), ip(iC), M, rk, smr, maxrk, C.c_double(acc), None,
+                                     C.byref(fc), *refs)
+    st = last_status()
+    if st:
+        return None, None, -st
+    r = keep["new_UVrk"].value
+    if between is not None:
+        between(r)
+    v = lambda nm: keep[nm].value if isinstance(keep[nm], C.c_int) else C.c_void_p(keep[nm].value)
+    L.HCORE_dgemm_ormqr(111, 112, M, M, C.c_double(alpha), _ptr(AU), _ptr(AV), ip(iA), M,
+                        C.c_double(beta), _ptr(CUw), _ptr(CVw), ip(iC), M, rk, smr, maxrk,
+                        C.c_double(acc), None, C.byref(fc), *[v(nm) for nm in names[1:]])
+    st = last_status()
+    if st:
+        return None, None, -st
+    r = int(iC[0])
+    return CUw[:, :r].copy(order="F"), CVw[:, :r].copy(order="F"), r
+
+
+def gemm(A, B, C_in, alpha=1.0, beta=0.0, transA=False, transB=False, path=0):
+    """HCORE_hip_gemm on device copies of numpy arrays (float64 or float32);
+    returns the new C."""
+    dt = C_in.dtype
+    a = DeviceBuffer.from_numpy(np.asfortranarray(A, dtype=dt))
+    b = DeviceBuffer.from_numpy(np.asfortranarray(B, dtype=dt))
+    c = DeviceBuffer.from_numpy(np.asfortranarray(C_in, dtype=dt))
+    m, n = C_in.shape
+    k = A.shape[0] if transA else A.shape[1]
+    st = lib().HCORE_hip_gemm(FP64 if dt == np.float64 else FP32, 112 if transA else 111,
+                              112 if transB else 111, m, n, k, alpha, a.ptr, A.shape[0], b.ptr,
+                              B.shape[0], beta, c.ptr, m, path, None)
+    _check(st, "HCORE_hip_gemm")
+    out = c.numpy()
+    for x in (a, b, c):
+        x.free()
+    return out
 
 
 def _parr(ptrs):

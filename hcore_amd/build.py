@@ -23,6 +23,9 @@ ARCH = "gfx950"
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-command-line-argument",
+         # a value-returning function that can fall off its end is an error:
+         # at -O3 the host code would run into the padding after it
+         "-Werror=return-type",
          "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 
 

@@ -37,6 +37,10 @@ int syrk_print_index = 0;
 int gemmcd_print_index = 0;
 int gemmcd_print_mat = 0;
 int gemmcd_use_scratch = 0;
+int gemmbdcd_print_index = 0;
+int gemmbdcd_print_mat = 0;
+int gemmfrk_print_index = 0;
+int gemmfrk_print_mat = 0;
 
 // Closed-form operation counts (reference src/misc/descutil/flop_counts.c:3-70),
 // including its float-precision thirds in the 'q' formula.
@@ -248,7 +252,8 @@ struct Packer {
 template <typename T>
 static void add_tasks(TaskList& tl, int prob, const GemmProb<T>& g) {
   const int m = g.m, n = g.n;
-  const bool tall = !(g.flags & (GF_A_TRANS | GF_LOWER_ZERO_UPPER)) && m >= kTallMinM &&
+  const bool tall = !(g.flags & (GF_A_TRANS | GF_LOWER_ZERO_UPPER | GF_LOWER_ONLY | GF_UPPER_ONLY)) &&
+                    m >= kTallMinM &&
                     n <= kTallMaxN && g.k <= kTallMaxK;
   if (tall) {
     for (int tj = 0; tj < (n + kTallCols - 1) / kTallCols; ++tj)
@@ -290,6 +295,13 @@ struct hcore_hip_plan {
   size_t fixed_cap = 0;
   hipStream_t fin_stream = nullptr;
   std::vector<char>* desc_host = nullptr;  // keeps the uploaded descriptors alive
+  bool measure = false;  // finalize only computes the device bytes (dbytes)
+  // element offsets in ws of update 0's Us, Vs, tau_A, tau_B, Zu, Zv
+  size_t first_off[6] = {0, 0, 0, 0, 0, 0};
+  // dense-B updates (HCORE_dgemm_qr_svd_b_dense): G = BD AV replaces
+  // G = BU (AV^T BV)^T, and F is not formed
+  std::vector<void*> BD;
+  int ldbd = 0;
   // gemm group
   int ng = 0;
   std::vector<void*> AU, AV, BU, BV, CU, CV, CUo, CVo;
@@ -471,6 +483,10 @@ static int plan_finalize_t(hcore_hip_plan* p) {
   }
   p->sigma_off = o_sg;
   p->sigma_double = sizeof(T) == 8;
+  if (n > 0) {
+    p->first_off[0] = o_Us[0]; p->first_off[1] = o_Vs[0]; p->first_off[2] = o_tA[0];
+    p->first_off[3] = o_tB[0]; p->first_off[4] = o_Zu[0]; p->first_off[5] = o_Zv[0];
+  }
   if (mixed) {  // sigma of the FP64 core, as an offset in doubles
     for (int u = 0; u < n; ++u) {
       const int K = Kv[u];
@@ -519,18 +535,26 @@ static int plan_finalize_t(hcore_hip_plan* p) {
         T* Vs = W + o_Vs[u];
         d.stack.push_back({(const T*)p->CU[u], (const T*)p->CV[u], (const T*)p->AU[u], Us, Vs,
                            rc, ra, p->ldc, p->lda});
-        // F = AV^T BV   (hcore_dgemm.c:175-178)
         GemmProb<T> g{};
-        g.A = (const T*)p->AV[u]; g.B = (const T*)p->BV[u]; g.C = W + o_F[u];
-        g.m = ra; g.n = rb; g.k = M; g.lda = p->lda; g.ldb = p->ldb; g.ldc = ra;
-        g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_BETA_ZERO;
-        add_tasks(d.tF, (int)gp.size(), g); gp.push_back(g);
-        // G = BU F^T into Vs(:, rc:K)   (hcore_dgemm.c:199-205)
-        g = GemmProb<T>{};
-        g.A = (const T*)p->BU[u]; g.B = W + o_F[u]; g.C = Vs + (size_t)rc * M;
-        g.m = M; g.n = ra; g.k = rb; g.lda = p->ldb; g.ldb = ra; g.ldc = M;
-        g.alpha = T(1); g.beta = T(0); g.flags = GF_B_TRANS | GF_BETA_ZERO;
-        add_tasks(d.tG, (int)gp.size(), g); gp.push_back(g);
+        if (p->BD.empty()) {
+          // F = AV^T BV   (hcore_dgemm.c:175-178)
+          g.A = (const T*)p->AV[u]; g.B = (const T*)p->BV[u]; g.C = W + o_F[u];
+          g.m = ra; g.n = rb; g.k = M; g.lda = p->lda; g.ldb = p->ldb; g.ldc = ra;
+          g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_BETA_ZERO;
+          add_tasks(d.tF, (int)gp.size(), g); gp.push_back(g);
+          // G = BU F^T into Vs(:, rc:K)   (hcore_dgemm.c:199-205)
+          g = GemmProb<T>{};
+          g.A = (const T*)p->BU[u]; g.B = W + o_F[u]; g.C = Vs + (size_t)rc * M;
+          g.m = M; g.n = ra; g.k = rb; g.lda = p->ldb; g.ldb = ra; g.ldc = M;
+          g.alpha = T(1); g.beta = T(0); g.flags = GF_B_TRANS | GF_BETA_ZERO;
+          add_tasks(d.tG, (int)gp.size(), g); gp.push_back(g);
+        } else {
+          // G = B AV into Vs(:, rc:K) for a dense B   (hcore_dgemm.c:255-262)
+          g.A = (const T*)p->BD[u]; g.B = (const T*)p->AV[u]; g.C = Vs + (size_t)rc * M;
+          g.m = M; g.n = ra; g.k = M; g.lda = p->ldbd; g.ldb = p->lda; g.ldc = M;
+          g.alpha = T(1); g.beta = T(0); g.flags = GF_BETA_ZERO;
+          add_tasks(d.tG, (int)gp.size(), g); gp.push_back(g);
+        }
         // T = R_A R_B^T   (hcore_dgemm.c:384-386)
         g = GemmProb<T>{};
         g.A = Us; g.B = Vs; g.C = W + o_Tc[u];
@@ -703,6 +727,10 @@ static int plan_finalize_t(hcore_hip_plan* p) {
                                 ~size_t(255));
   const size_t wsb = (ws_elems * es + 255) & ~size_t(255);
   const size_t total = head + wsb + pk.buf.size() + 256;
+  if (p->measure) {
+    p->dbytes = total;
+    return HCORE_HIP_OK;
+  }
   char* base = nullptr;
   const bool single = p->arena || p->fixed;
   if (p->fixed) {
@@ -773,8 +801,10 @@ static int collect_all(hcore_hip_plan* p) {
   return HCORE_HIP_OK;
 }
 
+// part: 0 the whole pipeline; 1 up to the core SVD and Zv = T^T Zu (the rank
+// is known after it); 2 the rest (the reassembly of the new factors).
 template <typename T>
-static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false) {
+static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false, int part = 0) {
   const int M = p->M;
   auto D = [&](size_t off) { return p->desc + off; };
   const auto* gp = reinterpret_cast<const GemmProb<T>*>(D(p->o_gp));
@@ -824,6 +854,8 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
     const bool own = l > 0 && !(tm && p->serial);  // the lane runs on a stream of its own
     hipStream_t ls = own ? L.stream : s;
     if (own) (void)hipStreamWaitEvent(ls, p->ev_start, 0);
+    const bool first = part != 2, second = part != 1;
+    if (first) {
     phase(l, 0, ls, true, [&] {
       launch_stack<T>(reinterpret_cast<const StackProb<T>*>(D(L.o_stack)), L.n, M, L.stack_cols,
                       T(p->alpha), T(p->beta), ls);
@@ -857,6 +889,8 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
       launch_convert(reinterpret_cast<const ConvProb*>(D(L.o_cvout)), L.n_cv, L.cv_max, false, ls);
     });
     phase(l, 7, ls, true, [&] { gemm(gp, L.tZ, ls); });
+    }
+    if (second) {
     if (L.wy) {
       phase(l, 6, ls, true, [&] {
         gemm(gp, L.tB, ls);
@@ -871,7 +905,8 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
         launch_apply<T>(reinterpret_cast<const ApplyProb<T>*>(D(L.o_apply)), L.n_apply, M, ls);
       });
     }
-    if (l == syrk_lane) syrk(ls);
+    }
+    if (l == syrk_lane && part == 0) syrk(ls);
     if (own) (void)hipEventRecord(L.ev_done, ls);
   }
   if (nl == 0) syrk(s);
@@ -1309,6 +1344,708 @@ void HCORE_dgemmcd(HCORE_enum transA, HCORE_enum transB, int M, double alpha, do
                               tc->stream) != hipSuccess ||
                hipStreamSynchronize(tc->stream) != hipSuccess))
     return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: dgemmcd copy-back failed");
+}
+
+// ---- dense neighbours of the recompression: potrf, trsm, dgemmbdcd, uncompress ----
+}  // extern "C"
+
+namespace hcore {
+
+// HCORE_enum values are the CBLAS ones (reference hcore.h:30-44).
+constexpr int kNoTrans = 111, kTrans = 112, kConjTrans = 113, kUpper = 121, kLower = 122,
+              kNonUnit = 131, kUnit = 132, kLeft = 141, kRight = 142;
+
+// A list of GEMM steps packed once, uploaded once to the thread's chain
+// arena, then launched one step at a time between other kernels.
+struct GemmSteps {
+  std::vector<GemmProb<double>> gp;
+  std::vector<TaskList> tl;
+  std::vector<TaskRef> ref;
+  size_t o_gp = 0, o_extra = 0;
+  char* dev = nullptr;
+  int add(const GemmProb<double>& g) {
+    gp.push_back(g);
+    tl.emplace_back();
+    add_tasks(tl.back(), (int)gp.size() - 1, g);
+    return (int)gp.size() - 1;
+  }
+  void pack(Packer& pk) {
+    o_gp = pk.add(gp);
+    ref.resize(gp.size());
+    for (size_t i = 0; i < gp.size(); ++i) ref[i] = pk.add(tl[i]);
+    o_extra = (pk.buf.size() + 255) & ~size_t(255);
+  }
+  // Reserve the thread's chain arena for the descriptors plus `extra` bytes
+  // of device scratch after them (256-aligned); the addresses are final
+  // from here on, so descriptors may point into the scratch.
+  int reserve(ThreadCtx* tc, size_t extra) {
+    Packer pk;
+    pack(pk);
+    int st = tc->chain.reserve(o_extra + extra + 256);
+    if (st) return st;
+    dev = tc->chain.p;
+    return HCORE_HIP_OK;
+  }
+  // Pack and copy the descriptors (after reserve) on the thread's stream; the
+  // host image stays alive in the thread state until the stream syncs.
+  int upload(ThreadCtx* tc) {
+    Packer pk;
+    pack(pk);
+    tc->desc.swap(pk.buf);
+    if (!tc->desc.empty())
+      HCHECK(hipMemcpyAsync(dev, tc->desc.data(), tc->desc.size(), hipMemcpyHostToDevice,
+                            tc->stream));
+    return HCORE_HIP_OK;
+  }
+  char* extra() const { return dev + o_extra; }
+  void launch(int i, hipStream_t s) const {
+    launch_tasks<double>(reinterpret_cast<const GemmProb<double>*>(dev + o_gp), dev, ref[i], s,
+                         true);
+  }
+};
+
+// Blocked triangular solve on device memory: op(A) X = B (left) or
+// X op(A) = B (right), B (M x N) overwritten by X, after B = alpha B.  The
+// diagonal kTrsmNB blocks are solved by k_trsm_diag, the rest of B is updated
+// by GEMMs (hcore_dtrsm.c:24-48 calls cblas_dtrsm).  *skip (device) != 0
+// skips every kernel.  The GEMM steps go through `steps` (already populated
+// steps are kept; the caller uploads and the solve appends its own).
+struct TrsmPlan {
+  TrsmDiagProb<double> d{};
+  std::vector<int> order;   // diagonal blocks in solve order
+  std::vector<int> gstep;   // GEMM step after each block (-1: none)
+};
+
+static TrsmPlan trsm_plan(GemmSteps& steps, bool left, bool lowerA, bool transA, bool unit, int M,
+                          int N, const double* A, int lda, double* B, int ldb, const int* skip) {
+  TrsmPlan t;
+  const bool Tlow = lowerA != transA;  // op(A) lower triangular
+  const int n = left ? M : N;          // order of op(A)
+  t.d.A = A;
+  t.d.lda = lda;
+  t.d.B = B;
+  t.d.alpha = 1.0;
+  t.d.skip = skip;
+  t.d.unit = unit;
+  t.d.nrhs = left ? N : M;
+  if (left) {
+    t.d.es = 1; t.d.cs = ldb; t.d.trans = transA; t.d.lower = Tlow;
+  } else {  // x^T T = b^T  <=>  T^T x = b for every row of B
+    t.d.es = ldb; t.d.cs = 1; t.d.trans = !transA; t.d.lower = !Tlow;
+  }
+  const bool fwd = t.d.lower;
+  const int nblk = (n + kTrsmNB - 1) / kTrsmNB;
+  for (int idx = 0; idx < nblk; ++idx) {
+    const int b = fwd ? idx : nblk - 1 - idx;
+    const int b0 = b * kTrsmNB, nbk = std::min(kTrsmNB, n - b0);
+    const int lo = fwd ? b0 + nbk : 0, hi = fwd ? n : b0;  // the part still to update
+    t.order.push_back(b);
+    if (hi <= lo) {
+      t.gstep.push_back(-1);
+      continue;
+    }
+    GemmProb<double> g{};
+    g.alpha = -1.0;
+    g.beta = 1.0;
+    g.skip = skip;
+    if (left) {  // B[lo:hi, :] -= op(A)[lo:hi, b] X_b
+      g.C = B + lo;
+      g.ldc = ldb;
+      g.m = hi - lo; g.n = N; g.k = nbk;
+      g.A = transA ? A + b0 + (size_t)lo * lda : A + lo + (size_t)b0 * lda;
+      g.lda = lda;
+      g.flags = transA ? GF_A_TRANS : 0;
+      g.B = B + b0;
+      g.ldb = ldb;
+    } else {  // B[:, lo:hi] -= X_b op(A)[b, lo:hi]
+      g.C = B + (size_t)lo * ldb;
+      g.ldc = ldb;
+      g.m = M; g.n = hi - lo; g.k = nbk;
+      g.A = B + (size_t)b0 * ldb;
+      g.lda = ldb;
+      g.B = transA ? A + lo + (size_t)b0 * lda : A + b0 + (size_t)lo * lda;
+      g.ldb = lda;
+      g.flags = transA ? GF_B_TRANS : 0;
+    }
+    t.gstep.push_back(steps.add(g));
+  }
+  return t;
+}
+
+static void trsm_run(const TrsmPlan& t, const GemmSteps& steps, int n, hipStream_t s) {
+  for (size_t i = 0; i < t.order.size(); ++i) {
+    TrsmDiagProb<double> d = t.d;
+    d.b0 = t.order[i] * kTrsmNB;
+    d.nbk = std::min(kTrsmNB, n - d.b0);
+    launch_trsm_diag<double>(d, s);
+    if (t.gstep[i] >= 0) steps.launch(t.gstep[i], s);
+  }
+}
+
+}  // namespace hcore
+
+extern "C" {
+
+void HCORE_dtrsm(HCORE_enum side, HCORE_enum uplo, HCORE_enum transA, HCORE_enum diag, int M,
+                 int Brank, double alpha, double* A, int LDA, double* BV, int LDBV,
+                 flop_counter* flops) {
+  t_last_status = HCORE_HIP_OK;
+  if (side != kLeft && side != kRight)  // the reference asserts (hcore_dtrsm.c:45-47)
+    return fail(HCORE_HIP_ERR_ARG, "HCORE_dtrsm: side is not CblasLeft or CblasRight");
+  if ((uplo != kUpper && uplo != kLower) ||
+      (transA != kNoTrans && transA != kTrans && transA != kConjTrans) ||
+      (diag != kNonUnit && diag != kUnit) || M < 0 || Brank < 0)
+    return fail(HCORE_HIP_ERR_ARG, "HCORE_dtrsm: invalid argument");
+  const int n = side == kLeft ? M : Brank;  // order of A
+  if (LDA < std::max(1, n) || LDBV < std::max(1, M))
+    return fail(HCORE_HIP_ERR_ARG, "HCORE_dtrsm: leading dimension too small");
+  if (flops) flops->trsm += flop_counts('t', M, Brank, side == kLeft ? 1 : 2, 0);
+  if (M == 0 || Brank == 0) return;
+  if (ctx_ensure() != HCORE_HIP_OK) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: GPU init failed");
+  ThreadCtx* tc = thread_ctx();
+  if (!tc) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: per-thread GPU state failed");
+  const bool dev = is_device_ptr(A) && is_device_ptr(BV);
+  Staged sg;
+  const int iA = sg.add(A, LDA, LDA, n, true), iB = sg.add(BV, LDBV, LDBV, Brank, true);
+  std::vector<double*> d;
+  if (sg.place(tc, dev, d) != HCORE_HIP_OK)
+    return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: staging failed");
+  hipStream_t s = tc->stream;
+  if (alpha != 1.0) launch_scale<double>(d[iB], LDBV, M, Brank, alpha, s);
+  if (alpha != 0.0) {
+    GemmSteps steps;
+    TrsmPlan t = trsm_plan(steps, side == kLeft, uplo == kLower, transA != kNoTrans,
+                           diag == kUnit, M, Brank, d[iA], LDA, d[iB], LDBV, nullptr);
+    if (steps.reserve(tc, 0) != HCORE_HIP_OK || steps.upload(tc) != HCORE_HIP_OK)
+      return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: dtrsm descriptors failed");
+    trsm_run(t, steps, n, s);
+  }
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && !dev)
+    e = hipMemcpy2DAsync(BV, sizeof(double) * LDBV, d[iB], sizeof(double) * LDBV,
+                         sizeof(double) * M, Brank, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: dtrsm failed");
+}
+
+void HCORE_dpotrf(HCORE_enum uplo, int N, double* A, int LDA, int* info, flop_counter* flops) {
+  t_last_status = HCORE_HIP_OK;
+  // the reference passes the enum to LAPACKE as the uplo character
+  // (hcore_dpotrf.c:29-32), which LAPACK rejects; the intended triangle is
+  // taken here, from the enum or from the LAPACK character
+  const bool upper = uplo == kUpper || uplo == 'U' || uplo == 'u';
+  const bool lower = uplo == kLower || uplo == 'L' || uplo == 'l';
+  if (!upper && !lower) {
+    if (info) *info = -1;
+    return;
+  }
+  if (N < 0 || LDA < std::max(1, N)) {
+    if (info) *info = N < 0 ? -2 : -4;
+    return;
+  }
+  if (flops) flops->potrf += flop_counts('c', N, 0, 0, 0);
+  if (info) *info = 0;
+  if (N == 0) return;
+  if (ctx_ensure() != HCORE_HIP_OK) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: GPU init failed");
+  ThreadCtx* tc = thread_ctx();
+  if (!tc) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: per-thread GPU state failed");
+  const bool dev = is_device_ptr(A);
+  Staged sg;
+  const int iA = sg.add(A, LDA, LDA, N, true);
+  std::vector<double*> d;
+  if (sg.place(tc, dev, d) != HCORE_HIP_OK)
+    return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: staging failed");
+  double* dA = d[iA];
+  hipStream_t s = tc->stream;
+  // right-looking blocked Cholesky: factor the diagonal block, solve the
+  // panel against it, update the trailing triangle (one GEMM per block)
+  GemmSteps steps;
+  std::vector<int> tstep;
+  const int nblk = (N + kTrsmNB - 1) / kTrsmNB;
+  for (int b = 0; b < nblk; ++b) {
+    const int b0 = b * kTrsmNB, nbk = std::min(kTrsmNB, N - b0), r1 = b0 + nbk;
+    if (r1 >= N) {
+      tstep.push_back(-1);
+      continue;
+    }
+    GemmProb<double> g{};
+    g.alpha = -1.0;
+    g.beta = 1.0;
+    g.m = g.n = N - r1;
+    g.k = nbk;
+    g.C = dA + r1 + (size_t)r1 * LDA;
+    g.ldc = g.lda = g.ldb = LDA;
+    if (lower) {  // A[r1:, r1:] -= L[r1:, b] L[r1:, b]^T, lower triangle
+      g.A = g.B = dA + r1 + (size_t)b0 * LDA;
+      g.flags = GF_B_TRANS | GF_LOWER_ONLY;
+    } else {  // A[r1:, r1:] -= U[b, r1:]^T U[b, r1:], upper triangle
+      g.A = g.B = dA + b0 + (size_t)r1 * LDA;
+      g.flags = GF_A_TRANS | GF_UPPER_ONLY;
+    }
+    tstep.push_back(steps.add(g));
+  }
+  if (steps.reserve(tc, sizeof(int)) != HCORE_HIP_OK)
+    return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: dpotrf descriptors failed");
+  int* dinfo = reinterpret_cast<int*>(steps.extra());  // device info word after the descriptors
+  for (auto& g : steps.gp) g.skip = dinfo;
+  if (steps.upload(tc) != HCORE_HIP_OK)
+    return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: dpotrf descriptors failed");
+  hipError_t e = hipMemsetAsync(dinfo, 0, sizeof(int), s);
+  for (int b = 0; b < nblk && e == hipSuccess; ++b) {
+    const int b0 = b * kTrsmNB, nbk = std::min(kTrsmNB, N - b0), r1 = b0 + nbk;
+    launch_potf2<double>(dA, LDA, b0, nbk, upper ? 1 : 0, dinfo, s);
+    if (r1 >= N) break;
+    TrsmDiagProb<double> pd{};
+    pd.A = dA; pd.lda = LDA; pd.alpha = 1.0; pd.skip = dinfo; pd.unit = 0;
+    pd.b0 = b0; pd.nbk = nbk; pd.nrhs = N - r1; pd.lower = 1;
+    if (lower) {  // X L_bb^T = A[r1:, b]: every row of the panel is a right-hand side
+      pd.B = dA + r1; pd.es = LDA; pd.cs = 1; pd.trans = 0;
+    } else {      // U_bb^T X = A[b, r1:]: every column of the panel
+      pd.B = dA + (size_t)r1 * LDA; pd.es = 1; pd.cs = LDA; pd.trans = 1;
+    }
+    launch_trsm_diag<double>(pd, s);
+    steps.launch(tstep[b], s);
+    e = hipGetLastError();
+  }
+  int* hres = tc->hres;
+  if (e == hipSuccess) e = hipMemcpyAsync(hres, dinfo, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess && !dev)
+    e = hipMemcpy2DAsync(A, sizeof(double) * LDA, dA, sizeof(double) * LDA, sizeof(double) * N, N,
+                         hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: dpotrf failed");
+  if (info) *info = hres[0];
+}
+
+void HCORE_dgemmbdcd(HCORE_enum transA, HCORE_enum transB, int M, int N, double alpha,
+                     double* AU, double* AV, double* Ark, int LDA, double* BD, int LDB, double beta,
+                     double* CD, int LDC, double* work, flop_counter* flops) {
+  (void)transB; (void)work;
+  t_last_status = HCORE_HIP_OK;
+  double rk = 0;
+  if (!read_rank(Ark, &rk)) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: cannot read rank");
+  const int ark = (int)rk;
+  // the reference's two branches (hcore_dgemmbdcd.c:76-116): inner size of
+  // AVBD = AV^T BD and row count of CD = alpha AU AVBD + beta CD
+  const int k1 = transA == kTrans ? M : LDB;
+  const int m2 = transA == kTrans ? LDA : M;
+  if (ark <= 0 || M <= 0 || N <= 0 || LDA < std::max(k1, m2) || LDB < k1 || LDC < m2)
+    return fail(HCORE_HIP_ERR_ARG, "HCORE_dgemmbdcd: invalid rank or dimensions");
+  if (ctx_ensure() != HCORE_HIP_OK) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: GPU init failed");
+  ThreadCtx* tc = thread_ctx();
+  if (!tc) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: per-thread GPU state failed");
+  const bool dev = is_device_ptr(AU) && is_device_ptr(AV) && is_device_ptr(BD) && is_device_ptr(CD);
+  Staged sg;
+  const int iAU = sg.add(AU, LDA, LDA, ark, true), iAV = sg.add(AV, LDA, LDA, ark, true);
+  const int iBD = sg.add(BD, LDB, LDB, N, true), iCD = sg.add(CD, LDC, LDC, N, beta != 0.0);
+  const int iW = sg.add(nullptr, ark, 0, N, false);
+  std::vector<double*> d;
+  if (sg.place(tc, dev, d) != HCORE_HIP_OK)
+    return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: staging failed");
+  std::vector<GemmProb<double>> gp(2);
+  gp[0] = GemmProb<double>{d[iAV], d[iBD], d[iW], ark, N, k1, LDA, LDB, ark, 1.0, 0.0,
+                           GF_A_TRANS | GF_BETA_ZERO, nullptr, nullptr};
+  gp[1] = GemmProb<double>{d[iAU], d[iW], d[iCD], m2, N, ark, LDA, ark, LDC, alpha, beta,
+                           beta == 0.0 ? GF_BETA_ZERO : 0, nullptr, nullptr};
+  if (run_gemm_chain(tc, gp) != HCORE_HIP_OK)
+    return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: dgemmbdcd failed");
+  if (!dev && (hipMemcpy2DAsync(CD, sizeof(double) * LDC, d[iCD], sizeof(double) * LDC,
+                                sizeof(double) * m2, N, hipMemcpyDeviceToHost,
+                                tc->stream) != hipSuccess ||
+               hipStreamSynchronize(tc->stream) != hipSuccess))
+    return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: dgemmbdcd copy-back failed");
+  if (flops)
+    flops->update += flop_counts('m', ark, N, k1, 0) + flop_counts('m', m2, N, ark, 0);
+}
+
+void HCORE_duncompress(HCORE_enum transA, HCORE_enum transB, int M, int N, double alpha,
+                       double* AU, double* Ark, int LDA, double* BV, double* Brk, int LDB,
+                       double beta, double* CD, int LDC) {
+  (void)transA; (void)transB; (void)alpha; (void)beta;
+  t_last_status = HCORE_HIP_OK;
+  double ra = 0, rb = 0;
+  if (!read_rank(Ark, &ra) || !read_rank(Brk, &rb))
+    return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: cannot read rank");
+  const int ark = (int)ra;
+  if ((int)ra != (int)rb)  // the reference asserts (hcore_duncompress.c:67)
+    return fail(HCORE_HIP_ERR_ARG, "HCORE_duncompress: the two ranks must be equal");
+  if (ark <= 0 || M <= 0 || N <= 0 || LDA < M || LDB < N || LDC < M)
+    return fail(HCORE_HIP_ERR_ARG, "HCORE_duncompress: invalid rank or dimensions");
+  if (ctx_ensure() != HCORE_HIP_OK) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: GPU init failed");
+  ThreadCtx* tc = thread_ctx();
+  if (!tc) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: per-thread GPU state failed");
+  const bool dev = is_device_ptr(AU) && is_device_ptr(BV) && is_device_ptr(CD);
+  Staged sg;
+  const int iAU = sg.add(AU, LDA, LDA, ark, true), iBV = sg.add(BV, LDB, LDB, ark, true);
+  const int iCD = sg.add(CD, LDC, LDC, N, false);
+  std::vector<double*> d;
+  if (sg.place(tc, dev, d) != HCORE_HIP_OK)
+    return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: staging failed");
+  std::vector<GemmProb<double>> gp(1);
+  gp[0] = GemmProb<double>{d[iAU], d[iBV], d[iCD], M, N, ark, LDA, LDB, LDC, 1.0, 0.0,
+                           GF_B_TRANS | GF_BETA_ZERO, nullptr, nullptr};
+  if (run_gemm_chain(tc, gp) != HCORE_HIP_OK)
+    return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: duncompress failed");
+  if (!dev && (hipMemcpy2DAsync(CD, sizeof(double) * LDC, d[iCD], sizeof(double) * LDC,
+                                sizeof(double) * M, N, hipMemcpyDeviceToHost,
+                                tc->stream) != hipSuccess ||
+               hipStreamSynchronize(tc->stream) != hipSuccess))
+    return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: duncompress copy-back failed");
+}
+
+// ---- two-phase recompression (hcore_dgemm.c:1022-1699) --------------------------
+}  // extern "C"
+
+namespace hcore {
+
+// One update split at the rank reveal: phase 1 (stacking, both QRs, the core
+// SVD and truncation, Zv) leaves its state on the device; phase 2 (the
+// reassembly) finds it again through the address it handed out as newU.  The
+// state owns its device memory (from a pool shared by all threads), so the
+// caller may run other HCORE calls, on any thread, between the two phases.
+struct PendingGemm {
+  hcore_hip_plan plan;
+  DevArena* mem = nullptr;
+  std::vector<char> desc;
+  hipStream_t stream = nullptr;
+  int M = 0, ra = 0, rb = 0, rc = 0, K = 0, r = 0;
+  double* oCU = nullptr;  // the new factors, M x K (ld M), written by phase 2
+  double* oCV = nullptr;
+  bool bdense = false;
+};
+static std::mutex g_pending_mu;
+static std::vector<std::pair<const void*, PendingGemm*>> g_pending;
+static std::vector<DevArena*> g_arena_pool;
+
+static DevArena* pool_take() {
+  std::lock_guard<std::mutex> lk(g_pending_mu);
+  if (g_arena_pool.empty()) return new DevArena();
+  DevArena* a = g_arena_pool.back();
+  g_arena_pool.pop_back();
+  return a;
+}
+static void pool_give(DevArena* a) {
+  std::lock_guard<std::mutex> lk(g_pending_mu);
+  g_arena_pool.push_back(a);
+}
+static void pending_release(PendingGemm* pg) {
+  if (pg->mem) pool_give(pg->mem);
+  delete pg;
+}
+
+// Phase 1 of the split recompression; B_dense != nullptr selects G = B AV.
+static void split_phase1(int M, double alpha, double* AU, double* AV, int* Ark, int LDA,
+                         double* BU, double* BV, int* Brk, double* B_dense, int LDB, double beta,
+                         double* CU, double* CV, int* Crk, int LDC, int rk, int maxrk, double acc,
+                         double* work, flop_counter* flops, double** p_work_new, double** p_CU,
+                         double** p_CV, int* p_CU_ncols, int* p_new_UVrk, double** p_newU,
+                         int* p_ld_newU, double** p_qrtauA, int* p_CV_ncols, double** p_newV,
+                         int* p_ld_newV, double** p_qrtauB, int* p_use_clone, double** p_CUclone,
+                         int* p_ld_CUclone, double** p_CU_save, double** p_CVclone,
+                         int* p_ld_CVclone, double** p_CV_save, const char* name) {
+  t_last_status = HCORE_HIP_OK;
+  char msg[512];
+  const bool bd = B_dense != nullptr;
+  int rks[3] = {0, 0, 0};
+  auto rd = [](const int* p, int* out) {
+    if (!is_device_ptr(p)) {
+      *out = *p;
+      return true;
+    }
+    return hipMemcpy(out, p, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess;
+  };
+  if (!rd(Ark, &rks[0]) || (!bd && !rd(Brk, &rks[1])) || !rd(Crk, &rks[2]))
+    return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: cannot read rank");
+  const int ra = rks[0], rb = bd ? 0 : rks[1], rc = rks[2];
+  if (ra <= 0 || rc <= 0 || (!bd && rb <= 0)) {
+    snprintf(msg, sizeof msg, "%s: _Ark=%d _Crk=%d. These rank values should not be zero.", name,
+             ra, rc);
+    return fail(HCORE_HIP_ERR_ZERO_RANK, msg);
+  }
+  const int K = ra + rc;
+  if (M <= 0 || LDA < M || LDB < M || LDC < M)
+    return fail(HCORE_HIP_ERR_ARG, "hcore_amd: size or leading dimension below M");
+  if (K > 2 * maxrk) {
+    snprintf(msg, sizeof msg, "%s: Sum of ranks (%d) is too big! _Ark:%d _Crk:%d maxrank:%d (x2: %d)",
+             name, K, ra, rc, maxrk, 2 * maxrk);
+    return fail(HCORE_HIP_ERR_SUM_RANKS, msg);
+  }
+  if (K > M || K > kCoreMaxK) {
+    snprintf(msg, sizeof msg, "%s: stacked rank %d exceeds the tile size %d or the maximum %d", name,
+             K, M, kCoreMaxK);
+    return fail(HCORE_HIP_ERR_K_GT_MAXRK, msg);
+  }
+  if (ctx_ensure() != HCORE_HIP_OK) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: GPU init failed");
+  ThreadCtx* tc = thread_ctx();
+  if (!tc) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: per-thread GPU state failed");
+  hipStream_t s = tc->stream;
+  PendingGemm* pg = new PendingGemm();
+  pg->M = M; pg->ra = ra; pg->rb = rb; pg->rc = rc; pg->K = K; pg->stream = s; pg->bdense = bd;
+  hcore_hip_plan& p = pg->plan;
+  p.M = M;
+  p.prec = HCORE_HIP_FP64;
+  p.ng = 1;
+  p.ra = {ra}; p.rb = {bd ? 1 : rb}; p.rc = {rc};
+  p.alpha = alpha; p.beta = beta; p.rk = rk; p.maxrk = maxrk; p.acc = acc;
+  p.fin_stream = s;
+  p.desc_host = &pg->desc;
+  p.lda = M; p.ldb = M; p.ldc = M; p.ldco = M; p.ldbd = M;
+  // inputs are copied (device or host) into the state's own memory, ld = M:
+  // the caller's buffers are free to change between the phases
+  const size_t nA = (size_t)M * ra, nB = bd ? (size_t)M * M : (size_t)M * rb, nC = (size_t)M * K;
+  const size_t stage = sizeof(double) * (2 * nA + 2 * nB + 4 * nC) + 2048;
+  p.measure = true;
+  p.AU = {nullptr}; p.AV = {nullptr}; p.BU = {nullptr}; p.BV = {nullptr};
+  p.CU = {nullptr}; p.CV = {nullptr}; p.CUo = {nullptr}; p.CVo = {nullptr};
+  if (bd) p.BD = {nullptr};
+  if (plan_finalize_t<double>(&p) != HCORE_HIP_OK) {
+    pending_release(pg);
+    return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: split plan sizing failed");
+  }
+  const size_t plan_bytes = (p.dbytes + 255) & ~size_t(255);
+  pg->mem = pool_take();
+  if (pg->mem->reserve(plan_bytes + stage) != HCORE_HIP_OK) {
+    pending_release(pg);
+    return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: split workspace allocation failed");
+  }
+  double* q = reinterpret_cast<double*>(pg->mem->p + plan_bytes);
+  double *dAU = q, *dAV = q + nA; q += 2 * nA;
+  double *dBU = q, *dBV = bd ? q : q + nB; q += bd ? nB : 2 * nB;
+  double *dCU = q, *dCV = q + nC; q += 2 * nC;
+  pg->oCU = q; pg->oCV = q + nC;
+  auto up = [&](double* d, const double* h, int ld, int cols) {
+    return hipMemcpy2DAsync(d, sizeof(double) * M, h, sizeof(double) * ld, sizeof(double) * M,
+                            cols, hipMemcpyDefault, s);
+  };
+  hipError_t e = up(dAU, AU, LDA, ra);
+  if (!e) e = up(dAV, AV, LDA, ra);
+  if (!e) e = bd ? up(dBU, B_dense, LDB, M) : up(dBU, BU, LDB, rb);
+  if (!e && !bd) e = up(dBV, BV, LDB, rb);
+  if (!e) e = up(dCU, CU, LDC, rc);
+  if (!e) e = up(dCV, CV, LDC, rc);
+  if (e) {
+    (void)hipStreamSynchronize(s);
+    pending_release(pg);
+    return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: split staging failed");
+  }
+  p.measure = false;
+  p.fixed = pg->mem->p;
+  p.fixed_cap = plan_bytes;
+  p.AU = {dAU}; p.AV = {dAV}; p.CU = {dCU}; p.CV = {dCV};
+  p.CUo = {pg->oCU}; p.CVo = {pg->oCV};
+  if (bd) {
+    p.BD = {dBU};
+    p.BU = {dBU}; p.BV = {dBU};  // not read
+  } else {
+    p.BU = {dBU}; p.BV = {dBV};
+  }
+  int st = plan_finalize_t<double>(&p);
+  if (st == HCORE_HIP_OK) st = plan_execute_t<double>(&p, s, false, 1);
+  int* hres = tc->hres;
+  if (st == HCORE_HIP_OK &&
+      (hipMemcpyAsync(hres, p.d_rank, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+       hipMemcpyAsync(hres + 1, p.d_status, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+       hipStreamSynchronize(s) != hipSuccess))
+    st = HCORE_HIP_ERR_RUNTIME;
+  if (st != HCORE_HIP_OK) {
+    (void)hipStreamSynchronize(s);
+    pending_release(pg);
+    return fail(st, "hcore_amd: GPU execution failed");
+  }
+  const int r = hres[0], dst = hres[1];
+  if (dst == ST_R_GT_MAXRK) {
+    pending_release(pg);
+    snprintf(msg, sizeof msg, "%s: Rank after truncation is too big! finalrank:%d maxrank:%d", name,
+             r, maxrk);
+    return fail(HCORE_HIP_ERR_R_GT_MAXRK, msg);
+  }
+  if (dst != ST_OK) {
+    pending_release(pg);
+    snprintf(msg, sizeof msg, "%s: ERROR in the core SVD (non-finite values), K=%d", name, K);
+    return fail(HCORE_HIP_ERR_NUMERIC, msg);
+  }
+  pg->r = r;
+  // the workspace layout puts Zu, Zv and the taus at fixed offsets; hand out
+  // their device addresses in the reference's out-parameters
+  double* ws = reinterpret_cast<double*>(p.ws);
+  double* Zu = ws + p.first_off[4];
+  double* Zv = ws + p.first_off[5];
+  double *Us = ws + p.first_off[0], *Vs = ws + p.first_off[1];
+  double *tauA = ws + p.first_off[2], *tauB = ws + p.first_off[3];
+  {
+    std::lock_guard<std::mutex> lk(g_pending_mu);
+    g_pending.push_back({Zu, pg});
+  }
+  if (flops) {  // hcore_dgemm.c:126, 179, 208, 224, 434, 493 (b_dense: 263, 287)
+    unsigned long f = flop_counts('q', M, K, 0, 0) + flop_counts('q', M, K, 0, 0) +
+                      flop_counts('s', K, 0, 0, 0) + (unsigned long)r * (unsigned long)K;
+    f += bd ? flop_counts('m', M, ra, M, 0)
+            : flop_counts('m', ra, rb, M, 0) + flop_counts('m', M, ra, rb, 0);
+    flops->update += f;
+  }
+  *p_work_new = work;
+  *p_CU = Us; *p_CV = Vs;
+  *p_CU_ncols = K; *p_CV_ncols = K;
+  *p_new_UVrk = r;
+  *p_newU = Zu; *p_ld_newU = K;
+  *p_newV = Zv; *p_ld_newV = K;
+  *p_qrtauA = tauA; *p_qrtauB = tauB;
+  *p_use_clone = 1;
+  *p_CUclone = Us; *p_ld_CUclone = M;
+  *p_CVclone = Vs; *p_ld_CVclone = M;
+  *p_CU_save = CU; *p_CV_save = CV;
+}
+
+}  // namespace hcore
+
+extern "C" {
+
+void HCORE_dgemm_qr_svd(int transA, int transB, int M, int N, double alpha, double* AU, double* AV,
+                        int* Ark, int LDA, double* BU, double* BV, int* Brk, int LDB, double beta,
+                        double* CU, double* CV, int* Crk, int LDC, int rk, int storage_maxrank,
+                        int maxrk, double acc, double* work, flop_counter* flops,
+                        double** _p_work_new, double** _p__CU, double** _p__CV, int* _p_CU_ncols,
+                        int* _p_new_UVrk, double** _p_newU, int* _p_ld_newU, double** _p_qrtauA,
+                        int* _p_CV_ncols, double** _p_newV, int* _p_ld_newV, double** _p_qrtauB,
+                        int* _p_use_CUV_clone, double** _p_CUclone, int* _p_ld_CUclone,
+                        double** _p__CU_save, double** _p_CVclone, int* _p_ld_CVclone,
+                        double** _p__CV_save) {
+  (void)transA; (void)transB; (void)N; (void)storage_maxrank;
+  split_phase1(M, alpha, AU, AV, Ark, LDA, BU, BV, Brk, nullptr, LDB, beta, CU, CV, Crk, LDC, rk,
+               maxrk, acc, work, flops, _p_work_new, _p__CU, _p__CV, _p_CU_ncols, _p_new_UVrk,
+               _p_newU, _p_ld_newU, _p_qrtauA, _p_CV_ncols, _p_newV, _p_ld_newV, _p_qrtauB,
+               _p_use_CUV_clone, _p_CUclone, _p_ld_CUclone, _p__CU_save, _p_CVclone,
+               _p_ld_CVclone, _p__CV_save, "HCORE_dgemm_qr_svd");
+}
+
+void HCORE_dgemm_qr_svd_b_dense(int transA, int transB, int M, int N, double alpha, double* AU,
+                                double* AV, int* Ark, int LDA, double* B, int LDB, double beta,
+                                double* CU, double* CV, int* Crk, int LDC, int rk,
+                                int storage_maxrank, int maxrk, double acc, double* work,
+                                flop_counter* flops, double** _p_work_new, double** _p__CU,
+                                double** _p__CV, int* _p_CU_ncols, int* _p_new_UVrk,
+                                double** _p_newU, int* _p_ld_newU, double** _p_qrtauA,
+                                int* _p_CV_ncols, double** _p_newV, int* _p_ld_newV,
+                                double** _p_qrtauB, int* _p_use_CUV_clone, double** _p_CUclone,
+                                int* _p_ld_CUclone, double** _p__CU_save, double** _p_CVclone,
+                                int* _p_ld_CVclone, double** _p__CV_save) {
+  (void)transA; (void)transB; (void)N; (void)storage_maxrank;
+  if (!B) return fail(HCORE_HIP_ERR_ARG, "HCORE_dgemm_qr_svd_b_dense: B is NULL");
+  split_phase1(M, alpha, AU, AV, Ark, LDA, nullptr, nullptr, nullptr, B, LDB, beta, CU, CV, Crk,
+               LDC, rk, maxrk, acc, work, flops, _p_work_new, _p__CU, _p__CV, _p_CU_ncols,
+               _p_new_UVrk, _p_newU, _p_ld_newU, _p_qrtauA, _p_CV_ncols, _p_newV, _p_ld_newV,
+               _p_qrtauB, _p_use_CUV_clone, _p_CUclone, _p_ld_CUclone, _p__CU_save, _p_CVclone,
+               _p_ld_CVclone, _p__CV_save, "HCORE_dgemm_qr_svd_b_dense");
+}
+
+void HCORE_dgemm_ormqr(int transA, int transB, int M, int N, double alpha, double* AU, double* AV,
+                       int* Ark, int LDA, double beta, double* CU, double* CV, int* Crk, int LDC,
+                       int rk, int storage_maxrank, int maxrk, double acc, double* work,
+                       flop_counter* flops, double* _CU, double* _CV, int CU_ncols, int new_UVrk,
+                       double* newU, int ld_newU, double* qrtauA, int CV_ncols, double* newV,
+                       int ld_newV, double* qrtauB, int use_CUV_clone, double* CUclone,
+                       int ld_CUclone, double* _CU_save, double* CVclone, int ld_CVclone,
+                       double* _CV_save) {
+  (void)transA; (void)transB; (void)N; (void)alpha; (void)AU; (void)AV; (void)Ark; (void)LDA;
+  (void)beta; (void)CU; (void)CV; (void)rk; (void)maxrk; (void)acc; (void)work; (void)_CU;
+  (void)_CV; (void)CU_ncols; (void)ld_newU; (void)qrtauA; (void)CV_ncols; (void)newV;
+  (void)ld_newV; (void)qrtauB; (void)use_CUV_clone; (void)CUclone; (void)ld_CUclone;
+  (void)CVclone; (void)ld_CVclone;
+  t_last_status = HCORE_HIP_OK;
+  char msg[512];
+  PendingGemm* pg = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_pending_mu);
+    for (size_t i = 0; i < g_pending.size(); ++i)
+      if (g_pending[i].first == newU) {
+        pg = g_pending[i].second;
+        g_pending.erase(g_pending.begin() + i);
+        break;
+      }
+  }
+  if (!pg)
+    return fail(HCORE_HIP_ERR_ARG, "HCORE_dgemm_ormqr: newU does not come from HCORE_dgemm_qr_svd");
+  if (new_UVrk > storage_maxrank) {  // hcore_dgemm.c:1648-1651
+    snprintf(msg, sizeof msg,
+             "HCORE_dgemm_ormqr: Increase storage maxrank %d. Observed rank for U and V is %d",
+             storage_maxrank, new_UVrk);
+    pending_release(pg);
+    return fail(HCORE_HIP_ERR_R_GT_MAXRK, msg);
+  }
+  const int Mp = pg->M, r = pg->r, K = pg->K;
+  if (M != Mp || LDC < Mp || new_UVrk != r) {
+    pending_release(pg);
+    return fail(HCORE_HIP_ERR_ARG, "HCORE_dgemm_ormqr: arguments differ from the first phase");
+  }
+  hipStream_t s = pg->stream;
+  int st = plan_execute_t<double>(&pg->plan, s, false, 2);
+  hipError_t e = st == HCORE_HIP_OK ? hipSuccess : hipErrorUnknown;
+  if (!e) e = hipMemcpy2DAsync(_CU_save, sizeof(double) * LDC, pg->oCU, sizeof(double) * Mp,
+                               sizeof(double) * Mp, r, hipMemcpyDefault, s);
+  if (!e) e = hipMemcpy2DAsync(_CV_save, sizeof(double) * LDC, pg->oCV, sizeof(double) * Mp,
+                               sizeof(double) * Mp, r, hipMemcpyDefault, s);
+  if (!e) e = hipStreamSynchronize(s);
+  pending_release(pg);
+  if (e) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: HCORE_dgemm_ormqr failed");
+  if (is_device_ptr(Crk)) {
+    if (hipMemcpy(Crk, &r, sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: cannot write rank");
+  } else {
+    Crk[0] = r;
+  }
+  if (flops)  // hcore_dgemm.c:595, 666
+    flops->update += flop_counts('o', Mp, r, K, 1) + flop_counts('o', r, Mp, K, 2);
+}
+
+// ---- the GEMM engine on its own ---------------------------------------------------
+int HCORE_hip_gemm(int precision, int transA, int transB, int m, int n, int k, double alpha,
+                   const void* A, int lda, const void* B, int ldb, double beta, void* C, int ldc,
+                   int path, void* stream) {
+  if (precision != HCORE_HIP_FP64 && precision != HCORE_HIP_FP32) return HCORE_HIP_ERR_ARG;
+  if (m < 0 || n < 0 || k < 0 || path < 0 || path > 4) return HCORE_HIP_ERR_ARG;
+  const bool ta = transA == kTrans || transA == kConjTrans, tb = transB == kTrans || transB == kConjTrans;
+  if (lda < std::max(1, ta ? k : m) || ldb < std::max(1, tb ? n : k) || ldc < std::max(1, m))
+    return HCORE_HIP_ERR_ARG;
+  if (m == 0 || n == 0) return HCORE_HIP_OK;
+  if (path == 4 && (ta || m < 1 || n > kTallMaxN || k > kTallMaxK)) return HCORE_HIP_ERR_ARG;
+  int st = ctx_ensure();
+  if (st) return st;
+  ThreadCtx* tc = thread_ctx();
+  if (!tc) return HCORE_HIP_ERR_RUNTIME;
+  hipStream_t s = stream ? (hipStream_t)stream : tc->stream;
+  const int flags = (ta ? GF_A_TRANS : 0) | (tb ? GF_B_TRANS : 0) | (beta == 0.0 ? GF_BETA_ZERO : 0);
+  auto run = [&](auto zero) -> int {
+    using T = decltype(zero);
+    GemmProb<T> g{};
+    g.A = (const T*)A; g.B = (const T*)B; g.C = (T*)C;
+    g.m = m; g.n = n; g.k = k; g.lda = lda; g.ldb = ldb; g.ldc = ldc;
+    g.alpha = T(alpha); g.beta = T(beta); g.flags = flags;
+    TaskList tl;
+    const int tile = path == 1 ? 64 : 32;
+    if (path == 0) {
+      add_tasks(tl, 0, g);
+    } else if (path == 4) {
+      for (int tj = 0; tj < (n + kTallCols - 1) / kTallCols; ++tj)
+        for (int ti = 0; ti < (m + kTallRows - 1) / kTallRows; ++ti) tl.tall.push_back({0, ti, tj});
+    } else {
+      auto& v = tile == 64 ? tl.big : tl.small;
+      for (int tj = 0; tj < (n + tile - 1) / tile; ++tj)
+        for (int ti = 0; ti < (m + tile - 1) / tile; ++ti) v.push_back({0, ti, tj});
+    }
+    Packer pk;
+    const size_t o_gp = pk.add(std::vector<GemmProb<T>>{g});
+    const TaskRef ref = pk.add(tl);
+    int st2 = tc->chain.reserve(pk.buf.size() + 256);
+    if (st2) return st2;
+    char* d = tc->chain.p;
+    tc->desc.swap(pk.buf);
+    HCHECK(hipMemcpyAsync(d, tc->desc.data(), tc->desc.size(), hipMemcpyHostToDevice, s));
+    launch_tasks<T>(reinterpret_cast<const GemmProb<T>*>(d + o_gp), d, ref, s, path == 0 || path == 3);
+    HCHECK(hipGetLastError());
+    HCHECK(hipStreamSynchronize(s));
+    return HCORE_HIP_OK;
+  };
+  return precision == HCORE_HIP_FP64 ? run(0.0) : run(0.0f);
 }
 
 // ---- context -----------------------------------------------------------------

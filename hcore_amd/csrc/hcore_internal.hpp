@@ -32,6 +32,8 @@ enum : int {
   GF_A_UNIT_LOWER = 64,     // stored A(p,q) read as 0 when p < q and 1 when p == q
   GF_B_UNIT_LOWER = 128,    // same for B
   GF_C0 = 256,              // the beta term reads C0 (rows < m0; 0 below) instead of C
+  GF_LOWER_ONLY = 512,      // write i >= j only; the strict upper triangle of C is not touched
+  GF_UPPER_ONLY = 1024,     // write i <= j only; the strict lower triangle of C is not touched
 };
 
 template <typename T>
@@ -214,6 +216,35 @@ void dgemm_fixed_workspace(int M, double alpha, double* AU, double* AV, double* 
                            hipStream_t s);
 // Print msg, record status for HCORE_hip_last_status and apply the error policy.
 void report_failure(int status, const char* msg);
+
+// ---- dense triangular kernels (k_dense.hip) ----------------------------------
+// One diagonal block of a triangular solve: for every right-hand side c <
+// nrhs, solve S y = alpha z in place, where z_k = B[(b0 + k) es + c cs]
+// (k < nbk <= kTrsmNB) and S(k, l) is A(b0 + k, b0 + l) (A(b0 + l, b0 + k)
+// when trans), lower (forward substitution) or upper (backward), with a unit
+// diagonal when unit.  Skipped when *skip != 0.
+constexpr int kTrsmNB = 64;
+template <typename T>
+struct TrsmDiagProb {
+  const T* A;
+  T* B;
+  long long es, cs;
+  const int* skip;
+  T alpha;
+  int lda, b0, nbk, nrhs;
+  int trans, lower, unit;
+};
+template <typename T>
+void launch_trsm_diag(const TrsmDiagProb<T>& p, hipStream_t s);
+// Unblocked Cholesky of the diagonal block A(b0.., b0..) of size nbk <=
+// kTrsmNB (lower: L L^T in the lower triangle; upper: U^T U in the upper);
+// the other triangle is not touched.  The first non-positive (or NaN) pivot
+// k sets *info = b0 + k + 1 and stops; nothing runs when *info != 0.
+template <typename T>
+void launch_potf2(T* A, int lda, int b0, int nbk, int upper, int* info, hipStream_t s);
+// B = alpha * B for an m x n block (alpha == 0: zero fill).
+template <typename T>
+void launch_scale(T* B, int ldb, int m, int n, T alpha, hipStream_t s);
 
 // Size in elements of the compact-WY storage for a K-column stack.
 inline int tp_elems(int K) { return ((K + kPB - 1) / kPB) * kPB * kPB; }

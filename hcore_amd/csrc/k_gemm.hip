@@ -24,11 +24,14 @@ namespace hcore {
 constexpr int GK = 32;  // k chunk staged in LDS
 
 // One 16 x 16 x 4 matrix-core step, D = A B + C, per precision.  Operand
-// layout (lane l): A(m = l % 16, k = l / 16), B(k = l / 16, n = l % 16);
-// result D(m = 4 (l / 16) + v, n = l % 16) in element v of the accumulator.
-// The kernels below feed op(B) as A and op(A) as B, so that lane l holds
-// C(i = l % 16, j = 4 (l / 16) + v): 16 consecutive rows of 4 columns, a
-// coalesced column-major store.
+// layout (lane l): A(m = l % 16, k = l / 16), B(k = l / 16, n = l % 16).
+// Element v of lane l's accumulator holds D(m, n = l % 16) with
+//   FP32 (v_mfma_f32_16x16x4_f32): m = 4 (l / 16) + v,
+//   FP64 (v_mfma_f64_16x16x4_f64): m = l / 16 + 4 v
+// (the double-precision result rows are interleaved by four).  The kernels
+// below feed op(B) as A and op(A) as B, so that lane l holds
+// C(i = l % 16, j = m(l, v)): 16 consecutive rows of 4 columns, a coalesced
+// column-major store.
 template <typename T>
 struct Mfma;
 template <>
@@ -37,6 +40,7 @@ struct Mfma<double> {
   static __device__ __forceinline__ V mac(double a, double b, V c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int m_of(int lane, int v) { return (lane >> 4) + 4 * v; }
 };
 template <>
 struct Mfma<float> {
@@ -44,6 +48,7 @@ struct Mfma<float> {
   static __device__ __forceinline__ V mac(float a, float b, V c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int m_of(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
 template <typename T>
@@ -110,6 +115,7 @@ __device__ __forceinline__ void gemm_stage(T (*As)[LDT], T (*Bs)[LDT], const T (
 template <typename T>
 __device__ __forceinline__ void gemm_store1(const GemmProb<T>& P, int i, int j, int n, T acc) {
   if (i >= P.m || j >= n) return;
+  if (((P.flags & GF_LOWER_ONLY) && i < j) || ((P.flags & GF_UPPER_ONLY) && i > j)) return;
   T* c = P.C + i + (size_t)j * P.ldc;
   if ((P.flags & GF_LOWER_ZERO_UPPER) && i < j) {
     *c = T(0);
@@ -134,7 +140,7 @@ __device__ __forceinline__ void gemm_store_wave(const GemmProb<T>& P, const type
     for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        gemm_store1(P, r0 + 16 * rb + (lane & 15), c0 + 16 * cb + 4 * (lane >> 4) + v, n,
+        gemm_store1(P, r0 + 16 * rb + (lane & 15), c0 + 16 * cb + Mfma<T>::m_of(lane, v), n,
                     acc[rb][cb][v]);
 }
 
@@ -178,6 +184,9 @@ __global__ __launch_bounds__((TILE / 4) * (TILE / 4)) void k_gemm(
   const int row0 = tk.ti * TILE, col0 = tk.tj * TILE;
   if (row0 >= P.m || col0 >= n) return;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  if (((P.flags & GF_LOWER_ONLY) && row0 + TILE - 1 < col0) ||
+      ((P.flags & GF_UPPER_ONLY) && col0 + TILE - 1 < row0))
+    return;  // the tile lies in the triangle that is not written
   const bool lzu = (P.flags & GF_LOWER_ZERO_UPPER) != 0;
   if (lzu && row0 + TILE - 1 < col0) {  // tile strictly above the diagonal: zero fill
     for (int e = t; e < TILE * TILE; e += NT) {
@@ -236,6 +245,9 @@ __global__ __launch_bounds__(64 * KS) void k_gemm_sk(const GemmProb<T>* __restri
   const int row0 = tk.ti * TILE, col0 = tk.tj * TILE;
   if (row0 >= P.m || col0 >= n) return;
   const int grp = threadIdx.x / NG, t = threadIdx.x % NG, lane = t;
+  if (((P.flags & GF_LOWER_ONLY) && row0 + TILE - 1 < col0) ||
+      ((P.flags & GF_UPPER_ONLY) && col0 + TILE - 1 < row0))
+    return;  // the tile lies in the triangle that is not written
   const bool lzu = (P.flags & GF_LOWER_ZERO_UPPER) != 0;
   if (lzu && row0 + TILE - 1 < col0) {
     for (int e = threadIdx.x; e < TILE * TILE; e += NT) {
@@ -278,7 +290,7 @@ __global__ __launch_bounds__(64 * KS) void k_gemm_sk(const GemmProb<T>* __restri
         for (int b = 0; b < 2; ++b)
 #pragma unroll
           for (int v = 0; v < 4; ++v)
-            buf[(16 * a + li) + (16 * b + 4 * lk + v) * TILE] = acc[a][b][v];
+            buf[(16 * a + li) + (16 * b + Mfma<T>::m_of(lane, v)) * TILE] = acc[a][b][v];
     }
     __syncthreads();
     if (grp < h) {
@@ -288,7 +300,8 @@ __global__ __launch_bounds__(64 * KS) void k_gemm_sk(const GemmProb<T>* __restri
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
-          for (int v = 0; v < 4; ++v) acc[a][b][v] += buf[(16 * a + li) + (16 * b + 4 * lk + v) * TILE];
+          for (int v = 0; v < 4; ++v)
+            acc[a][b][v] += buf[(16 * a + li) + (16 * b + Mfma<T>::m_of(lane, v)) * TILE];
     }
     __syncthreads();
   }
@@ -375,7 +388,7 @@ __global__ __launch_bounds__(256) void k_gemm_tall(const GemmProb<T>* __restrict
     for (int b = 0; b < CBK; ++b)
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        gemm_store1(P, rw + 16 * a + li, col0 + 16 * b + 4 * lk + v, n, acc[a][b][v]);
+        gemm_store1(P, rw + 16 * a + li, col0 + 16 * b + Mfma<T>::m_of(lane, v), n, acc[a][b][v]);
 }
 
 template <typename T>

@@ -333,6 +333,7 @@ bool launch_qr_reg(const QrProb<T>* probs, int n, int nb, int kmax, hipStream_t 
     else if (nb <= 512) launch_qr_reg_shape<T, 512, 1, 32>(probs, n, nb, s);
     else launch_qr_reg_shape<T, 512, 2, 32>(probs, n, nb, s);
   }
+  return true;
 }
 
 template <typename T>
@@ -345,6 +346,7 @@ bool launch_qr_fused(const QrProb<T>* probs, int n, int nb, hipStream_t s) {
   if (nb <= 256) hipLaunchKernelGGL((k_qr_fused<T, 256, 1, 32>), dim3(n), dim3(256), 0, s, probs, nb);
   else if (rpt <= 1) hipLaunchKernelGGL((k_qr_fused<T, NT, 1, PB>), dim3(n), dim3(NT), 0, s, probs, nb);
   else hipLaunchKernelGGL((k_qr_fused<T, NT, 2, PB>), dim3(n), dim3(NT), 0, s, probs, nb);
+  return true;
 }
 
 }  // namespace hcore

@@ -82,6 +82,68 @@ void HCORE_dsyrk(HCORE_enum uplo, HCORE_enum trans,
                  double *CD, int LDCD,
                  double *work, flop_counter *flops);
 
+/*
+ * CD <- alpha * AU*(AV^T*BD) + beta*CD for a low-rank A = AU*AV^T (rank
+ * Ark[0]) and a dense BD.  Replaces reference src/compute/hcore_dgemmbdcd.c:35-122
+ * (prototype src/include/hcore_d.h:87-101), with its two branches: for
+ * transA == HCORE_Trans the inner size of AV^T*BD is M and CD has LDA rows,
+ * otherwise the inner size is LDB and CD has M rows.  transB and work are
+ * ignored; flops->update gets both products' counts.
+ */
+void HCORE_dgemmbdcd(HCORE_enum transA, HCORE_enum transB,
+                     int M, int N,
+                     double alpha,
+                     double *AU, double *AV, double *Ark, int LDA,
+                     double *BD, int LDB,
+                     double beta,
+                     double *CD, int LDC,
+                     double *work, flop_counter *flops);
+
+/*
+ * Cholesky factorisation of one dense N x N tile in place: A = L*L^T (lower)
+ * or U^T*U (upper); the other triangle is not touched.  Replaces reference
+ * src/compute/hcore_dpotrf.c:23-34 (prototype src/include/hcore_d.h:102-106).
+ * The reference hands its HCORE_enum to LAPACKE as the uplo character, which
+ * LAPACK rejects; here uplo selects the intended triangle (HCORE_Upper = 121
+ * or 'U', HCORE_Lower = 122 or 'L'; anything else gives *info = -1).
+ * *info = k > 0 when the leading minor of order k is not positive definite.
+ * flops->potrf += the reference's 'c' count.
+ */
+void HCORE_dpotrf(HCORE_enum uplo,
+                  int N,
+                  double *A, int LDA,
+                  int *info,
+                  flop_counter *flops);
+
+/*
+ * Triangular solve op(A)*X = alpha*BV (side HCORE_Left, A M x M) or
+ * X*op(A) = alpha*BV (HCORE_Right, A Brank x Brank), BV (M x Brank) is
+ * overwritten by X.  Replaces reference src/compute/hcore_dtrsm.c:24-48
+ * (prototype src/include/hcore_d.h:117-125), which calls cblas_dtrsm with
+ * the same enum values.  flops->trsm += the reference's 't' count.
+ */
+void HCORE_dtrsm(HCORE_enum side, HCORE_enum uplo,
+                 HCORE_enum transA, HCORE_enum diag,
+                 int M, int Brank,
+                 double alpha,
+                 double *A, int LDA,
+                 double *BV, int LDBV,
+                 flop_counter *flops);
+
+/*
+ * Decompression CD = AU*BV^T (M x N, inner size Ark[0], which must equal
+ * Brk[0]).  Replaces reference src/compute/hcore_duncompress.c:33-84
+ * (prototype src/include/hcore_d.h:126-139): transA, transB, alpha and beta
+ * are ignored, as there.
+ */
+void HCORE_duncompress(HCORE_enum transA, HCORE_enum transB,
+                       int M, int N,
+                       double alpha,
+                       double *AU, double *Ark, int LDA,
+                       double *BV, double *Brk, int LDB,
+                       double beta,
+                       double *CD, int LDC);
+
 /* Data symbols the reference exports or imports (reference
  * src/compute/hcore_dgemm.c:37-43, hcore_dsyrk.c:21-22,
  * hcore_dgemmcd.c:24-26).  `use_scratch` is defined weakly (0) by the
@@ -96,6 +158,10 @@ extern int syrk_print_index;
 extern int gemmcd_print_index;
 extern int gemmcd_print_mat;
 extern int gemmcd_use_scratch;
+extern int gemmbdcd_print_index;   /* hcore_dgemmbdcd.c:23-24 */
+extern int gemmbdcd_print_mat;
+extern int gemmfrk_print_index;    /* read by hcore_duncompress.c:23-24 */
+extern int gemmfrk_print_mat;
 
 #ifdef __cplusplus
 }

@@ -134,6 +134,18 @@ void HCORE_hip_plan_destroy(hcore_hip_plan *plan);
 int HCORE_hip_plan_set_timing(hcore_hip_plan *plan, int enable);
 int HCORE_hip_plan_phase_times(hcore_hip_plan *plan, double *ms, int *count);
 
+/* The library's batched matrix-core GEMM on one problem (no counterpart in
+ * the reference): C = alpha*op(A)*op(B) + beta*C on device pointers, FP64 or
+ * FP32 (precision HCORE_HIP_FP64 / HCORE_HIP_FP32), column-major, trans
+ * values as HCORE_enum (111 no transpose, 112/113 transpose).  path selects
+ * the kernel family: 0 the one the plans would choose, 1 64 x 64 tiles,
+ * 2 32 x 32 tiles, 3 32 x 32 tiles with the k range split over four
+ * wavefronts, 4 the tall-skinny kernel (op(A) = A, n <= 128, k <= 128).
+ * stream NULL: the calling thread's library stream.  Synchronizes. */
+int HCORE_hip_gemm(int precision, int transA, int transB, int m, int n, int k, double alpha,
+                   const void *A, int lda, const void *B, int ldb, double beta, void *C, int ldc,
+                   int path, void *stream);
+
 /* Device memory and stream helpers (the library's own HIP runtime; callers
  * that share device buffers with the library allocate them here). */
 int HCORE_hip_malloc(void **ptr, size_t bytes);
