@@ -14,6 +14,8 @@
  */
 #ifndef HCORE_AMD_HCORE_D_H
 #define HCORE_AMD_HCORE_D_H
+#include <stddef.h>
+
 #include "flop_util_structs.h"
 #include "hcore.h"
 #ifdef __cplusplus
@@ -143,6 +145,29 @@ void HCORE_duncompress(HCORE_enum transA, HCORE_enum transB,
                        double *BV, double *Brk, int LDB,
                        double beta,
                        double *CD, int LDC);
+
+/*
+ * HCORE_dgemm without any device allocation: the same arguments and results,
+ * factors as DEVICE pointers only (ranks may be host or device doubles),
+ * run on the null stream.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The one-update
+ * workspace is a static device array of HCORE_dgemm_noalloc_workspace_bytes()
+ * (64 MiB) per GPU, held until the process ends; an update that needs more
+ * (e.g. M = 16384 with K = 512) is rejected with HCORE_HIP_ERR_ARG before any
+ * kernel runs.  Calls that overlap on one GPU share the array and race.
+ * Call HCORE_hip_init first.
+ */
+void HCORE_dgemm_noalloc(HCORE_enum transA, int transB,
+                         int M, int N,
+                         double alpha,
+                         double *AU, double *AV, double *Ark, int LDA,
+                         double *BU, double *BV, double *Brk, int LDB,
+                         double beta,
+                         double *CU, double *CV, double *Crk, int LDC,
+                         int rk, int maxrk, double acc,
+                         double *work, flop_counter *flops);
+size_t HCORE_dgemm_noalloc_workspace_bytes(void);
 
 /* Data symbols the reference exports or imports (reference
  * src/compute/hcore_dgemm.c:37-43, hcore_dsyrk.c:21-22,
