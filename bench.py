@@ -216,6 +216,12 @@ def main():
     assert np.all(status == 0), "device status errors: %s" % np.unique(status)
     sw = plan.sweeps()
     lane_of = plan.lanes()
+    # autotuning: every core launch timed with both wavefront counts, the
+    # faster kept (results unchanged; later plans of this process reuse it)
+    tune = plan.autotune(stream, reps=3)
+    for t in tune:
+        log("autotune lane %d: core variant %d -> %d (%.3f ms vs %.3f ms, %d tiles)" %
+            (int(t[0]), int(t[1]), int(t[2]), t[3], t[4], int(t[5])))
     log("Jacobi sweeps: mean %.1f max %d" % (sw.mean(), sw.max()))
     # numpy check of a sample (largest stacked ranks first): the product within
     # 10*acc of the dense update, the rank by the truncation rule on numpy's SVD
@@ -447,6 +453,8 @@ def main():
         "phase_ms": {lane_names[l]: {PHASES[i]: float(phase_ms[l, i]) for i in range(12)
                                      if c1[l, i] > c0[l, i]} for l in range(4)},
         "check": {"updates": len(sample), "worst_rel_err": worst, "max_rank_drift": rank_drift},
+        "autotune": [{"lane": int(t[0]), "base_variant": int(t[1]), "chosen_variant": int(t[2]),
+                      "ms_base": t[3], "ms_alternative": t[4], "tiles": int(t[5])} for t in tune],
         "device": devname,
     }
     print(json.dumps(out), flush=True)

@@ -103,6 +103,11 @@ def lib():
         L.HCORE_hip_plan_workspace_bytes.argtypes = [_vp]
         L.HCORE_hip_plan_workspace_bytes.restype = C.c_longlong
         L.HCORE_hip_plan_flops.argtypes = [_vp, C.POINTER(C.c_ulong), C.POINTER(C.c_ulong)]
+        L.HCORE_hip_plan_autotune.argtypes = [_vp, _vp, i, _dp, i]
+        L.HCORE_hip_plan_add_dgemm_dev.argtypes = [_vp, i, _pp, _pp, _pp, _ip, i, _pp, _pp, _pp, _ip,
+                                                   i, _pp, _pp, _pp, _ip, i, _pp, _pp, i, d, d, i,
+                                                   i, d]
+        L.HCORE_hip_plan_add_dsyrk_dev.argtypes = [_vp, i, _pp, _pp, _pp, _ip, i, _pp, i, d]
         L.HCORE_hip_plan_destroy.argtypes = [_vp]
         L.HCORE_hip_plan_destroy.restype = None
         L.HCORE_hip_plan_set_timing.argtypes = [_vp, i]
@@ -569,6 +574,46 @@ class Plan:
         _check(self.L.HCORE_hip_plan_phase_times(self.h, ms.ctypes.data_as(_dp),
                                                  cnt.ctypes.data_as(_ip)), "phase_times")
         return ms.reshape(4, 12), cnt.reshape(4, 12)
+
+    def autotune(self, stream=None, reps=3):
+        """HCORE_hip_plan_autotune: list of (lane, base variant, chosen variant,
+        ms base, ms alternative, tiles) per tuned core launch."""
+        rep = np.zeros(6 * 32)
+        n = self.L.HCORE_hip_plan_autotune(self.h, C.c_void_p(stream.h if stream else None), reps,
+                                           rep.ctypes.data_as(_dp), 32)
+        if n < 0:
+            raise RuntimeError("autotune failed: %d" % -n)
+        return [tuple(rep[6 * k:6 * k + 6]) for k in range(min(n, 32))]
+
+    def add_dgemm_dev(self, AU, AV, ra_ptrs, ra_cap, lda, BU, BV, rb_ptrs, rb_cap, ldb, CU, CV,
+                      rc_ptrs, rc_cap, ldc, alpha, beta, rk, maxrk, acc, CUout=None, CVout=None,
+                      ldco=None):
+        """HCORE_hip_plan_add_dgemm_dev: ranks read on the device from the given
+        int addresses, capacities as host ints."""
+        n = len(AU)
+        ldco = self.M if ldco is None else ldco
+        arrs = [_parr(x) for x in (AU, AV, ra_ptrs, BU, BV, rb_ptrs, CU, CV, rc_ptrs)]
+        caps = [_iarr(x) for x in (ra_cap, rb_cap, rc_cap)]
+        cuo = _parr(CUout) if CUout is not None else None
+        cvo = _parr(CVout) if CVout is not None else None
+        self._keep += arrs + [c[0] for c in caps] + [cuo, cvo]
+        st = self.L.HCORE_hip_plan_add_dgemm_dev(
+            self.h, n, arrs[0], arrs[1], arrs[2], caps[0][1], lda, arrs[3], arrs[4], arrs[5],
+            caps[1][1], ldb, arrs[6], arrs[7], arrs[8], caps[2][1], ldc, cuo, cvo, ldco, alpha,
+            beta, rk, maxrk, acc)
+        if st:
+            raise RuntimeError("HCORE_hip_plan_add_dgemm_dev failed: %d" % st)
+        self.n = n
+
+    def add_dsyrk_dev(self, AU, AV, k_ptrs, k_cap, ldau, CD, ldcd, alpha):
+        n = len(AU)
+        arrs = [_parr(x) for x in (AU, AV, k_ptrs, CD)]
+        kc = _iarr(k_cap)
+        self._keep += arrs + [kc[0]]
+        st = self.L.HCORE_hip_plan_add_dsyrk_dev(self.h, n, arrs[0], arrs[1], arrs[2], kc[1], ldau,
+                                                 arrs[3], ldcd, alpha)
+        if st:
+            raise RuntimeError("HCORE_hip_plan_add_dsyrk_dev failed: %d" % st)
 
     def workspace_bytes(self):
         return int(self.L.HCORE_hip_plan_workspace_bytes(self.h))

@@ -162,9 +162,34 @@ struct DevArena {
 // threads at once, one tile each; with per-thread state those calls share
 // no lock and overlap on the GPU, and a call allocates nothing once its
 // thread's buffers have grown to the tile size.
+// Grow-only pinned host buffer (hipHostMalloc): host operands are packed into
+// it on the CPU and moved by one DMA transfer each way, instead of one
+// pageable transfer per column.
+struct PinnedArena {
+  char* p = nullptr;
+  size_t cap = 0;
+  int reserve(size_t bytes) {
+    if (cap >= bytes) return HCORE_HIP_OK;
+    if (p) HCHECK(hipHostFree(p));
+    p = nullptr;
+    cap = 0;
+    const size_t mib = size_t(1) << 20;
+    const size_t c = ((bytes + mib - 1) / mib) * mib;
+    HCHECK(hipHostMalloc(reinterpret_cast<void**>(&p), c, hipHostMallocDefault));
+    cap = c;
+    return HCORE_HIP_OK;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
 struct ThreadCtx {
   hipStream_t stream = nullptr;
   DevArena plan, stage, chain;
+  PinnedArena hstage;      // pinned image of the staged operands and results
   std::vector<char> desc;  // host image of the last descriptors (alive until the stream syncs)
   int* hres = nullptr;     // pinned: [0] rank, [1] status
   int generation = -1;
@@ -200,6 +225,7 @@ static void release_thread_ctxs() {
     c->plan.release();
     c->stage.release();
     c->chain.release();
+    c->hstage.release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->hres) (void)hipHostFree(c->hres);
     delete c;
@@ -209,6 +235,62 @@ static void release_thread_ctxs() {
 
 template <typename T>
 static int core_kl() { return sizeof(T) == 8 ? g_ctx.core_kl_d : g_ctx.core_kl_f; }
+
+// ------------------------------------------------------------------------------
+// autotuning cache: for a tile size M, the core's precision and the core
+// variant its stacked rank selects, the variant HCORE_hip_plan_autotune found
+// fastest (the same algorithm with the other wavefront count).  Plans
+// finalized later start from it.  HCORE_TUNE_CACHE names a text file that is
+// read at start-up and rewritten after each autotuning ("M fp64 base choice"
+// per line).
+// ------------------------------------------------------------------------------
+struct TuneEntry {
+  int M, fp64, base, choice;
+};
+static std::mutex g_tune_mu;
+static std::vector<TuneEntry> g_tune;
+static bool g_tune_loaded = false;
+
+static void tune_load_locked() {
+  if (g_tune_loaded) return;
+  g_tune_loaded = true;
+  const char* path = getenv("HCORE_TUNE_CACHE");
+  if (!path) return;
+  FILE* f = fopen(path, "r");
+  if (!f) return;
+  TuneEntry e;
+  while (fscanf(f, "%d %d %d %d", &e.M, &e.fp64, &e.base, &e.choice) == 4)
+    if (e.base >= 0 && e.base < CORE_NVARIANTS && e.choice >= 0 && e.choice < CORE_NVARIANTS &&
+        (e.choice == e.base || e.choice == core_alt_variant(e.base)))
+      g_tune.push_back(e);
+  fclose(f);
+}
+
+static int tuned_variant(int M, bool fp64, int base) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  tune_load_locked();
+  for (const auto& e : g_tune)
+    if (e.M == M && e.fp64 == (int)fp64 && e.base == base) return e.choice;
+  return base;
+}
+
+static void tune_store(int M, bool fp64, int base, int choice) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  tune_load_locked();
+  bool found = false;
+  for (auto& e : g_tune)
+    if (e.M == M && e.fp64 == (int)fp64 && e.base == base) {
+      e.choice = choice;
+      found = true;
+    }
+  if (!found) g_tune.push_back({M, (int)fp64, base, choice});
+  const char* path = getenv("HCORE_TUNE_CACHE");
+  if (!path) return;
+  FILE* f = fopen(path, "w");
+  if (!f) return;
+  for (const auto& e : g_tune) fprintf(f, "%d %d %d %d\n", e.M, e.fp64, e.base, e.choice);
+  fclose(f);
+}
 
 // ------------------------------------------------------------------------------
 // host-side descriptor packing
@@ -313,6 +395,14 @@ struct hcore_hip_plan {
   int ns = 0;
   std::vector<void*> sAU, sAV, sCD;
   std::vector<int> sK;
+  // device-side ranks (HCORE_hip_plan_add_dgemm_dev / _dsyrk_dev): the host
+  // rank vectors above hold the capacities the workspace is carved for
+  bool dev_ranks = false, sdev_ranks = false;
+  std::vector<const int*> ra_dev, rb_dev, sk_dev;
+  std::vector<int*> rc_dev;
+  size_t o_patch = 0, o_tab = 0, o_cap = 0;
+  int n_patch = 0;
+  int* d_rank_in = nullptr;  // 3 per gemm update + 1 per dsyrk: ranks in use
   int ldau = 0, ldcd = 0;
   double salpha = -1;
   // device memory
@@ -330,7 +420,8 @@ struct hcore_hip_plan {
   struct CoreLaunch {
     size_t off;
     int count, variant, kmax;
-    bool dbl;  // FP64 core of a mixed-precision plan
+    bool dbl;          // FP64 core of a mixed-precision plan
+    int base_variant;  // the variant chosen by K (the autotuner may swap in its alternative)
   };
   struct Lane {
     int n = 0;
@@ -506,6 +597,26 @@ static int plan_finalize_t(hcore_hip_plan* p) {
     T* W = reinterpret_cast<T*>(wsbase);
     pk.buf.clear();
     std::vector<GemmProb<T>> gp;
+    // rank-dependent descriptor fields (device-side ranks): (vector, element,
+    // field offset) resolved to byte offsets once the vectors are packed
+    struct PendingPatch {
+      const void* vec;
+      size_t esz, idx, field;
+      int kind, upd;
+      long long base;
+    };
+    std::vector<PendingPatch> pend;
+    std::vector<std::pair<const void*, size_t>> vec_off;
+    const bool dvr = p->dev_ranks, dvs = p->sdev_ranks;
+    auto rec = [&](const auto& vec, size_t idx, size_t field, int kind, int upd, long long base = 0) {
+      pend.push_back({&vec, sizeof(vec[0]), idx, field, kind, upd, base});
+    };
+    auto addv = [&](const auto& vec) {
+      const size_t off = pk.add(vec);
+      vec_off.push_back({&vec, off});
+      return off;
+    };
+#define HCORE_GF(f) offsetof(GemmProb<T>, f)
     struct LaneDesc {
       std::vector<StackProb<T>> stack;
       TaskList tF, tG, tT, tZ, tB, tX;
@@ -523,9 +634,11 @@ static int plan_finalize_t(hcore_hip_plan* p) {
     };
     constexpr int NV = CORE_NVARIANTS;
     std::vector<LaneDesc> ld;
+    ld.reserve(3);  // built in place: patches refer to the lanes' vectors by address
     for (int l = 0; l < 3; ++l) {
       if (lane_members[l].empty()) continue;
-      LaneDesc d;
+      ld.emplace_back();
+      LaneDesc& d = ld.back();
       d.core_b.resize(NV);
       d.core_b64.resize(NV);
       d.core_kmax.assign(NV, 1);
@@ -535,6 +648,10 @@ static int plan_finalize_t(hcore_hip_plan* p) {
         T* Vs = W + o_Vs[u];
         d.stack.push_back({(const T*)p->CU[u], (const T*)p->CV[u], (const T*)p->AU[u], Us, Vs,
                            rc, ra, p->ldc, p->lda});
+        if (dvr) {
+          rec(d.stack, d.stack.size() - 1, offsetof(StackProb<T>, rc), PK_RC, u);
+          rec(d.stack, d.stack.size() - 1, offsetof(StackProb<T>, ra), PK_RA, u);
+        }
         GemmProb<T> g{};
         if (p->BD.empty()) {
           // F = AV^T BV   (hcore_dgemm.c:175-178)
@@ -542,12 +659,23 @@ static int plan_finalize_t(hcore_hip_plan* p) {
           g.m = ra; g.n = rb; g.k = M; g.lda = p->lda; g.ldb = p->ldb; g.ldc = ra;
           g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_BETA_ZERO;
           add_tasks(d.tF, (int)gp.size(), g); gp.push_back(g);
+          if (dvr) {
+            rec(gp, gp.size() - 1, HCORE_GF(m), PK_RA, u);
+            rec(gp, gp.size() - 1, HCORE_GF(n), PK_RB, u);
+            rec(gp, gp.size() - 1, HCORE_GF(ldc), PK_RA, u);
+          }
           // G = BU F^T into Vs(:, rc:K)   (hcore_dgemm.c:199-205)
           g = GemmProb<T>{};
           g.A = (const T*)p->BU[u]; g.B = W + o_F[u]; g.C = Vs + (size_t)rc * M;
           g.m = M; g.n = ra; g.k = rb; g.lda = p->ldb; g.ldb = ra; g.ldc = M;
           g.alpha = T(1); g.beta = T(0); g.flags = GF_B_TRANS | GF_BETA_ZERO;
           add_tasks(d.tG, (int)gp.size(), g); gp.push_back(g);
+          if (dvr) {
+            rec(gp, gp.size() - 1, HCORE_GF(n), PK_RA, u);
+            rec(gp, gp.size() - 1, HCORE_GF(k), PK_RB, u);
+            rec(gp, gp.size() - 1, HCORE_GF(ldb), PK_RA, u);
+            rec(gp, gp.size() - 1, HCORE_GF(C), PK_PTR_G, u, (long long)(size_t)Vs);
+          }
         } else {
           // G = B AV into Vs(:, rc:K) for a dense B   (hcore_dgemm.c:255-262)
           g.A = (const T*)p->BD[u]; g.B = (const T*)p->AV[u]; g.C = Vs + (size_t)rc * M;
@@ -562,6 +690,11 @@ static int plan_finalize_t(hcore_hip_plan* p) {
         g.alpha = T(1); g.beta = T(0);
         g.flags = GF_A_UPPER | GF_B_TRANS | GF_B_UPPER | GF_BETA_ZERO;
         add_tasks(d.tT, (int)gp.size(), g); gp.push_back(g);
+        auto recK = [&](std::initializer_list<size_t> fields) {
+          if (dvr)
+            for (size_t f : fields) rec(gp, gp.size() - 1, f, PK_K, u);
+        };
+        recK({HCORE_GF(m), HCORE_GF(n), HCORE_GF(k), HCORE_GF(ldc)});
         // Zv = T^T Zu = V_r S_r
         g = GemmProb<T>{};
         g.A = W + o_Tc[u]; g.B = W + o_Zu[u]; g.C = W + o_Zv[u];
@@ -569,6 +702,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
         g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_BETA_ZERO;
         g.n_dev = d_rank + u; g.skip = d_status + u;
         add_tasks(d.tZ, (int)gp.size(), g); gp.push_back(g);
+        recK({HCORE_GF(m), HCORE_GF(n), HCORE_GF(k), HCORE_GF(lda), HCORE_GF(ldb), HCORE_GF(ldc)});
         // register-resident QR (it also yields the Gram S and the panel factors)
         const bool qreg = lane_wy[l] && g_ctx.qr_reg && K <= kQrRegMaxK && M <= kQrRegMaxNb;
         if (lane_wy[l]) {
@@ -592,6 +726,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
               add_tasks(d.tT, (int)gp.size(), g);
               gp.push_back(g);
               d.gram = true;
+              recK({HCORE_GF(m), HCORE_GF(n), HCORE_GF(ldc)});
             }
             // B = Y1^T Z (Z = Zu with Zv's launch, Z = Zv after it)
             g = GemmProb<T>{};
@@ -600,6 +735,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
             g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_A_UNIT_LOWER | GF_BETA_ZERO;
             g.n_dev = d_rank + u; g.skip = d_status + u;
             add_tasks(side == 0 ? d.tZ : d.tB, (int)gp.size(), g); gp.push_back(g);
+            recK({HCORE_GF(m), HCORE_GF(n), HCORE_GF(k), HCORE_GF(ldb), HCORE_GF(ldc)});
             // X = [Z; 0] - Y M into the caller's factor
             g = GemmProb<T>{};
             g.A = Ys[side]; g.B = Ms[side]; g.C = Xo[side];
@@ -608,22 +744,33 @@ static int plan_finalize_t(hcore_hip_plan* p) {
             g.C0 = Zs[side]; g.ldc0 = K; g.m0 = K;
             g.n_dev = d_rank + u; g.skip = d_status + u;
             add_tasks(d.tX, (int)gp.size(), g); gp.push_back(g);
+            recK({HCORE_GF(n), HCORE_GF(k), HCORE_GF(ldb), HCORE_GF(ldc0), HCORE_GF(m0)});
             d.wy.push_back({Ss[side], side == 0 ? W + o_TA[u] : W + o_TB[u], Ms[side], d_rank + u,
                             d_status + u, K});
+            if (dvr) rec(d.wy, d.wy.size() - 1, offsetof(WySolveProb<T>, K), PK_K, u);
           }
         }
         // QR of both stacks
+        auto recQ = [&](const std::vector<QrProb<T>>& q) {
+          if (dvr) {
+            rec(q, q.size() - 2, offsetof(QrProb<T>, K), PK_K, u);
+            rec(q, q.size() - 1, offsetof(QrProb<T>, K), PK_K, u);
+          }
+        };
         if (qreg) {  // one register-resident pass (also yields S)
           T* SA = W + o_S[u];
           auto& q = K <= 16 ? d.qrr16 : d.qrr;
           q.push_back({Us, W + o_tA[u], W + o_TA[u], SA, K});
           q.push_back({Vs, W + o_tB[u], W + o_TB[u], SA + al((size_t)K * K), K});
+          recQ(q);
         } else if (g_ctx.qr_reg && M <= kQrRegMaxNb) {  // register panels + in-place trailing updates
           d.qrp.push_back({Us, W + o_tA[u], W + o_TA[u], nullptr, K});
           d.qrp.push_back({Vs, W + o_tB[u], W + o_TB[u], nullptr, K});
+          recQ(d.qrp);
         } else {  // blocked Householder QR with LDS panels
           d.qr.push_back({Us, W + o_tA[u], W + o_TA[u], nullptr, K});
           d.qr.push_back({Vs, W + o_tB[u], W + o_TB[u], nullptr, K});
+          recQ(d.qr);
         }
         const int b = variant[u];
         if (mixed) {
@@ -638,11 +785,17 @@ static int plan_finalize_t(hcore_hip_plan* p) {
           d.cv_in.push_back({W + o_Tc[u], Tc64, K * K});
           d.cv_out.push_back({Zu64, W + o_Zu[u], K * K});
           d.cv_max = std::max(d.cv_max, K * K);
+          if (dvr) {
+            rec(d.core_b64[b], d.core_b64[b].size() - 1, offsetof(CoreProb<double>, K), PK_K, u);
+            rec(d.cv_in, d.cv_in.size() - 1, offsetof(ConvProb, n), PK_KK, u);
+            rec(d.cv_out, d.cv_out.size() - 1, offsetof(ConvProb, n), PK_KK, u);
+          }
         } else {
           CoreProb<T> c{W + o_Tc[u], global_core(variant[u]) ? W + o_Tj[u] : nullptr, W + o_Zu[u],
                         W + o_sg[u], d_rank + u, d_status + u, d_sweeps + u,
                         d_clk ? d_clk + (size_t)u * kCoreClk : nullptr, K};
           d.core_b[b].push_back(c);
+          if (dvr) rec(d.core_b[b], d.core_b[b].size() - 1, offsetof(CoreProb<T>, K), PK_K, u);
         }
         d.core_kmax[b] = std::max(d.core_kmax[b], K);
         T* CUo = (T*)(p->CUo[u] ? p->CUo[u] : p->CU[u]);
@@ -650,11 +803,14 @@ static int plan_finalize_t(hcore_hip_plan* p) {
         if (!lane_wy[l]) {
           d.ap.push_back({Us, W + o_TA[u], W + o_Zu[u], CUo, d_rank + u, d_status + u, K, p->ldco});
           d.ap.push_back({Vs, W + o_TB[u], W + o_Zv[u], CVo, d_rank + u, d_status + u, K, p->ldco});
+          if (dvr) {
+            rec(d.ap, d.ap.size() - 2, offsetof(ApplyProb<T>, K), PK_K, u);
+            rec(d.ap, d.ap.size() - 1, offsetof(ApplyProb<T>, K), PK_K, u);
+          }
         }
         d.kmax = std::max(d.kmax, K);
       }
       d.wy_lane = lane_wy[l];
-      ld.push_back(std::move(d));
     }
     // syrk (hcore_dsyrk.c:103, 144, 165-181)
     std::vector<GemmProb<T>> sp;
@@ -666,49 +822,58 @@ static int plan_finalize_t(hcore_hip_plan* p) {
       g.m = K; g.n = K; g.k = p->ldau; g.lda = p->ldau; g.ldb = p->ldau; g.ldc = K;
       g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_BETA_ZERO;
       add_tasks(tW, (int)sp.size(), g); sp.push_back(g);
+      auto recS = [&](std::initializer_list<size_t> fields) {
+        if (dvs)
+          for (size_t f : fields) rec(sp, sp.size() - 1, f, PK_SK, s);
+      };
+      recS({HCORE_GF(m), HCORE_GF(n), HCORE_GF(ldc)});
       g = GemmProb<T>{};
       g.A = (const T*)p->sAU[s]; g.B = W + o_W[s]; g.C = W + o_X[s];
       g.m = M; g.n = K; g.k = K; g.lda = p->ldau; g.ldb = K; g.ldc = M;
       g.alpha = T(1); g.beta = T(0); g.flags = GF_BETA_ZERO;
       add_tasks(tX, (int)sp.size(), g); sp.push_back(g);
+      recS({HCORE_GF(n), HCORE_GF(k), HCORE_GF(ldb)});
       g = GemmProb<T>{};
       g.A = W + o_X[s]; g.B = (const T*)p->sAU[s]; g.C = (T*)p->sCD[s];
       g.m = M; g.n = M; g.k = K; g.lda = M; g.ldb = p->ldau; g.ldc = p->ldcd;
       g.alpha = T(p->salpha); g.beta = T(1); g.flags = GF_B_TRANS | GF_LOWER_ZERO_UPPER;
       add_tasks(tC, (int)sp.size(), g); sp.push_back(g);
+      recS({HCORE_GF(k)});
     }
-    p->o_gp = pk.add(gp);
+    p->o_gp = addv(gp);
     if (p->lanes.size() != ld.size()) p->lanes.resize(ld.size());
     for (size_t l = 0; l < ld.size(); ++l) {
       auto& L = p->lanes[l];
       auto& d = ld[l];
       L.n = (int)d.stack.size();
-      L.o_stack = pk.add(d.stack);
+      L.o_stack = addv(d.stack);
       L.stack_cols = 0;
       for (const auto& sp_ : d.stack) L.stack_cols = std::max(L.stack_cols, 2 * sp_.rc + sp_.ra);
       L.tF = pk.add(d.tF);
       L.tG = pk.add(d.tG);
       L.tT = pk.add(d.tT);
       L.tZ = pk.add(d.tZ);
-      L.o_qr = pk.add(d.qr); L.n_qr = (int)d.qr.size();
+      L.o_qr = addv(d.qr); L.n_qr = (int)d.qr.size();
       L.gram = d.gram;
-      L.o_qrr16 = pk.add(d.qrr16); L.n_qrr16 = (int)d.qrr16.size();
-      L.o_qrr = pk.add(d.qrr); L.n_qrr = (int)d.qrr.size();
-      L.o_qrp = pk.add(d.qrp); L.n_qrp = (int)d.qrp.size();
-      L.o_apply = pk.add(d.ap); L.n_apply = (int)d.ap.size();
+      L.o_qrr16 = addv(d.qrr16); L.n_qrr16 = (int)d.qrr16.size();
+      L.o_qrr = addv(d.qrr); L.n_qrr = (int)d.qrr.size();
+      L.o_qrp = addv(d.qrp); L.n_qrp = (int)d.qrp.size();
+      L.o_apply = addv(d.ap); L.n_apply = (int)d.ap.size();
       L.tB = pk.add(d.tB);
       L.tX = pk.add(d.tX);
-      L.o_wy = pk.add(d.wy); L.n_wy = (int)d.wy.size();
+      L.o_wy = addv(d.wy); L.n_wy = (int)d.wy.size();
       L.kmax = d.kmax;
       L.wy = d.wy_lane;
       L.core.clear();
       for (int b = 0; b < NV; ++b) {
         if (!d.core_b[b].empty())
-          L.core.push_back({pk.add(d.core_b[b]), (int)d.core_b[b].size(), b, d.core_kmax[b], false});
+          L.core.push_back({addv(d.core_b[b]), (int)d.core_b[b].size(), tuned_variant(M, false, b),
+                            d.core_kmax[b], false, b});
         if (!d.core_b64[b].empty())
-          L.core.push_back({pk.add(d.core_b64[b]), (int)d.core_b64[b].size(), b, d.core_kmax[b], true});
+          L.core.push_back({addv(d.core_b64[b]), (int)d.core_b64[b].size(),
+                            tuned_variant(M, true, b), d.core_kmax[b], true, b});
       }
-      L.o_cvin = pk.add(d.cv_in); L.o_cvout = pk.add(d.cv_out);
+      L.o_cvin = addv(d.cv_in); L.o_cvout = addv(d.cv_out);
       L.n_cv = (int)d.cv_in.size(); L.cv_max = d.cv_max;
       // largest K first: the longest jobs start first
       std::stable_sort(L.core.begin(), L.core.end(),
@@ -716,13 +881,47 @@ static int plan_finalize_t(hcore_hip_plan* p) {
                          return x.kmax > y.kmax;
                        });
     }
-    p->o_sp = pk.add(sp);
+    p->o_sp = addv(sp);
     p->tW = pk.add(tW);
     p->tXs = pk.add(tX);
     p->tC = pk.add(tC);
+#undef HCORE_GF
+    // the patch list and the device rank tables
+    std::vector<RankPatch> patches;
+    for (const auto& q : pend) {
+      size_t base_off = 0;
+      for (const auto& vo : vec_off)
+        if (vo.first == q.vec) base_off = vo.second;
+      patches.push_back({(unsigned long long)(base_off + q.idx * q.esz + q.field), q.base, q.kind,
+                         q.upd});
+    }
+    p->o_patch = pk.add(patches);
+    p->n_patch = (int)patches.size();
+    std::vector<const int*> tabs;
+    std::vector<int> caps;
+    if (dvr) {
+      for (int u = 0; u < n; ++u) tabs.push_back(p->ra_dev[u]);
+      for (int u = 0; u < n; ++u) tabs.push_back(p->rb_dev[u]);
+      for (int u = 0; u < n; ++u) tabs.push_back(p->rc_dev[u]);
+      for (int u = 0; u < n; ++u) {
+        caps.push_back(p->ra[u]);
+        caps.push_back(p->rb[u]);
+        caps.push_back(p->rc[u]);
+      }
+    }
+    if (dvs) {
+      if (!dvr) caps.assign(3 * (size_t)n, 0);
+      for (int v = 0; v < p->ns; ++v) tabs.push_back(p->sk_dev[v]);
+      for (int v = 0; v < p->ns; ++v) caps.push_back(p->sK[v]);
+    }
+    p->o_tab = pk.add(tabs);
+    p->o_cap = pk.add(caps);
   };
   build(nullptr, nullptr, nullptr);  // dry run: descriptor size
-  const size_t head_i = ((size_t)std::max(n, 1) * 3 * sizeof(int) + 255) & ~size_t(255);
+  // [rank | status | sweeps | ranks in use (3 per update, 1 per dsyrk)] ints
+  const size_t n_in = 3 * (size_t)std::max(n, 1) + (size_t)p->ns;
+  const size_t head_i =
+      (((size_t)std::max(n, 1) * 3 + n_in) * sizeof(int) + 255) & ~size_t(255);
   const size_t head = head_i + (((size_t)std::max(n, 1) * kCoreClk * sizeof(long long) + 255) &
                                 ~size_t(255));
   const size_t wsb = (ws_elems * es + 255) & ~size_t(255);
@@ -749,6 +948,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
   p->d_rank = reinterpret_cast<int*>(base);
   p->d_status = p->d_rank + std::max(n, 1);
   p->d_sweeps = p->d_status + std::max(n, 1);
+  p->d_rank_in = p->d_sweeps + std::max(n, 1);
   d_sweeps = p->d_sweeps;
   p->d_clk = reinterpret_cast<long long*>(base + head_i);
   d_clk = p->d_clk;
@@ -848,6 +1048,30 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
     phase(lt, 10, ss, true, [&] { gemm(sp, p->tXs, ss); });
     phase(lt, 11, ss, true, [&] { gemm(sp, p->tC, ss); });
   };
+  // per-execution bookkeeping of batched plans: statuses reset and, with
+  // device-side ranks, the ranks read and the descriptors patched
+  const bool batched = !(p->arena || p->fixed);
+  RankTables rt{};
+  if (batched) {
+    const int* const* tab = reinterpret_cast<const int* const*>(D(p->o_tab));
+    rt.ra = p->dev_ranks ? tab : nullptr;
+    rt.rb = p->dev_ranks ? tab + p->ng : nullptr;
+    rt.rc = p->dev_ranks ? const_cast<int* const*>(tab + 2 * p->ng) : nullptr;
+    rt.sk = p->sdev_ranks ? tab + (p->dev_ranks ? 3 * p->ng : 0) : nullptr;
+    rt.cap = reinterpret_cast<const int*>(D(p->o_cap));
+    rt.rank_in = p->d_rank_in;
+    rt.status = p->d_status;
+    rt.rank_out = p->d_rank;
+    rt.n = p->ng;
+    rt.ns = p->sdev_ranks ? p->ns : 0;
+    rt.M = M;
+    rt.elem = (int)sizeof(T);
+    if (part != 2) {
+      launch_rank_gather(rt, s);
+      launch_rank_patch(reinterpret_cast<const RankPatch*>(D(p->o_patch)), p->n_patch, p->desc,
+                        p->d_rank_in, p->ng, M, (int)sizeof(T), s);
+    }
+  }
   if (p->ev_start) (void)hipEventRecord(p->ev_start, s);
   for (int l = 0; l < nl; ++l) {
     auto& L = p->lanes[l];
@@ -912,6 +1136,7 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
   if (nl == 0) syrk(s);
   if (!(tm && p->serial))
     for (int l = 1; l < nl; ++l) (void)hipStreamWaitEvent(s, p->lanes[l].ev_done, 0);
+  if (batched && p->dev_ranks && part != 1) launch_rank_publish(rt, s);
   if (tm) p->pending[k] = true;
   HCHECK(hipGetLastError());
   return HCORE_HIP_OK;
@@ -1059,20 +1284,32 @@ static void single_gemm(int M, T alpha, T* AU, T* AV, T* Ark, int LDA, T* BU, T*
   // host operands are staged in the thread's staging arena (M rows, packed)
   T *dAU = AU, *dAV = AV, *dBU = BU, *dBV = BV, *dCU = CU, *dCV = CV;
   T *oCU = nullptr, *oCV = nullptr;
+  const size_t nA = (size_t)M * ra, nB = (size_t)M * rb, nC = (size_t)M * K;
   if (!dev) {
-    const size_t nA = (size_t)M * ra, nB = (size_t)M * rb, nC = (size_t)M * K;
-    const size_t bytes = sizeof(T) * (2 * nA + 2 * nB + 4 * nC) + 1024;
-    if (mem.tc->stage.reserve(bytes) != HCORE_HIP_OK)
+    // the six factors packed (ld M) into the pinned image, then one transfer;
+    // the new factors come back through the same image
+    const size_t in_elems = 2 * nA + 2 * nB + 2 * nC;
+    const size_t bytes = sizeof(T) * (in_elems + 2 * nC) + 1024;
+    if (mem.tc->stage.reserve(bytes) != HCORE_HIP_OK || mem.tc->hstage.reserve(bytes) != HCORE_HIP_OK)
       return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: staging allocation failed");
     T* q = reinterpret_cast<T*>(mem.tc->stage.p);
     dAU = q; q += nA; dAV = q; q += nA; dBU = q; q += nB; dBV = q; q += nB;
     dCU = q; q += nC; dCV = q; q += nC; oCU = q; q += nC; oCV = q;
-    auto up = [&](T* d, const T* h, int ld, int cols) {
-      return hipMemcpy2DAsync(d, sizeof(T) * M, h, sizeof(T) * ld, sizeof(T) * M, cols,
-                              hipMemcpyHostToDevice, s);
+    T* h = reinterpret_cast<T*>(mem.tc->hstage.p);
+    auto pack = [&](T* dst, const T* src, int ld, int cols, int cap) {
+      if (ld == M) memcpy(dst, src, sizeof(T) * (size_t)M * cols);
+      else
+        for (int c = 0; c < cols; ++c) memcpy(dst + (size_t)c * M, src + (size_t)c * ld, sizeof(T) * M);
+      (void)cap;
     };
-    if (up(dAU, AU, LDA, ra) || up(dAV, AV, LDA, ra) || up(dBU, BU, LDB, rb) ||
-        up(dBV, BV, LDB, rb) || up(dCU, CU, LDC, rc) || up(dCV, CV, LDC, rc))
+    pack(h, AU, LDA, ra, ra);
+    pack(h + nA, AV, LDA, ra, ra);
+    pack(h + 2 * nA, BU, LDB, rb, rb);
+    pack(h + 2 * nA + nB, BV, LDB, rb, rb);
+    pack(h + 2 * nA + 2 * nB, CU, LDC, rc, K);
+    pack(h + 2 * nA + 2 * nB + nC, CV, LDC, rc, K);
+    // columns rc..K-1 of the C blocks are not read; the whole image goes in one copy
+    if (hipMemcpyAsync(dAU, h, sizeof(T) * in_elems, hipMemcpyHostToDevice, s) != hipSuccess)
       return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: host-to-device staging failed");
     p.lda = M; p.ldb = M; p.ldc = M; p.ldco = M;
     p.CUo = {oCU}; p.CVo = {oCV};
@@ -1095,11 +1332,16 @@ static void single_gemm(int M, T alpha, T* AU, T* AV, T* Ark, int LDA, T* BU, T*
   }
   if (st == HCORE_HIP_OK) st = plan_execute_t<T>(&p, s);
   int* hres = mem.hres;
+  T* hout = dev ? nullptr : reinterpret_cast<T*>(mem.tc->hstage.p) + 2 * nA + 2 * nB + 2 * nC;
   if (st == HCORE_HIP_OK) {
     if (hipMemcpyAsync(hres, p.d_rank, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipMemcpyAsync(hres + 1, p.d_status, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
+        hipMemcpyAsync(hres + 1, p.d_status, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess)
       st = HCORE_HIP_ERR_RUNTIME;
+    // the new factors (at most K columns each) come back with the rank, in one sync
+    if (st == HCORE_HIP_OK && !dev &&
+        hipMemcpyAsync(hout, oCU, sizeof(T) * 2 * nC, hipMemcpyDeviceToHost, s) != hipSuccess)
+      st = HCORE_HIP_ERR_RUNTIME;
+    if (st == HCORE_HIP_OK && hipStreamSynchronize(s) != hipSuccess) st = HCORE_HIP_ERR_RUNTIME;
   }
   if (st != HCORE_HIP_OK) {
     (void)hipStreamSynchronize(s);
@@ -1115,13 +1357,11 @@ static void single_gemm(int M, T alpha, T* AU, T* AV, T* Ark, int LDA, T* BU, T*
     snprintf(msg, sizeof msg, "%s: ERROR in the core SVD (non-finite values), K=%d", name, K);
     return fail(HCORE_HIP_ERR_NUMERIC, msg);
   }
-  if (!dev) {
-    hipError_t e1 = hipMemcpy2DAsync(CU, sizeof(T) * LDC, oCU, sizeof(T) * M, sizeof(T) * M, r,
-                                     hipMemcpyDeviceToHost, s);
-    hipError_t e2 = hipMemcpy2DAsync(CV, sizeof(T) * LDC, oCV, sizeof(T) * M, sizeof(T) * M, r,
-                                     hipMemcpyDeviceToHost, s);
-    hipError_t e3 = hipStreamSynchronize(s);
-    if (e1 || e2 || e3) return fail(HCORE_HIP_ERR_RUNTIME, "hcore_amd: device-to-host copy failed");
+  if (!dev) {  // oCV follows oCU in the staging arena: hout holds both
+    for (int c = 0; c < r; ++c) {
+      memcpy(CU + (size_t)c * LDC, hout + (size_t)c * M, sizeof(T) * M);
+      memcpy(CV + (size_t)c * LDC, hout + nC + (size_t)c * M, sizeof(T) * M);
+    }
   }
   if (is_device_ptr(Crk)) {
     T rv = (T)r;
@@ -2122,6 +2362,39 @@ int HCORE_hip_plan_add_dgemm(hcore_hip_plan* p, int n, void* const* AU, void* co
   return HCORE_HIP_OK;
 }
 
+int HCORE_hip_plan_add_dgemm_dev(hcore_hip_plan* p, int n, void* const* AU, void* const* AV,
+                                 const int* const* Ark, const int* Ark_cap, int LDA,
+                                 void* const* BU, void* const* BV, const int* const* Brk,
+                                 const int* Brk_cap, int LDB, void* const* CU, void* const* CV,
+                                 int* const* Crk, const int* Crk_cap, int LDC, void* const* CUout,
+                                 void* const* CVout, int LDCout, double alpha, double beta, int rk,
+                                 int maxrk, double acc) {
+  if (!p || n < 0 || (n > 0 && (!Ark || !Brk || !Crk))) return HCORE_HIP_ERR_ARG;
+  for (int u = 0; u < n; ++u)
+    if (!Ark[u] || !Brk[u] || !Crk[u]) return HCORE_HIP_ERR_ARG;
+  int st = HCORE_hip_plan_add_dgemm(p, n, AU, AV, Ark_cap, LDA, BU, BV, Brk_cap, LDB, CU, CV,
+                                    Crk_cap, LDC, CUout, CVout, LDCout, alpha, beta, rk, maxrk, acc);
+  if (st) return st;
+  p->dev_ranks = true;
+  p->ra_dev.assign(Ark, Ark + n);
+  p->rb_dev.assign(Brk, Brk + n);
+  p->rc_dev.assign(Crk, Crk + n);
+  return HCORE_HIP_OK;
+}
+
+int HCORE_hip_plan_add_dsyrk_dev(hcore_hip_plan* p, int n, void* const* AU, void* const* AV,
+                                 const int* const* K, const int* K_cap, int LDAU,
+                                 void* const* CD, int LDCD, double alpha) {
+  if (!p || n < 0 || (n > 0 && !K)) return HCORE_HIP_ERR_ARG;
+  for (int u = 0; u < n; ++u)
+    if (!K[u]) return HCORE_HIP_ERR_ARG;
+  int st = HCORE_hip_plan_add_dsyrk(p, n, AU, AV, K_cap, LDAU, CD, LDCD, alpha);
+  if (st) return st;
+  p->sdev_ranks = true;
+  p->sk_dev.assign(K, K + n);
+  return HCORE_HIP_OK;
+}
+
 int HCORE_hip_plan_add_dsyrk(hcore_hip_plan* p, int n, void* const* AU, void* const* AV,
                              const int* K, int LDAU, void* const* CD, int LDCD, double alpha) {
   if (!p || p->finalized || p->ns > 0 || n < 0) return HCORE_HIP_ERR_ARG;
@@ -2170,6 +2443,82 @@ int HCORE_hip_plan_execute_graph(hcore_hip_plan* p, void* stream) {
   return HCORE_HIP_OK;
 }
 
+}  // extern "C"
+
+namespace hcore {
+
+// Time every register-resident core launch of an executed plan with its
+// variant and with the alternative wavefront count; keep the faster.  The
+// core only reads T (kept intact) and rewrites Zu, sigma, rank and status, so
+// the re-runs leave the plan's results as they were (rank, status and sweep
+// words are saved and restored).
+template <typename T>
+static int plan_autotune_t(hcore_hip_plan* p, hipStream_t s, int reps, double* report, int nrep) {
+  const int n = std::max(p->ng, 1);
+  int* save = nullptr;
+  HCHECK(hipDeviceSynchronize());
+  HCHECK(hipMalloc(&save, sizeof(int) * 3 * n));
+  HCHECK(hipMemcpy(save, p->d_rank, sizeof(int) * 3 * n, hipMemcpyDeviceToDevice));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HCHECK(hipEventCreate(&e0));
+  HCHECK(hipEventCreate(&e1));
+  int nr = 0;
+  for (size_t l = 0; l < p->lanes.size(); ++l) {
+    for (auto& c : p->lanes[l].core) {
+      const int alt = core_alt_variant(c.base_variant);
+      if (alt < 0) continue;
+      const int cand[2] = {c.base_variant, alt};
+      float best[2] = {1e30f, 1e30f};
+      for (int r = 0; r < reps; ++r)
+        for (int k = 0; k < 2; ++k) {
+          HCHECK(hipEventRecord(e0, s));
+          if (c.dbl)
+            launch_core<double>(reinterpret_cast<const CoreProb<double>*>(p->desc + c.off), c.count,
+                                cand[k], c.kmax, p->rk, p->maxrk, p->acc, s);
+          else
+            launch_core<T>(reinterpret_cast<const CoreProb<T>*>(p->desc + c.off), c.count, cand[k],
+                           c.kmax, p->rk, p->maxrk, T(p->acc), s);
+          HCHECK(hipEventRecord(e1, s));
+          HCHECK(hipEventSynchronize(e1));
+          float ms = 0.f;
+          HCHECK(hipEventElapsedTime(&ms, e0, e1));
+          best[k] = std::min(best[k], ms);
+        }
+      c.variant = best[1] < best[0] ? alt : c.base_variant;
+      tune_store(p->M, c.dbl || sizeof(T) == 8, c.base_variant, c.variant);
+      if (report && nr < nrep) {
+        double* q = report + 6 * nr;
+        q[0] = (double)l; q[1] = c.base_variant; q[2] = c.variant; q[3] = best[0]; q[4] = best[1];
+        q[5] = c.count;
+      }
+      ++nr;
+    }
+  }
+  HCHECK(hipMemcpy(p->d_rank, save, sizeof(int) * 3 * n, hipMemcpyDeviceToDevice));
+  HCHECK(hipDeviceSynchronize());
+  (void)hipFree(save);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // a captured graph holds the old launches: capture again on the next replay
+  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  p->gexec = nullptr;
+  p->graph = nullptr;
+  return nr;
+}
+
+}  // namespace hcore
+
+extern "C" {
+
+int HCORE_hip_plan_autotune(hcore_hip_plan* p, void* stream, int reps, double* report, int nrep) {
+  if (!p || !p->finalized || p->arena || p->fixed || reps < 1) return -HCORE_HIP_ERR_ARG;
+  const int r = p->prec == HCORE_HIP_FP64
+                    ? plan_autotune_t<double>(p, (hipStream_t)stream, reps, report, nrep)
+                    : plan_autotune_t<float>(p, (hipStream_t)stream, reps, report, nrep);
+  return r;
+}
+
 const int* HCORE_hip_plan_ranks_device(const hcore_hip_plan* p) { return p ? p->d_rank : nullptr; }
 
 int HCORE_hip_plan_get_ranks(hcore_hip_plan* p, int* out) {
@@ -2215,9 +2564,29 @@ int HCORE_hip_plan_get_core_profile(hcore_hip_plan* p, long long* out, double* t
   return HCORE_HIP_OK;
 }
 
+// The ranks in use by the last execution (device-side ranks) or the host ranks.
+static int plan_ranks_in(hcore_hip_plan* p, std::vector<int>& r3) {
+  r3.resize(3 * (size_t)p->ng);
+  if (p->dev_ranks) {
+    HCHECK(hipDeviceSynchronize());
+    if (p->ng)
+      HCHECK(hipMemcpy(r3.data(), p->d_rank_in, sizeof(int) * 3 * p->ng, hipMemcpyDeviceToHost));
+  } else {
+    for (int u = 0; u < p->ng; ++u) {
+      r3[3 * u] = p->ra[u];
+      r3[3 * u + 1] = p->rb[u];
+      r3[3 * u + 2] = p->rc[u];
+    }
+  }
+  return HCORE_HIP_OK;
+}
+
 int HCORE_hip_plan_get_sigma(hcore_hip_plan* p, int u, double* out, int* K_out) {
   if (!p || !p->finalized || u < 0 || u >= p->ng) return HCORE_HIP_ERR_ARG;
-  const int K = p->ra[u] + p->rc[u];
+  std::vector<int> r3;
+  int st = plan_ranks_in(p, r3);
+  if (st) return st;
+  const int K = r3[3 * u] + r3[3 * u + 2];
   if (K_out) *K_out = K;
   if (!out) return HCORE_HIP_OK;
   HCHECK(hipDeviceSynchronize());
@@ -2242,8 +2611,11 @@ int HCORE_hip_plan_flops(hcore_hip_plan* p, unsigned long* update, unsigned long
   if ((int)p->last_ranks.size() != p->ng) return HCORE_HIP_ERR_ARG;
   unsigned long fu = 0, fs = 0;
   const unsigned long M = p->M;
+  std::vector<int> r3;
+  int st = plan_ranks_in(p, r3);
+  if (st) return st;
   for (int u = 0; u < p->ng; ++u) {
-    const unsigned long ra = p->ra[u], rb = p->rb[u], K = p->ra[u] + p->rc[u];
+    const unsigned long ra = r3[3 * u], rb = r3[3 * u + 1], K = r3[3 * u] + r3[3 * u + 2];
     const unsigned long r = p->last_ranks[u];
     fu += flop_counts('q', M, K, 0, 0) * 2 + flop_counts('m', ra, rb, M, 0) +
           flop_counts('m', M, ra, rb, 0) + flop_counts('s', K, 0, 0, 0) + r * K +

@@ -18,6 +18,7 @@ enum : int {
   ST_OK = 0,
   ST_R_GT_MAXRK = 4,  // truncated rank above maxrk (reference hcore_dgemm.c:465-468)
   ST_NONFINITE = 5,   // NaN/Inf reached the core SVD (the reference's LAPACK info!=0 class)
+  ST_BAD_RANK = 7,    // a device-side input rank outside 1..its capacity (HCORE_HIP_ERR_ARG)
 };
 
 // ---- generic batched GEMM on 64x64 output tiles ------------------------------
@@ -179,9 +180,15 @@ bool launch_qr_fused(const QrProb<T>* probs, int n, int nb, hipStream_t s);
 // (LDS/global variants), then the register-resident variants (k_core_reg)
 enum : int {
   CORE_K16, CORE_K32, CORE_K64, CORE_K128, CORE_KLDS, CORE_K192, CORE_K256, CORE_K512,
-  CORE_R64, CORE_R128, CORE_R192, CORE_NVARIANTS
+  CORE_R64, CORE_R128, CORE_R192,
+  // the register-resident variants with twice the wavefronts per tile (half
+  // the Jacobi block width): more parallelism per tile, more registers per CU
+  CORE_R64W, CORE_R128W, CORE_R192W, CORE_NVARIANTS
 };
 int core_variant_for(int K, int lds_max_k, bool reg);
+// The other wavefront count of a register-resident variant (CORE_R64 <->
+// CORE_R64W, ...), or -1: the alternative an autotuned plan may pick.
+int core_alt_variant(int variant);
 // the register-resident variants (CORE_R64 .. CORE_R192); launch_core forwards to it
 template <typename T>
 void launch_core_reg(const CoreProb<T>* probs, int n, int variant, int rk, int maxrk, T acc,
@@ -216,6 +223,42 @@ void dgemm_fixed_workspace(int M, double alpha, double* AU, double* AV, double* 
                            hipStream_t s);
 // Print msg, record status for HCORE_hip_last_status and apply the error policy.
 void report_failure(int status, const char* msg);
+
+// ---- device-side ranks of a plan (k_plan.hip) --------------------------------
+// Plans whose input ranks live in device memory keep every rank-dependent
+// descriptor field in a patch list; at the start of each execution the
+// ranks are read (and checked against the capacities the workspace was
+// carved for), then every patch rewrites its field, so one captured graph
+// serves successive steps whose ranks changed on the device.
+enum : int {
+  PK_RA,     // int field = rank of A
+  PK_RB,     // int field = rank of B
+  PK_RC,     // int field = rank of C
+  PK_K,      // int field = ra + rc (stacked rank)
+  PK_KK,     // int field = K * K
+  PK_PTR_G,  // pointer field = base + rc * M elements (G's place in the V stack)
+  PK_SK,     // int field = rank of a dsyrk update
+};
+struct RankPatch {
+  unsigned long long off;  // byte offset of the field in the descriptor buffer
+  long long base;          // PK_PTR_G: device address of the V stack
+  int kind, upd;
+};
+struct RankTables {
+  const int* const* ra;  // per gemm update: device address of its rank of A (null: host ranks)
+  const int* const* rb;
+  int* const* rc;        // also receives the new rank after a successful update
+  const int* const* sk;  // per dsyrk update
+  const int* cap;        // 3 per gemm update (ra, rb, rc capacities), then 1 per dsyrk
+  int* rank_in;          // out: 3 per gemm update, then 1 per dsyrk (the ranks in use)
+  int* status;           // per gemm update: reset to ST_OK, or ST_BAD_RANK
+  const int* rank_out;   // per gemm update: the new rank (publish)
+  int n, ns, M, elem;
+};
+void launch_rank_gather(const RankTables& t, hipStream_t s);
+void launch_rank_patch(const RankPatch* patches, int np, char* desc, const int* rank_in, int n,
+                       int M, int elem, hipStream_t s);
+void launch_rank_publish(const RankTables& t, hipStream_t s);
 
 // ---- dense triangular kernels (k_dense.hip) ----------------------------------
 // One diagonal block of a triangular solve: for every right-hand side c <

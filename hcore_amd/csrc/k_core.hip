@@ -303,7 +303,7 @@ __device__ void core_epilogue(const CoreProb<T>& P, const T* W, int ld, int k, c
   for (int i = tid; i < K; i += NT) P.sigma[i] = sig[perm[i]];
   if (tid == 0) {
     *P.rank = r;
-    *P.status = r > maxrk ? ST_R_GT_MAXRK : ST_OK;
+    if (*P.status == ST_OK) *P.status = r > maxrk ? ST_R_GT_MAXRK : ST_OK;  // keep a preset error
   }
   if (tid == 0) flags[0] = 0;
   block_sync<NT>();
@@ -436,6 +436,18 @@ void launch_core(const CoreProb<T>* probs, int n, int variant, int kmax, int rk,
     HCORE_CORE_VARIANTS(HCORE_LAUNCH)
 #undef HCORE_LAUNCH
     default: break;
+  }
+}
+
+int core_alt_variant(int v) {
+  switch (v) {
+    case CORE_R64: return CORE_R64W;
+    case CORE_R128: return CORE_R128W;
+    case CORE_R192: return CORE_R192W;
+    case CORE_R64W: return CORE_R64;
+    case CORE_R128W: return CORE_R128;
+    case CORE_R192W: return CORE_R192;
+    default: return -1;
   }
 }
 

@@ -624,7 +624,7 @@ __global__ __launch_bounds__(64 * W) void k_core_reg(const CoreProb<T>* __restri
   for (int i = tid; i < K; i += NT) P.sigma[i] = sig[perm[i]];
   if (tid == 0) {
     *P.rank = r;
-    *P.status = r > maxrk ? ST_R_GT_MAXRK : ST_OK;
+    if (*P.status == ST_OK) *P.status = r > maxrk ? ST_R_GT_MAXRK : ST_OK;  // keep a preset error
   }
   if (tid == 0) flags[2] = 0;
   __syncthreads();
@@ -660,7 +660,10 @@ __global__ __launch_bounds__(64 * W) void k_core_reg(const CoreProb<T>* __restri
 #define HCORE_CORE_REG_VARIANTS(X) \
   X(CORE_R64, 4, 1, 16, 8)         \
   X(CORE_R128, 8, 2, 16, 8)        \
-  X(CORE_R192, 8, 3, 24, 12)
+  X(CORE_R192, 8, 3, 24, 12)       \
+  X(CORE_R64W, 8, 1, 8, 4)         \
+  X(CORE_R128W, 16, 2, 8, 4)       \
+  X(CORE_R192W, 16, 3, 12, 6)
 
 template <typename T>
 void launch_core_reg(const CoreProb<T>* probs, int n, int variant, int rk, int maxrk, T acc,

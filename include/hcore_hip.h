@@ -81,6 +81,31 @@ int HCORE_hip_plan_add_dsyrk(hcore_hip_plan *plan, int n,
                              void *const *AU, void *const *AV, const int *K, int LDAU,
                              void *const *CD, int LDCD, double alpha);
 
+/*
+ * The same, with the input ranks in DEVICE memory, read when the plan
+ * executes: Ark/Brk/Crk are host arrays of n device addresses of int ranks,
+ * and Ark_cap/Brk_cap/Crk_cap host int upper bounds that size the workspace
+ * and choose the lanes.  After every execution the new rank of each update
+ * that succeeded replaces *Crk[u] on the device, so a runtime that keeps its
+ * tile ranks on the GPU (and their factors at fixed addresses) replays one
+ * captured graph for every step.  A rank outside 1..capacity gives the update
+ * status HCORE_HIP_ERR_ARG and leaves its factors and rank untouched.
+ */
+int HCORE_hip_plan_add_dgemm_dev(hcore_hip_plan *plan, int n,
+                                 void *const *AU, void *const *AV, const int *const *Ark,
+                                 const int *Ark_cap, int LDA,
+                                 void *const *BU, void *const *BV, const int *const *Brk,
+                                 const int *Brk_cap, int LDB,
+                                 void *const *CU, void *const *CV, int *const *Crk,
+                                 const int *Crk_cap, int LDC,
+                                 void *const *CUout, void *const *CVout, int LDCout,
+                                 double alpha, double beta, int rk, int maxrk, double acc);
+/* dsyrk updates with device-side ranks K[u] (device addresses) bounded by K_cap. */
+int HCORE_hip_plan_add_dsyrk_dev(hcore_hip_plan *plan, int n,
+                                 void *const *AU, void *const *AV, const int *const *K,
+                                 const int *K_cap, int LDAU, void *const *CD, int LDCD,
+                                 double alpha);
+
 /* Upload descriptors and carve the device workspace.  Called once. */
 int HCORE_hip_plan_finalize(hcore_hip_plan *plan);
 
@@ -89,6 +114,21 @@ int HCORE_hip_plan_execute(hcore_hip_plan *plan, void *stream);
 
 /* Same, through a hipGraph captured on the first call and replayed after. */
 int HCORE_hip_plan_execute_graph(hcore_hip_plan *plan, void *stream);
+
+/*
+ * Autotuning (no counterpart in the reference).  After at least one
+ * execution, time each core-SVD launch of the plan with its register-resident
+ * variant and with the same algorithm at twice (or half) the wavefronts per
+ * tile, reps times each, and keep the faster for later executions (a captured
+ * graph is captured again).  Results are not changed.  The choice is cached
+ * per (M, precision, variant) for plans finalized later in the process, and
+ * in the file named by HCORE_TUNE_CACHE across processes.  report (may be
+ * NULL) receives up to nreport entries of 6 doubles: lane, base variant,
+ * chosen variant, best ms of the base, best ms of the alternative, tiles.
+ * Returns the number of launches tuned, or -HCORE_HIP_* on error.
+ */
+int HCORE_hip_plan_autotune(hcore_hip_plan *plan, void *stream, int reps, double *report,
+                            int nreport);
 
 /* Device array of the n new ranks (int), valid after execution. */
 const int *HCORE_hip_plan_ranks_device(const hcore_hip_plan *plan);
