@@ -35,8 +35,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12         # B/s
 FP64_PEAK = 78.6e12       # FLOP/s (MI355X dense FP64, matrix and vector)
-# execution unit of each phase's kernels (all FP64 work on the vector ALUs)
-ISSUED_ON = {}
+# execution unit of each phase's kernels: the products run on FP64 MFMA, the
+# stacking, the Householder QR, the core SVD and the compact-WY solve on the
+# vector ALUs
+ISSUED_ON = {p: "MFMA" for p in ("F=AV^T BV", "G=BU F^T", "T=RA RB^T+Gram", "Zv=T^T Zu",
+                                 "reassembly", "syrk_W", "syrk_X", "syrk_CD")}
 PHASES = ["stack", "F=AV^T BV", "G=BU F^T", "geqrf", "T=RA RB^T+Gram", "core_svd",
           "wy_solve", "Zv=T^T Zu", "reassembly", "syrk_W", "syrk_X", "syrk_CD"]
 

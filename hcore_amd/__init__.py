@@ -78,6 +78,17 @@ def lib():
         L.HCORE_dgemmbdcd.restype = None
         L.HCORE_duncompress.argtypes = [i, i, i, i, d, _dp, _dp, i, _dp, _dp, i, d, _dp, i]
         L.HCORE_duncompress.restype = None
+        L.HCORE_dcompress.argtypes = [i, i, _dp, i, _dp, i, _dp, i, _dp, i, d]
+        L.HCORE_dcompress.restype = None
+        L.HCORE_dgytlr.argtypes = [i, i, _dp, _dp, _dp, _dp, i, i, i, i, i, i, C.c_ulonglong, i,
+                                   d, i, _dp]
+        L.HCORE_dgytlr.restype = None
+        L.HCORE_hip_gytlr_batch.argtypes = [i, i, i, _ip, _ip, i, C.c_ulonglong, _pp, i, _pp, i,
+                                            _vp, _pp, i, i, d, _vp]
+        L.HCORE_hip_compress_batch.argtypes = [i, i, i, _pp, i, _pp, i, _pp, i, _vp, i, d, _vp]
+        L.HCORE_hip_set_covariance.argtypes = [d, d]
+        L.HCORE_hip_tlr_dpotrf.argtypes = [i, i, _pp, _pp, _pp, _ip, i, i, d, _vp, _ip, fl, _ip]
+        L.HCORE_hip_covariance_points.argtypes = [i, C.c_ulonglong, _dp]
         for fn in ("HCORE_dgemm_qr_svd", "HCORE_dgemm_qr_svd_b_dense", "HCORE_dgemm_ormqr"):
             getattr(L, fn).restype = None
         L.HCORE_hip_gemm.argtypes = [i, i, i, i, i, i, d, _vp, i, _vp, i, d, _vp, i, i, _vp]
@@ -272,6 +283,90 @@ def duncompress(AU, BV):
     lib().HCORE_duncompress(111, 112, M, N, 1.0, _ptr(AU), _ptr(ra), M, _ptr(BV), _ptr(rb), N,
                             0.0, _ptr(CD), M)
     return CD
+
+
+def dcompress(A, maxrank=None, acc=1e-8):
+    """HCORE_dcompress on a host tile A (m x n); returns (U, V, rank) with
+    U (m x r) the left singular vectors and V (n x r) the sigma-scaled right
+    ones."""
+    A = _fortran(A, np.float64)
+    m, n = A.shape
+    mn = min(m, n)
+    maxrank = mn if maxrank is None else maxrank
+    U = np.zeros((m, mn), order="F")
+    V = np.zeros((n, mn), order="F")
+    rk = np.zeros(1)
+    lib().HCORE_dcompress(m, n, _ptr(A), m, _ptr(U), m, _ptr(V), n, _ptr(rk), maxrank, acc)
+    r = int(rk[0])
+    return np.asfortranarray(U[:, :r]), np.asfortranarray(V[:, :r]), r
+
+
+def dgytlr(m, n, bigM, m0, n0, seed, maxrank=None, tol=1e-8, compress_diag=False, dense=False):
+    """HCORE_dgytlr into host arrays.  Returns (U, V, rank, D): for a diagonal
+    tile without compress_diag U, V are None, rank is None and D the dense
+    tile; otherwise D is the dense tile when `dense`, else None."""
+    mn = min(m, n)
+    maxrank = mn if maxrank is None else maxrank
+    U = np.zeros((m, mn), order="F")
+    V = np.zeros((n, mn), order="F")
+    AD = np.zeros((m, n), order="F")
+    D = np.zeros((m, n), order="F") if dense else None
+    rk = np.array([-1.0])
+    lib().HCORE_dgytlr(m, n, _ptr(U), _ptr(V), _ptr(AD), _ptr(rk), m, m, n, bigM, m0, n0, seed,
+                       maxrank, tol, int(compress_diag), _ptr(D) if D is not None else None)
+    if m0 == n0 and not compress_diag:
+        return None, None, None, AD
+    r = int(rk[0])
+    return np.asfortranarray(U[:, :r]), np.asfortranarray(V[:, :r]), r, D
+
+
+def gytlr_batch(m, n, m0, n0, bigM, seed, AU, ldu, AV, ldv, rank_ptr, AD=None, lda=0,
+                maxrank=None, tol=1e-8, stream=None):
+    """HCORE_hip_gytlr_batch on device pointers (lists of addresses; AD
+    entries None/0 for compressed tiles).  rank_ptr: device int array."""
+    k = len(m0)
+    a0, p0 = _iarr(m0)
+    a1, p1 = _iarr(n0)
+    ad = _parr([x or 0 for x in AD]) if AD is not None else None
+    st = lib().HCORE_hip_gytlr_batch(k, m, n, p0, p1, bigM, seed, _parr(AU), ldu, _parr(AV), ldv,
+                                     rank_ptr, ad, lda, min(m, n) if maxrank is None else maxrank,
+                                     tol, stream)
+    return st
+
+
+def compress_batch(m, n, A, lda, AU, ldu, AV, ldv, rank_ptr, maxrank=None, tol=1e-8, stream=None):
+    """HCORE_hip_compress_batch on device pointers."""
+    return lib().HCORE_hip_compress_batch(len(A), m, n, _parr(A), lda, _parr(AU), ldu, _parr(AV),
+                                          ldv, rank_ptr, min(m, n) if maxrank is None else maxrank,
+                                          tol, stream)
+
+
+def tlr_dpotrf(nb, D, U, V, ranks, cap, maxrk, acc, stream=None, flops=None):
+    """HCORE_hip_tlr_dpotrf: D (nt device addresses), U, V (nt*nt addresses,
+    tile (i, j) at i + j*nt, 0 above the diagonal), ranks (nt x nt ints,
+    column-major, updated in place).  Returns (status, info, step_max)."""
+    nt = len(D)
+    rk = np.asfortranarray(ranks, dtype=np.int32)
+    flat = np.ascontiguousarray(rk.reshape(-1, order="F"))
+    info = C.c_int(0)
+    smax = np.zeros(nt, dtype=np.int32)
+    fc = flops if flops is not None else FlopCounter()
+    st = lib().HCORE_hip_tlr_dpotrf(nt, nb, _parr(D), _parr(U), _parr(V),
+                                    flat.ctypes.data_as(_ip), cap, maxrk, acc, stream,
+                                    C.byref(info), C.byref(fc), smax.ctypes.data_as(_ip))
+    ranks[...] = flat.reshape(nt, nt, order="F")
+    return st, info.value, smax
+
+
+def set_covariance(beta, nugget):
+    return lib().HCORE_hip_set_covariance(beta, nugget)
+
+
+def covariance_points(bigM, seed):
+    """The covariance problem's Morton-sorted points, as (bigM, 2)."""
+    xy = np.zeros(2 * bigM)
+    _check(lib().HCORE_hip_covariance_points(bigM, seed, _ptr(xy)), "covariance_points")
+    return np.stack([xy[:bigM], xy[bigM:]], axis=1)
 
 
 def dgemm_split(alpha, AU, AV, BU, BV, beta, CU, CV, rk=0, maxrk=None, acc=1e-8,

@@ -20,6 +20,7 @@
 #include "flop_counts.h"
 #include "hcore_d.h"
 #include "hcore_hip.h"
+#include "hcore_host.hpp"
 #include "hcore_internal.hpp"
 #include "hcore_s.h"
 
@@ -97,15 +98,6 @@ __attribute__((constructor)) static void policy_from_env() {
   if (qreg && strcmp(qreg, "0") == 0) g_ctx.qr_reg = false;
 }
 
-static int hip_fail(hipError_t e, const char* what) {
-  fprintf(stderr, "hcore_amd: HIP error in %s: %s\n", what, hipGetErrorString(e));
-  return HCORE_HIP_ERR_RUNTIME;
-}
-#define HCHECK(call)                                          \
-  do {                                                        \
-    hipError_t e_ = (call);                                   \
-    if (e_ != hipSuccess) return hip_fail(e_, #call);         \
-  } while (0)
 
 static int ctx_init_locked(int device) {
   if (g_ctx.ready) return HCORE_HIP_OK;
@@ -131,28 +123,6 @@ static int ctx_ensure() {
   return ctx_init_locked(-1);
 }
 
-// Grow-only device buffer.  Growing drops the contents; callers grow it only
-// when no work of theirs on it is in flight.
-struct DevArena {
-  char* p = nullptr;
-  size_t cap = 0;
-  int reserve(size_t bytes) {
-    if (cap >= bytes) return HCORE_HIP_OK;
-    if (p) HCHECK(hipFree(p));
-    p = nullptr;
-    cap = 0;
-    const size_t mib = size_t(1) << 20;
-    const size_t c = ((std::max(bytes, cap * 2) + mib - 1) / mib) * mib;
-    HCHECK(hipMalloc(&p, c));
-    cap = c;
-    return HCORE_HIP_OK;
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    cap = 0;
-  }
-};
 
 // Per-thread state of the single-tile entry points (HCORE_dgemm, HCORE_sgemm,
 // HCORE_dsyrk, HCORE_dgemmcd): a stream of its own, and grow-only device
@@ -162,33 +132,10 @@ struct DevArena {
 // threads at once, one tile each; with per-thread state those calls share
 // no lock and overlap on the GPU, and a call allocates nothing once its
 // thread's buffers have grown to the tile size.
-// Grow-only pinned host buffer (hipHostMalloc): host operands are packed into
-// it on the CPU and moved by one DMA transfer each way, instead of one
-// pageable transfer per column.
-struct PinnedArena {
-  char* p = nullptr;
-  size_t cap = 0;
-  int reserve(size_t bytes) {
-    if (cap >= bytes) return HCORE_HIP_OK;
-    if (p) HCHECK(hipHostFree(p));
-    p = nullptr;
-    cap = 0;
-    const size_t mib = size_t(1) << 20;
-    const size_t c = ((bytes + mib - 1) / mib) * mib;
-    HCHECK(hipHostMalloc(reinterpret_cast<void**>(&p), c, hipHostMallocDefault));
-    cap = c;
-    return HCORE_HIP_OK;
-  }
-  void release() {
-    if (p) (void)hipHostFree(p);
-    p = nullptr;
-    cap = 0;
-  }
-};
-
 struct ThreadCtx {
   hipStream_t stream = nullptr;
   DevArena plan, stage, chain;
+  DevArena tile_work, tile_desc;  // tile generation and compression (hcore_tiles.cpp)
   PinnedArena hstage;      // pinned image of the staged operands and results
   std::vector<char> desc;  // host image of the last descriptors (alive until the stream syncs)
   int* hres = nullptr;     // pinned: [0] rank, [1] status
@@ -225,6 +172,8 @@ static void release_thread_ctxs() {
     c->plan.release();
     c->stage.release();
     c->chain.release();
+    c->tile_work.release();
+    c->tile_desc.release();
     c->hstage.release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->hres) (void)hipHostFree(c->hres);
@@ -292,74 +241,6 @@ static void tune_store(int M, bool fp64, int base, int choice) {
   fclose(f);
 }
 
-// ------------------------------------------------------------------------------
-// host-side descriptor packing
-// ------------------------------------------------------------------------------
-// Output tiles of one batched GEMM phase: 64 x 64 tiles, or 32 x 32 tiles
-// (one wavefront each) for problems with m or n <= 32, which would otherwise
-// compute mostly padding.
-struct TaskList {
-  std::vector<GemmTask> big, small, tall;
-  size_t size() const { return big.size() + small.size() + tall.size(); }
-};
-struct TaskRef {
-  size_t ob = 0, os = 0, ot = 0;
-  int nb = 0, ns = 0, nt = 0;
-};
-
-struct Packer {
-  std::vector<char> buf;
-  template <typename X>
-  size_t add(const std::vector<X>& v) {
-    size_t off = (buf.size() + 255) & ~size_t(255);
-    buf.resize(off + v.size() * sizeof(X));
-    if (!v.empty()) memcpy(buf.data() + off, v.data(), v.size() * sizeof(X));
-    return off;
-  }
-  TaskRef add(const TaskList& tl) {
-    TaskRef r;
-    r.ob = add(tl.big);
-    r.nb = (int)tl.big.size();
-    r.os = add(tl.small);
-    r.ns = (int)tl.small.size();
-    r.ot = add(tl.tall);
-    r.nt = (int)tl.tall.size();
-    return r;
-  }
-};
-
-// Output tiles of one product: tall-skinny blocks (kTallRows x kTallCols) for
-// products of the nb rows of a tile with a few dozen columns, else 64 x 64
-// tiles, or 32 x 32 ones when m or n <= 32.
-template <typename T>
-static void add_tasks(TaskList& tl, int prob, const GemmProb<T>& g) {
-  const int m = g.m, n = g.n;
-  const bool tall = !(g.flags & (GF_A_TRANS | GF_LOWER_ZERO_UPPER | GF_LOWER_ONLY | GF_UPPER_ONLY)) &&
-                    m >= kTallMinM &&
-                    n <= kTallMaxN && g.k <= kTallMaxK;
-  if (tall) {
-    for (int tj = 0; tj < (n + kTallCols - 1) / kTallCols; ++tj)
-      for (int ti = 0; ti < (m + kTallRows - 1) / kTallRows; ++ti) tl.tall.push_back({prob, ti, tj});
-    return;
-  }
-  // 32-tiles for thin outputs, and for long reductions (k = nb) into outputs
-  // of at most 128 x 128, whose phases share each tile's k range among four
-  // wavefronts (split-k)
-  const bool sm = m <= 32 || n <= 32 || (g.k >= 256 && m <= 128 && n <= 128);
-  const int t = sm ? 32 : 64;
-  auto& v = sm ? tl.small : tl.big;
-  for (int tj = 0; tj < (n + t - 1) / t; ++tj)
-    for (int ti = 0; ti < (m + t - 1) / t; ++ti) v.push_back({prob, ti, tj});
-}
-
-// The launches of one task list: 64-tiles, 32-tiles, tall-skinny blocks.
-template <typename T>
-static void launch_tasks(const GemmProb<T>* probs, const char* base, const TaskRef& r,
-                         hipStream_t s, bool split) {
-  launch_gemm<T>(probs, reinterpret_cast<const GemmTask*>(base + r.ob), r.nb, 64, s, split);
-  launch_gemm<T>(probs, reinterpret_cast<const GemmTask*>(base + r.os), r.ns, 32, s, split);
-  launch_gemm_tall<T>(probs, reinterpret_cast<const GemmTask*>(base + r.ot), r.nt, s);
-}
 
 }  // namespace hcore
 
@@ -520,8 +401,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
   }
   // variants that need the K x K global scratch (work matrix, or the R^T
   // transposition of the register-resident kernels)
-  auto global_core = [](int v) { return v == CORE_K192 || v == CORE_K256 || v == CORE_K512 ||
-                                        v >= CORE_R64; };
+  auto global_core = core_needs_scratch;
   // reassembly per lane: the compact-WY solve and GEMMs up to kWyMaxK,
   // the per-panel reflector application beyond
   bool lane_wy[3];
@@ -1473,6 +1353,24 @@ void dgemm_fixed_workspace(int M, double alpha, double* AU, double* AV, double* 
 
 void report_failure(int status, const char* msg) { fail(status, msg); }
 
+int ensure_ready() { return ctx_ensure(); }
+bool device_pointer(const void* ptr) { return is_device_ptr(ptr); }
+void set_last_status(int status) { t_last_status = status; }
+int core_variant_ctx(int K, size_t elem) {
+  return core_variant_for(K, elem == 8 ? g_ctx.core_kl_d : g_ctx.core_kl_f, g_ctx.core_reg);
+}
+bool thread_res(ThreadRes* out) {
+  ThreadCtx* tc = thread_ctx();
+  if (!tc) return false;
+  out->stream = tc->stream;
+  out->work = &tc->tile_work;
+  out->stage = &tc->stage;
+  out->desc = &tc->tile_desc;
+  out->pinned = &tc->hstage;
+  out->hdesc = &tc->desc;
+  return true;
+}
+
 }  // namespace hcore
 
 using namespace hcore;
@@ -2298,6 +2196,7 @@ int HCORE_hip_finalize(void) {
   std::lock_guard<std::mutex> lk(g_ctx.mu);
   if (!g_ctx.ready) return HCORE_HIP_OK;
   release_thread_ctxs();
+  release_tile_state();
   ++g_ctx.generation;
   g_ctx.ready = false;
   return HCORE_HIP_OK;

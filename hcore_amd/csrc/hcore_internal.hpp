@@ -186,6 +186,11 @@ enum : int {
   CORE_R64W, CORE_R128W, CORE_R192W, CORE_NVARIANTS
 };
 int core_variant_for(int K, int lds_max_k, bool reg);
+// Variants whose work matrix (or R^T transposition) lives in the K x (K + 64)
+// global scratch CoreProb::Tj.
+inline bool core_needs_scratch(int v) {
+  return v == CORE_K192 || v == CORE_K256 || v == CORE_K512 || v >= CORE_R64;
+}
 // The other wavefront count of a register-resident variant (CORE_R64 <->
 // CORE_R64W, ...), or -1: the alternative an autotuned plan may pick.
 int core_alt_variant(int variant);
@@ -288,6 +293,42 @@ void launch_potf2(T* A, int lda, int b0, int nbk, int upper, int* info, hipStrea
 // B = alpha * B for an m x n block (alpha == 0: zero fill).
 template <typename T>
 void launch_scale(T* B, int ldb, int m, int n, T alpha, hipStream_t s);
+
+// ---- tile generation and compression (k_tile.hip) ----------------------------
+// One tile written into out (leading dimension ldo): entries (i, j) with i < m
+// and j < n come from the dense source src (leading dimension lds) or, when
+// src is null, from the covariance of points p_(m0+i) and p_(n0+j),
+// exp(-|p - q| / beta), plus nugget where the global row equals the global
+// column; entries with i < rows, j < cols outside m x n are written as 0.
+struct TileFillProb {
+  const double* src;
+  const double* px;  // sorted point coordinates: x[bigM], then y[bigM]
+  const double* py;
+  double* out;
+  int lds, ldo;
+  int m0, n0, m, n, rows, cols;
+  double beta, nugget;
+};
+constexpr int kFillCols = 8;  // tile columns per workgroup of k_tile_fill
+void launch_tile_fill(const TileFillProb* probs, int n, int max_cols, hipStream_t s);
+
+// After the core SVD of a K x K padded tile: the truncated rank (0 when the
+// core failed or sigma_0 < acc, else the core's rank capped by min(m, n) and
+// maxrank) to rank_out (and rank_out_d as a double when set), and the first
+// rank columns of Zu (rows < m) to U.
+struct CompressProb {
+  const double* Zu;
+  const double* sigma;
+  const int* rank;
+  const int* status;
+  int* rank_out;
+  double* rank_out_d;
+  double* U;
+  int ldu, K, m, mn, maxrank;
+  double acc;
+};
+constexpr int kFinishCols = 16;  // columns of U per workgroup of k_compress_finish
+void launch_compress_finish(const CompressProb* probs, int n, int kmax, hipStream_t s);
 
 // Size in elements of the compact-WY storage for a K-column stack.
 inline int tp_elems(int K) { return ((K + kPB - 1) / kPB) * kPB * kPB; }

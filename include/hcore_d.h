@@ -147,6 +147,40 @@ void HCORE_duncompress(HCORE_enum transA, HCORE_enum transB,
                        double *CD, int LDC);
 
 /*
+ * Compression of one dense m x n tile A (max(m, n) <= 512): U (m x r, ldu)
+ * receives the first r left singular vectors, V (n x r, ldv) the right
+ * singular vectors scaled by the singular values (A ~ U V^T), Ark[0] = r,
+ * the number of singular values >= acc, capped by maxrank.  No library
+ * symbol of the reference: it restates the compression step of the
+ * reference's tile generator generate_compress
+ * (testing/testing_dgemm.c:119-236, truncation at :208-212).  Host or
+ * device pointers; U and V must hold min(m, n, maxrank) columns.
+ */
+void HCORE_dcompress(int m, int n, const double *A, int lda,
+                     double *U, int ldu, double *V, int ldv,
+                     double *Ark, int maxrank, double acc);
+
+/*
+ * Generate tile (m0, n0) (m x n) of the bigM x bigM covariance problem and
+ * compress it: the real-precision counterpart of HCORE_zgytlr, which the
+ * reference declares without defining (src/include/hcore_z.h:63-75), with
+ * the same arguments.  The problem: bigM points in [0,1)^2 drawn from the
+ * splitmix64 sequence of `seed`, Morton-sorted; entry (i, j) =
+ * exp(-|p_i - p_j| / beta) + nugget [i == j] (beta 0.1, nugget 1e-4 unless
+ * HCORE_hip_set_covariance says otherwise).  A diagonal tile (m0 == n0) with
+ * compress_diag == 0 is written dense into AD (lda) and Ark is not touched;
+ * otherwise AU/AV/Ark as HCORE_dcompress with acc = tol.  Dense, when not
+ * NULL, also receives the dense tile (lda).
+ */
+void HCORE_dgytlr(int m, int n,
+                  double *AU, double *AV, double *AD, double *Ark,
+                  int lda, int ldu, int ldv,
+                  int bigM, int m0, int n0, unsigned long long int seed,
+                  int maxrank, double tol,
+                  int compress_diag,
+                  double *Dense);
+
+/*
  * HCORE_dgemm without any device allocation: the same arguments and results,
  * factors as DEVICE pointers only (ranks may be host or device doubles),
  * run on the null stream.  No counterpart in the reference: it is for

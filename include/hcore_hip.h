@@ -186,6 +186,42 @@ int HCORE_hip_gemm(int precision, int transA, int transB, int m, int n, int k, d
                    const void *A, int lda, const void *B, int ldb, double beta, void *C, int ldc,
                    int path, void *stream);
 
+/* Tile generation and compression in batches (all pointers device memory,
+ * tiles of one size m x nn, max(m, nn) <= 512).  Tile i of the covariance
+ * problem (HCORE_dgytlr's) starts at global row m0[i], column n0[i] (host
+ * arrays); when AD and AD[i] are not NULL it is written dense to AD[i] (lda),
+ * else compressed to AU[i] (ldu), AV[i] (ldv) with its rank in the device
+ * int rank[i].  HCORE_hip_compress_batch compresses the caller's dense tiles
+ * A[i] (lda).  stream NULL: the calling thread's library stream.  Both
+ * synchronize before returning; HCORE_HIP_ERR_NUMERIC when a tile held
+ * NaN/Inf. */
+int HCORE_hip_gytlr_batch(int n, int m, int nn, const int *m0, const int *n0, int bigM,
+                          unsigned long long seed, double *const *AU, int ldu,
+                          double *const *AV, int ldv, int *rank, double *const *AD, int lda,
+                          int maxrank, double tol, void *stream);
+int HCORE_hip_compress_batch(int n, int m, int nn, const double *const *A, int lda,
+                             double *const *AU, int ldu, double *const *AV, int ldv, int *rank,
+                             int maxrank, double tol, void *stream);
+/* Kernel parameters of the covariance problem (defaults beta 0.1, nugget 1e-4). */
+int HCORE_hip_set_covariance(double beta, double nugget);
+/* The problem's sorted points on the host: xy[0..bigM) = x, xy[bigM..2 bigM) = y. */
+int HCORE_hip_covariance_points(int bigM, unsigned long long seed, double *xy);
+
+/* The TLR Cholesky update stream (the loop HiCMA runs on top of HCORE):
+ * for k = 0 .. nt-1, HCORE_dpotrf of the dense diagonal tile D[k] (lower),
+ * V_ik <- L_kk^{-1} V_ik for i > k, HCORE_dsyrk D_i -= A_ik A_ik^T and the
+ * recompression A_ij -= A_ik A_jk^T (i > j > k) as one batched plan per
+ * step.  D: nt device pointers (nb x nb, ld nb); U, V: nt*nt device pointers
+ * (tile (i, j) at i + j*nt, used for i > j; nb x cap, ld nb, cap >= maxrk);
+ * rank: host ints, updated in place; maxrk, acc: as HCORE_dgemm.  When the V
+ * factors of a panel lie cap columns apart in one buffer, the panel's trsm
+ * is one call.  info: 0, or the global index of the first non-positive
+ * pivot.  step_ranks_max (optional, nt ints): largest new rank per step.
+ * Synchronous; follows the library's error policy. */
+int HCORE_hip_tlr_dpotrf(int nt, int nb, double *const *D, double *const *U, double *const *V,
+                         int *rank, int cap, int maxrk, double acc, void *stream, int *info,
+                         flop_counter *flops, int *step_ranks_max);
+
 /* Device memory and stream helpers (the library's own HIP runtime; callers
  * that share device buffers with the library allocate them here). */
 int HCORE_hip_malloc(void **ptr, size_t bytes);

@@ -30,6 +30,11 @@
  *   orc_dtrsm         src/compute/hcore_dtrsm.c:24-48
  *   orc_dgemmbdcd     src/compute/hcore_dgemmbdcd.c:35-122
  *   orc_duncompress   src/compute/hcore_duncompress.c:33-84
+ *   orc_dcompress     testing/testing_dgemm.c:180-226 (compression step of
+ *                     the tile generator)
+ *   orc_covariance_points / orc_covariance_tile  the covariance problem of
+ *                     HCORE_dgytlr (declared-only HCORE_zgytlr,
+ *                     src/include/hcore_z.h:63-75; problem defined here)
  *
  * Parity pins (tests/test_oracle.py): the reference's own assertions
  * (testing_dgemm.c:377-378, testing_sgemm.c:378-379, testing_dsyrk.c:181,
@@ -511,4 +516,111 @@ int orc_duncompress(int M, int N, const double *AU, int Ark, int LDA, const doub
     if (Ark != Brk) return -1;
     orc_dgemm_gemm('N', 'T', M, N, Ark, 1.0, AU, LDA, BV, LDB, 0.0, CD, LDC);
     return 0;
+}
+
+/* ---- tile compression and the covariance problem ---------------------------
+ * orc_dcompress: the compression step of the reference's generator
+ * (testing_dgemm.c:180-226): dgesvd('A','A') of a copy of the m x n tile,
+ * r = #{S_k >= acc} (k < min(m, n)) capped at maxrank (:208-212),
+ * U = the first r left vectors (:219), V = (S V^T)^T's first r columns
+ * (:162-166, :224).  mutate 1: V left unscaled; mutate 2: one singular
+ * value too many kept (planted mistakes the pins must catch). */
+int orc_dcompress(int m, int n, const double *A, int lda, double *U, int ldu, double *V,
+                  int ldv, int maxrank, double acc, int mutate, int *prank)
+{
+    lint M = m, N = n, ld = m, info = 0, lwork = -1, ldu_ = m, ldvt = n;
+    int mn = m < n ? m : n;
+    double *cl = (double *)malloc(sizeof(double) * (size_t)m * n);
+    double *tU = (double *)malloc(sizeof(double) * (size_t)m * m);
+    double *tV = (double *)malloc(sizeof(double) * (size_t)n * n);
+    double *S = (double *)malloc(sizeof(double) * (size_t)(mn > 0 ? mn : 1));
+    for (int j = 0; j < n; ++j) memcpy(cl + (size_t)j * m, A + (size_t)j * lda, sizeof(double) * m);
+    double q;
+    scipy_dgesvd_("A", "A", &M, &N, cl, &ld, S, tU, &ldu_, tV, &ldvt, &q, &lwork, &info, 1, 1);
+    lwork = (lint)q;
+    double *w = (double *)malloc(sizeof(double) * (size_t)lwork);
+    scipy_dgesvd_("A", "A", &M, &N, cl, &ld, S, tU, &ldu_, tV, &ldvt, w, &lwork, &info, 1, 1);
+    free(w);
+    int r = 0;
+    if (!info) {
+        while (r < mn && S[r] >= acc) ++r;
+        if (mutate == 2 && r < mn) ++r;
+        if (r > maxrank) r = maxrank;
+        for (int k = 0; k < r; ++k) {
+            for (int i = 0; i < m; ++i) U[(size_t)k * ldu + i] = tU[(size_t)k * m + i];
+            const double s = mutate == 1 ? 1.0 : S[k];
+            for (int i = 0; i < n; ++i) V[(size_t)k * ldv + i] = s * tV[(size_t)i * n + k];
+        }
+    }
+    *prank = r;
+    free(cl); free(tU); free(tV); free(S);
+    return (int)info;
+}
+
+/* The covariance problem of HCORE_dgytlr (the real counterpart of the
+ * declared-only HCORE_zgytlr, src/include/hcore_z.h:63-75; the reference
+ * defines no problem, so this is the build's own definition): bigM points,
+ * point i = (u_{2i+1}, u_{2i+2}) with u_k = splitmix64 output k of `seed`
+ * mapped to [0, 1) by its top 53 bits, sorted by Morton code on a
+ * 2^16 x 2^16 grid (x in the even bits), ties by index. */
+static uint64_t orc_splitmix(uint64_t seed, uint64_t k)
+{
+    uint64_t z = seed + k * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+uint64_t orc_splitmix64(uint64_t seed, uint64_t k) { return orc_splitmix(seed, k); }
+
+static uint32_t orc_morton(double x, double y)
+{
+    double qx = x * 65536.0, qy = y * 65536.0;
+    uint32_t ix = qx >= 65535.0 ? 65535u : (uint32_t)qx, iy = qy >= 65535.0 ? 65535u : (uint32_t)qy;
+    uint32_t c = 0;
+    for (int b = 0; b < 16; ++b) c |= ((ix >> b) & 1u) << (2 * b) | ((iy >> b) & 1u) << (2 * b + 1);
+    return c;
+}
+
+static const uint32_t *orc_sort_codes;
+static int orc_cmp_code(const void *a, const void *b)
+{
+    int i = *(const int *)a, j = *(const int *)b;
+    if (orc_sort_codes[i] != orc_sort_codes[j]) return orc_sort_codes[i] < orc_sort_codes[j] ? -1 : 1;
+    return (i > j) - (i < j);
+}
+
+void orc_covariance_points(int bigM, uint64_t seed, double *xy)
+{
+    double *x = (double *)malloc(sizeof(double) * (size_t)bigM);
+    double *y = (double *)malloc(sizeof(double) * (size_t)bigM);
+    uint32_t *code = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)bigM);
+    int *ord = (int *)malloc(sizeof(int) * (size_t)bigM);
+    for (int i = 0; i < bigM; ++i) {
+        x[i] = (double)(orc_splitmix(seed, 2 * (uint64_t)i + 1) >> 11) * 0x1.0p-53;
+        y[i] = (double)(orc_splitmix(seed, 2 * (uint64_t)i + 2) >> 11) * 0x1.0p-53;
+        code[i] = orc_morton(x[i], y[i]);
+        ord[i] = i;
+    }
+    orc_sort_codes = code;
+    qsort(ord, (size_t)bigM, sizeof(int), orc_cmp_code);
+    for (int i = 0; i < bigM; ++i) {
+        xy[i] = x[ord[i]];
+        xy[bigM + i] = y[ord[i]];
+    }
+    free(x); free(y); free(code); free(ord);
+}
+
+/* Tile (m0, n0) of size m x n: exp(-|p_i - p_j| / beta) + nugget on the
+ * global diagonal; xy as orc_covariance_points wrote it. */
+void orc_covariance_tile(int bigM, const double *xy, int m0, int n0, int m, int n, double beta,
+                         double nugget, double *out, int ld)
+{
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < m; ++i) {
+            const int gi = m0 + i, gj = n0 + j;
+            const double dx = xy[gi] - xy[gj], dy = xy[bigM + gi] - xy[bigM + gj];
+            double v = exp(-sqrt(dx * dx + dy * dy) / beta);
+            if (gi == gj) v += nugget;
+            out[(size_t)j * ld + i] = v;
+        }
 }

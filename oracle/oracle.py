@@ -64,6 +64,14 @@ def lib():
         L.orc_dgemmbdcd.argtypes = [i, i, i, d, _dp, _dp, i, i, _dp, i, d, _dp, i, _ulp]
         L.orc_duncompress.restype = i
         L.orc_duncompress.argtypes = [i, i, _dp, i, i, _dp, i, i, _dp, i]
+        L.orc_dcompress.restype = i
+        L.orc_dcompress.argtypes = [i, i, _dp, i, _dp, i, _dp, i, i, d, i, _ip]
+        L.orc_splitmix64.restype = C.c_uint64
+        L.orc_splitmix64.argtypes = [C.c_uint64, C.c_uint64]
+        L.orc_covariance_points.restype = None
+        L.orc_covariance_points.argtypes = [i, C.c_uint64, _dp]
+        L.orc_covariance_tile.restype = None
+        L.orc_covariance_tile.argtypes = [i, _dp, i, i, i, i, d, d, _dp, i]
         L.orc_blas_config.restype = C.c_char_p
         L.orc_max_threads.restype = C.c_int
         L.orc_set_blas_threads.argtypes = [C.c_int]
@@ -250,3 +258,38 @@ def duncompress(AU, BV):
                                _ptr(CD), M)
     assert st == 0
     return CD
+
+
+def dcompress(A, maxrank=None, acc=1e-8, mutate=0):
+    """Oracle compression (testing_dgemm.c:180-226): returns (U, V, rank)."""
+    A = _f64(A)
+    m, n = A.shape
+    mn = min(m, n)
+    U = np.zeros((m, mn), order="F")
+    V = np.zeros((n, mn), order="F")
+    r = C.c_int(0)
+    st = lib().orc_dcompress(m, n, _ptr(A), m, _ptr(U), m, _ptr(V), n,
+                             mn if maxrank is None else maxrank, acc, mutate, C.byref(r))
+    assert st == 0
+    k = r.value
+    return np.asfortranarray(U[:, :k]), np.asfortranarray(V[:, :k]), k
+
+
+def splitmix64(seed, k):
+    return int(lib().orc_splitmix64(seed, k))
+
+
+def covariance_points(bigM, seed):
+    """The covariance problem's sorted points, (bigM, 2)."""
+    xy = np.zeros(2 * bigM)
+    lib().orc_covariance_points(bigM, seed, _ptr(xy))
+    return np.stack([xy[:bigM], xy[bigM:]], axis=1)
+
+
+def covariance_tile(bigM, seed, m0, n0, m, n, beta=0.1, nugget=1e-4, points=None):
+    """Tile (m0, n0) (m x n) of the covariance problem."""
+    P = covariance_points(bigM, seed) if points is None else points
+    xy = np.ascontiguousarray(np.concatenate([P[:, 0], P[:, 1]]))
+    out = np.zeros((m, n), order="F")
+    lib().orc_covariance_tile(bigM, _ptr(xy), m0, n0, m, n, beta, nugget, _ptr(out), m)
+    return out
