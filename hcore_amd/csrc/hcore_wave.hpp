@@ -140,6 +140,39 @@ __device__ __forceinline__ double eps_of<double>() { return DBL_EPSILON * 0.5; }
 template <>
 __device__ __forceinline__ float eps_of<float>() { return FLT_EPSILON * 0.5f; }
 
+// Reciprocal and reciprocal square root from the hardware approximations
+// (v_rcp / v_rsq) refined by Newton steps: two for double (each doubles the
+// correct bits of the ~2^-26 seed), one for float.  For positive normal x
+// (rcp: any normal x) they are within an ulp or two of 1/x and 1/sqrt(x),
+// enough for rotation parameters and reflector scalings, in a fraction of
+// the latency of IEEE division and square root.
+template <typename T>
+__device__ __forceinline__ T fast_rcp(T x);
+template <>
+__device__ __forceinline__ double fast_rcp<double>(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(fma(-x, r, 1.0), r, r);
+  return fma(fma(-x, r, 1.0), r, r);
+}
+template <>
+__device__ __forceinline__ float fast_rcp<float>(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  return fmaf(fmaf(-x, r, 1.0f), r, r);
+}
+template <typename T>
+__device__ __forceinline__ T fast_rsq(T x);
+template <>
+__device__ __forceinline__ double fast_rsq<double>(double x) {
+  double r = __builtin_amdgcn_rsq(x);
+  r = fma(0.5 * r, fma(-x * r, r, 1.0), r);
+  return fma(0.5 * r, fma(-x * r, r, 1.0), r);
+}
+template <>
+__device__ __forceinline__ float fast_rsq<float>(float x) {
+  const float r = __builtin_amdgcn_rsqf(x);
+  return fmaf(0.5f * r, fmaf(-x * r, r, 1.0f), r);
+}
+
 template <int NT>
 __device__ __forceinline__ void block_sync() {
   if constexpr (NT == 64) wave_sync();

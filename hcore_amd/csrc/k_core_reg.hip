@@ -64,10 +64,17 @@ __device__ __forceinline__ void wave_sums(T (&d)[NV], T* xs, int lane) {
 // the smaller of the two angles (|t| <= 1).
 template <typename T>
 __device__ __forceinline__ bool jacobi_angle(T app, T aqq, T apq, T& cs, T& sn, T& t) {
-  const T zeta = (aqq - app) / (T(2) * apq);
-  if (fabs(zeta) > T(1e30)) t = T(0.5) / zeta;
-  else t = (zeta >= T(0) ? T(1) : T(-1)) / (fabs(zeta) + sqrt(fma(zeta, zeta, T(1))));
-  cs = T(1) / sqrt(fma(t, t, T(1)));
+  // divisions and square roots through the refined hardware approximations:
+  // this chain is on the critical path of every Jacobi round
+  const T zeta = (aqq - app) * (T(0.5) * fast_rcp(apq));
+  if (fabs(zeta) > T(1e30)) {
+    t = T(0.5) * fast_rcp(zeta);
+  } else {
+    const T h2 = fma(zeta, zeta, T(1));
+    const T h = h2 * fast_rsq(h2);  // sqrt(1 + zeta^2)
+    t = (zeta >= T(0) ? T(1) : T(-1)) * fast_rcp(fabs(zeta) + h);
+  }
+  cs = fast_rsq(fma(t, t, T(1)));
   sn = cs * t;
   return true;
 }
@@ -93,10 +100,11 @@ __device__ __forceinline__ bool jacobi_params(T app, T aqq, T apq, T tol, T thr2
   cs = T(1);
   sn = T(0);
   t = T(0);
-  if (!(fmin(app, aqq) > thr2)) return false;
-  const T nn = sqrt(app * aqq);
-  if (nn > T(0)) rho = fmaxf(rho, (float)fabs(apq) / (float)nn);
-  if (!(fabs(apq) > tol * nn)) return false;
+  const T pq = app * aqq;
+  if (!(fmin(app, aqq) > thr2) || !(pq > T(0))) return false;
+  const T rel = fabs(apq) * fast_rsq(pq);  // |apq| / sqrt(app aqq)
+  rho = fmaxf(rho, (float)rel);
+  if (!(rel > tol)) return false;
   return jacobi_angle(app, aqq, apq, cs, sn, t);
 }
 
@@ -366,10 +374,11 @@ __global__ __launch_bounds__(64 * W) void k_core_reg(const CoreProb<T>* __restri
       const T alpha = __shfl(sel, ls, 64);
       T beta = alpha, tau = T(0), scal = T(0);
       if (xn2 > T(0)) {
-        const T nrm = sqrt(fma(alpha, alpha, xn2));
+        const T h = fma(alpha, alpha, xn2);
+        const T nrm = h > T(1e-200) ? h * fast_rsq(h) : sqrt(h);  // the seed needs a normal h
         beta = alpha >= T(0) ? -nrm : nrm;
-        tau = (beta - alpha) / beta;
-        scal = T(1) / (alpha - beta);
+        tau = (beta - alpha) * fast_rcp(beta);
+        scal = fast_rcp(alpha - beta);
       }
 #pragma unroll
       for (int i = 0; i < RPL; ++i) {
