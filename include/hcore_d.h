@@ -181,6 +181,61 @@ void HCORE_dgytlr(int m, int n,
                   double *Dense);
 
 /*
+ * The recompression split at the rank reveal, for runtimes that size C's
+ * storage after the rank is known.  Replaces reference
+ * src/compute/hcore_dgemm.c:1022-1303 (HCORE_dgemm_qr_svd_b_dense),
+ * :1312-1597 (HCORE_dgemm_qr_svd) and :1602-1699 (HCORE_dgemm_ormqr);
+ * prototypes src/include/hcore_d.h:146-268.  Ranks are int here, as there.
+ *
+ * Phase 1 (qr_svd / qr_svd_b_dense) runs stacking, both QRs, the core SVD
+ * and the truncation on the GPU and returns the new rank in *_p_new_UVrk; its
+ * state stays on the device, reached again through the address it stores in
+ * *_p_newU (the other _p_ outputs are set for the reference's bookkeeping and
+ * are not dereferenced by the caller).  Phase 2 (ormqr), given those values
+ * back, writes the new factors (first new_UVrk columns) into CU/CV, which the
+ * caller may have reallocated in between (ld LDC >= M), sets Crk, and
+ * releases the state.  A new rank above storage_maxrank is rejected in phase
+ * 2 (hcore_dgemm.c:1648-1651) and the state released.  The _b_dense form
+ * takes B densely (M x M, ld LDB): G = B AV replaces G = BU (AV^T BV)^T.
+ */
+void HCORE_dgemm_qr_svd_b_dense(int transA, int transB, int M, int N, double alpha,
+                                double *AU, double *AV, int *Ark, int LDA,
+                                double *B, int LDB, double beta,
+                                double *CU, double *CV, int *Crk, int LDC,
+                                int rk, int storage_maxrank, int maxrk, double acc,
+                                double *work, flop_counter *flops,
+                                double **_p_work_new, double **_p__CU, double **_p__CV,
+                                int *_p_CU_ncols, int *_p_new_UVrk, double **_p_newU,
+                                int *_p_ld_newU, double **_p_qrtauA, int *_p_CV_ncols,
+                                double **_p_newV, int *_p_ld_newV, double **_p_qrtauB,
+                                int *_p_use_CUV_clone, double **_p_CUclone, int *_p_ld_CUclone,
+                                double **_p__CU_save, double **_p_CVclone, int *_p_ld_CVclone,
+                                double **_p__CV_save);
+void HCORE_dgemm_qr_svd(int transA, int transB, int M, int N, double alpha,
+                        double *AU, double *AV, int *Ark, int LDA,
+                        double *BU, double *BV, int *Brk, int LDB, double beta,
+                        double *CU, double *CV, int *Crk, int LDC,
+                        int rk, int storage_maxrank, int maxrk, double acc,
+                        double *work, flop_counter *flops,
+                        double **_p_work_new, double **_p__CU, double **_p__CV,
+                        int *_p_CU_ncols, int *_p_new_UVrk, double **_p_newU,
+                        int *_p_ld_newU, double **_p_qrtauA, int *_p_CV_ncols,
+                        double **_p_newV, int *_p_ld_newV, double **_p_qrtauB,
+                        int *_p_use_CUV_clone, double **_p_CUclone, int *_p_ld_CUclone,
+                        double **_p__CU_save, double **_p_CVclone, int *_p_ld_CVclone,
+                        double **_p__CV_save);
+void HCORE_dgemm_ormqr(int transA, int transB, int M, int N, double alpha,
+                       double *AU, double *AV, int *Ark, int LDA, double beta,
+                       double *CU, double *CV, int *Crk, int LDC,
+                       int rk, int storage_maxrank, int maxrk, double acc,
+                       double *work, flop_counter *flops,
+                       double *_CU, double *_CV, int CU_ncols, int new_UVrk,
+                       double *newU, int ld_newU, double *qrtauA, int CV_ncols,
+                       double *newV, int ld_newV, double *qrtauB, int use_CUV_clone,
+                       double *CUclone, int ld_CUclone, double *_CU_save,
+                       double *CVclone, int ld_CVclone, double *_CV_save);
+
+/*
  * HCORE_dgemm without any device allocation: the same arguments and results,
  * factors as DEVICE pointers only (ranks may be host or device doubles),
  * run on the null stream.  No counterpart in the reference: it is for

@@ -197,6 +197,31 @@ def test_oracle_flop_count_worked_value(orc):
     assert orc.flop_counts('s', 10) == 22000
 
 
+def test_flop_counts_hand_values(orc):
+    """flop_counts.c:5-51 evaluated by hand, including its float-precision
+    thirds, for the oracle and for the library (host code, no GPU):
+    q(1024, 1000): 2*1024*1000^2 = 2048000000; 2*1000^3 = 2e9 is exact in
+      float, 2e9/3.0f rounds to the float 666666688 (spacing 64 there, not
+      the exact 666666666); 2*1024*1000 = 2048000; 17000/3.0f -> 5666;
+      2048000000 - 666666688 + 2048000 + 5666 = 1383386978.
+    q(1024, 187): 71616512 - 4359468 (13078406/3.0f = 4359468.5) + 382976
+      + 1059 = 67641079.   q(500, 64): 4096000 - 174762 + 64000 + 362 = 3985600.
+    c(70): 343000/3 = 114333 (integer), - 4900/2.0, + 70/6 = 11 -> 111894.
+    o(10, 7, 5, 1): 4*7*10*5 - 2*7*25 + 3*7*5 = 1400 - 350 + 105 = 1155.
+    o(10, 7, 5, 2): 1400 - 2*10*25 + 2*10*5 + 7*5 - 25/2 + 5/2
+      = 1400 - 500 + 100 + 35 - 12 + 2 = 1025 (integer halves).
+    t(3, 4, 1) = 4*3*3 = 36, t(3, 4, 2) = 3*4*4 = 48; m(3, 4, 5) = 120;
+    s(10) = 22000."""
+    import hcore_amd
+    cases = [(("q", 1024, 1000), 1383386978), (("q", 1024, 187), 67641079),
+             (("q", 500, 64), 3985600), (("c", 70), 111894), (("o", 10, 7, 5, 1), 1155),
+             (("o", 10, 7, 5, 2), 1025), (("t", 3, 4, 1), 36), (("t", 3, 4, 2), 48),
+             (("m", 3, 4, 5), 120), (("s", 10), 22000)]
+    for args, want in cases:
+        assert orc.flop_counts(*args) == want, args
+        assert hcore_amd.flop_counts(*args) == want, args
+
+
 # --------------------------------------------------------------- planted mistakes
 def _pins_fail(orc, mutate, case=(3, 128, 24, 24, 40, -1.0, 1.0, 0, 1e-8)):
     seed, M, ra, rb, rc, alpha, beta, rk, acc = case
