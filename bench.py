@@ -134,14 +134,44 @@ def single_tile_leg(reps):
     args = (-1.0, AU, AV, BU, BV, 1.0, CU, CV)
     E = CU @ CV.T - (AU @ AV.T) @ (BU @ BV.T).T
     out = {}
-    for name, dev in (("gpu_host_pointers_us", False), ("gpu_device_pointers_us", True)):
-        hc.dgemm(*args, maxrk=250, acc=1e-8, device=dev)
+    # host pointers: the drop-in call as a HiCMA codelet makes it (staging included)
+    hc.dgemm(*args, maxrk=250, acc=1e-8)
+    t = time.perf_counter()
+    for _ in range(reps):
+        U, V, r = hc.dgemm(*args, maxrk=250, acc=1e-8)
+    out["gpu_host_pointers_us"] = (time.perf_counter() - t) / reps * 1e6
+    out["rank"] = r
+    out["rel_err"] = float(np.linalg.norm(U @ V.T - E) / np.linalg.norm(E))
+    # device pointers: factors resident in HBM, C restored from a pristine copy
+    # (device-to-device) before each call; only the HCORE_dgemm call is timed
+    import ctypes as C
+    L = hc.lib()
+    M, K = AU.shape[0], CU.shape[1] + AU.shape[1]
+    D = {k: hc.DeviceBuffer.from_numpy(v) for k, v in (("AU", AU), ("AV", AV), ("BU", BU),
+                                                         ("BV", BV))}
+    C0U, C0V = hc.DeviceBuffer.from_numpy(CU, K), hc.DeviceBuffer.from_numpy(CV, K)
+    CUd, CVd = hc.DeviceBuffer(M, K), hc.DeviceBuffer(M, K)
+    dp = lambda b: C.cast(C.c_void_p(b.ptr), C.POINTER(C.c_double))
+    ra, rb = np.array([AU.shape[1]], dtype=np.float64), np.array([BU.shape[1]], dtype=np.float64)
+    rcv = np.zeros(1)
+    fl = hc.FlopCounter()
+    tot = 0.0
+    for it in range(reps + 1):
+        CUd.copy_from(C0U)
+        CVd.copy_from(C0V)
+        rcv[0] = CU.shape[1]
         t = time.perf_counter()
-        for _ in range(reps):
-            U, V, r = hc.dgemm(*args, maxrk=250, acc=1e-8, device=dev)
-        out[name] = (time.perf_counter() - t) / reps * 1e6
-        out["rank"] = r
-        out["rel_err"] = float(np.linalg.norm(U @ V.T - E) / np.linalg.norm(E))
+        L.HCORE_dgemm(111, 112, M, M, -1.0, dp(D["AU"]), dp(D["AV"]), hc._ptr(ra), M,
+                      dp(D["BU"]), dp(D["BV"]), hc._ptr(rb), M, 1.0, dp(CUd), dp(CVd),
+                      hc._ptr(rcv), M, 0, 250, 1e-8, None, C.byref(fl))
+        if it:
+            tot += time.perf_counter() - t
+    out["gpu_device_pointers_us"] = tot / reps * 1e6
+    rd = int(rcv[0])
+    out["rel_err_device"] = float(np.linalg.norm(CUd.numpy(rd) @ CVd.numpy(rd).T - E) /
+                                  np.linalg.norm(E))
+    for b in list(D.values()) + [C0U, C0V, CUd, CVd]:
+        b.free()
     oracle.set_blas_threads(1)
     oracle.dgemm(*args, maxrk=250, acc=1e-8)
     t = time.perf_counter()
@@ -150,9 +180,115 @@ def single_tile_leg(reps):
     out["cpu_oracle_1core_us"] = (time.perf_counter() - t) / reps * 1e6
     out["oracle_rank"] = ro
     out["reps"] = reps
-    out["note"] = ("device-pointer time includes the host-side copies of the factors to and "
-                   "from device buffers made by the Python front end")
+    out["note"] = ("host pointers: staging through pinned memory and PCIe included; device "
+                   "pointers: the call alone on HBM-resident factors")
     return out
+
+
+def batch2048_leg(steps, warmup):
+    """The 512-update batch on nb = 2048 tiles (ranks 32-256 per tile, stacked
+    ranks to 506, acc 1e-8, maxrank 512): graph replays of one plan, the
+    reference's flop accounting with the plan's ranks."""
+    import hcore_amd as hc
+    from hcore_amd.tiles import TrailingStep
+    t = time.time()
+    step = TrailingStep.batch_512()
+    t_gen = time.time() - t
+    nb = step.nb
+    ra, rb, rc = step.ranks()
+    K = ra + rc
+    B = hc.DeviceBuffer.from_numpy
+    panel = {i: (B(u), B(v)) for i, (u, v) in step.panel.items()}
+    CUd = [B(c[0], k) for c, k in zip(step.C, K)]
+    CVd = [B(c[1], k) for c, k in zip(step.C, K)]
+    OUd = [hc.DeviceBuffer(nb, k) for k in K]
+    OVd = [hc.DeviceBuffer(nb, k) for k in K]
+    plan = hc.Plan(nb, hc.FP64)
+    plan.add_dgemm([panel[i][0].ptr for i, j in step.pairs], [panel[i][1].ptr for i, j in step.pairs],
+                   ra, nb, [panel[j][0].ptr for i, j in step.pairs],
+                   [panel[j][1].ptr for i, j in step.pairs], rb, nb,
+                   [t.ptr for t in CUd], [t.ptr for t in CVd], rc, nb, -1.0, 1.0, 0, 512, 1e-8,
+                   CUout=[t.ptr for t in OUd], CVout=[t.ptr for t in OVd], ldco=nb)
+    plan.finalize()
+    stream = hc.Stream()
+    ms = graph_ms(plan, stream, steps, warmup)
+    assert np.all(plan.status() == 0)
+    # per-phase durations with the lanes one after another (HIP events)
+    plan.set_timing(2)
+    plan.execute(stream)
+    stream.synchronize()
+    ph_ms, ph_n = plan.phase_times()
+    plan.set_timing(False)
+    phases = {}
+    for l in range(ph_ms.shape[0]):
+        row = {PHASES[i]: float(ph_ms[l, i] / ph_n[l, i]) for i in range(len(PHASES))
+               if ph_n[l, i] > 0}
+        if row:
+            phases["lane%d" % l] = row
+            log("  batch 2048 lane %d: %s" % (l, " ".join("%s %.2f" % kv for kv in row.items())))
+    r = plan.ranks().astype(np.int64)
+    Kl = K.astype(np.int64)
+    fc = hc.flop_counts
+    f = (sum(2 * q_flops(nb, int(k)) for k in Kl) + int(np.sum(4 * ra.astype(np.int64) * rb * nb)) +
+         int(np.sum(22 * Kl ** 3)) + int(np.sum(r * Kl)) +
+         sum(fc('o', nb, int(x), int(k), 1) + fc('o', int(x), nb, int(k), 2) for x, k in zip(r, Kl)))
+    plan.destroy()
+    for b in CUd + CVd + OUd + OVd + [x for pr in panel.values() for x in pr]:
+        b.free()
+    res = {"updates": len(step.pairs), "nb": nb, "K_min": int(K.min()), "K_max": int(K.max()),
+           "ms_per_step": ms, "updates_per_s": len(step.pairs) / (ms / 1e3),
+           "tflops": f / (ms / 1e3) / 1e12, "flops_per_step": f, "generate_s": t_gen,
+           "phase_ms_serialized": phases}
+    log("batch 512 x nb 2048 (K %d-%d): %.2f ms/step, %.0f updates/s, %.2f TF/s" %
+        (K.min(), K.max(), ms, res["updates_per_s"], res["tflops"]))
+    return res
+
+
+def tlr_leg(bigM, nb):
+    """Generation, compression and TLR Cholesky of the covariance problem
+    (bigM Morton-sorted points, exp(-d/0.1) + 1e-4 I, acc 1e-8, compression
+    maxrank nb/2, recompression maxrank nb), all on the GPU; the factor is
+    checked on sampled entries of L L^T against the matrix (numpy, from the
+    library's points)."""
+    import hcore_amd as hc
+    from hcore_amd.tlr import TlrMatrix
+    acc = 1e-8
+    M = TlrMatrix(bigM, nb, 2024, maxrank=nb // 2, acc=acc, cap=nb)
+    t = time.time()
+    M.generate()
+    t_gen = time.time() - t
+    r0 = M.ranks.copy()
+    nt = M.nt
+    low = [(i, k) for k in range(nt) for i in range(k + 1, nt)]
+    t = time.time()
+    st, info, smax = M.factor(maxrk=nb, acc=acc)
+    t_fac = time.time() - t
+    assert st == 0 and info == 0, (st, info)
+    fl = M.flops
+    rng = np.random.default_rng(7)
+    log("tlr cholesky %d points nb %d: generate %.2f s, factor %.2f s" % (bigM, nb, t_gen, t_fac))
+    rows = sorted(int(g) for g in rng.choice(bigM, 6, replace=False))
+    P = hc.covariance_points(bigM, 2024)
+    Lr = {g: M.row(g) for g in rows}
+    worst = 0.0
+    for a in rows:
+        for b in rows:
+            d = np.hypot(*(P[a] - P[b]))
+            ref = np.exp(-d / 0.1) + (1e-4 if a == b else 0.0)
+            worst = max(worst, abs(Lr[a] @ Lr[b] - ref))
+    M.free()
+    res = {"points": bigM, "nb": nb, "tiles": nt, "gemm_updates": nt * (nt - 1) * (nt - 2) // 6,
+           "syrk_updates": nt * (nt - 1) // 2, "generate_s": t_gen, "factor_s": t_fac,
+           "factor_gflops": (fl.update + fl.syrk + fl.trsm + fl.potrf) / t_fac / 1e9,
+           "rank_max_generated": int(max(r0[i, k] for i, k in low)),
+           "rank_mean_generated": float(np.mean([r0[i, k] for i, k in low])),
+           "rank_max_per_step": [int(x) for x in smax[:-1]],
+           "sampled_LLt_max_abs_err": worst}
+    log("tlr cholesky %d points nb %d: generate %.2f s, factor %.2f s (%.0f GF/s), ranks mean %.1f "
+        "max %d, sampled |LL^T - A| %.2e" % (bigM, nb, t_gen, t_fac, res["factor_gflops"],
+                                             res["rank_mean_generated"],
+                                             res["rank_max_generated"], worst))
+    return res
 
 
 def main():
@@ -167,6 +303,12 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--single-reps", type=int, default=200)
     ap.add_argument("--check", type=int, default=24, help="updates checked against the oracle")
+    ap.add_argument("--tlr", type=int, default=0,
+                    help="also run the TLR Cholesky of a covariance problem of this many "
+                         "points (tiles of --tlr-nb; 32768 is the 64 x 64-tile stream)")
+    ap.add_argument("--tlr-nb", type=int, default=512)
+    ap.add_argument("--batch2048", action="store_true",
+                    help="also time the 512-update batch on nb = 2048 tiles (stacked ranks to 506)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on exactly one GPU (no sharding, no collectives); "
@@ -207,8 +349,10 @@ def main():
     plan.add_dsyrk([panel[i][0].ptr for i in step.diag_idx], [panel[i][1].ptr for i in step.diag_idx],
                    [step.panel[i][0].shape[1] for i in step.diag_idx], nb,
                    [t.ptr for t in Dd], nb, -1.0)
+    t = time.perf_counter()
     plan.finalize()
-    log("plan workspace %.2f GB" % (plan.workspace_bytes() / 1e9))
+    finalize_ms = (time.perf_counter() - t) * 1e3
+    log("plan workspace %.2f GB, finalize %.1f ms" % (plan.workspace_bytes() / 1e9, finalize_ms))
     stream = hc.Stream()
 
     # ---- one checked execution ------------------------------------------------
@@ -270,8 +414,10 @@ def main():
         ns * 8 * 2 * nb * nb
 
     # ---- timed region: the step replayed through its captured hipGraph --------
+    t = time.perf_counter()
     plan.execute(stream, graph=True)     # capture + first replay
     stream.synchronize()
+    capture_ms = (time.perf_counter() - t) * 1e3
     for _ in range(args.warmup):
         plan.execute(stream, graph=True)
     stream.synchronize()
@@ -286,6 +432,17 @@ def main():
     ms_per_step = e0.elapsed_ms(e1) / args.steps
     updates = n + ns
     value = updates / (ms_per_step / 1e3)
+    # the same step issued eagerly on the plan's streams (no graph), for comparison
+    for _ in range(args.warmup):
+        plan.execute(stream)
+    stream.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        plan.execute(stream)
+    e1.record(stream)
+    stream.synchronize()
+    eager_conc_ms = e0.elapsed_ms(e1) / args.steps
+    log("eager (4 streams, no graph) %.3f ms/step" % eager_conc_ms)
 
     # ---- per-kernel durations: the same step executed eagerly, lanes one after
     # another on the stream, with a HIP event pair around every launch, so that
@@ -425,6 +582,9 @@ def main():
                "runs_s": [float(x) for x in times]}
         log("cpu baseline: %.3f s/step on %d threads" % (tmed, ncores))
 
+    tlr = tlr_leg(args.tlr, args.tlr_nb) if args.tlr else None
+    b2048 = batch2048_leg(max(3, args.steps // 4), 1) if args.batch2048 else None
+
     out = {
         "metric": "TLR GEMM updates/s at rel.err<=1e-8: mixed-prec vs FP64 GPU, % roofline",
         "value": value,
@@ -443,11 +603,16 @@ def main():
                                "acc=1e-8, maxrank=512",
                    "grid": args.grid, "nb": nb, "updates_per_step": updates,
                    "precision": "fp64", "execution": "hipGraph replay of the step",
-                   "serialized_lanes_ms_per_step": eager_ms},
+                   "serialized_lanes_ms_per_step": eager_ms,
+                   "eager_streams_ms_per_step": eager_conc_ms,
+                   "plan_finalize_ms": finalize_ms,
+                   "graph_capture_and_first_replay_ms": capture_ms},
         "roofline": roof,
         "reduced_precision": mixed,
         "single_tile": single,
         "cpu_baseline": cpu,
+        "tlr_cholesky": tlr,
+        "batch_nb2048": b2048,
         "gflops": F_total / (ms_per_step / 1e3) / 1e9,
         "step_roofline_frac": max(F_total / (ms_per_step / 1e3) / FP64_PEAK,
                                   b_unique / (ms_per_step / 1e3) / HBM_PEAK),

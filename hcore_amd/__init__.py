@@ -138,6 +138,7 @@ def lib():
         L.HCORE_hip_event_record.argtypes = [_vp, _vp]
         L.HCORE_hip_event_elapsed_ms.argtypes = [_vp, _vp, C.POINTER(C.c_float)]
         L.HCORE_hip_device_info.argtypes = [C.c_char_p, i, _ip]
+        L.HCORE_hip_mem_info.argtypes = [C.POINTER(sz), C.POINTER(sz)]
         _lib = L
     return _lib
 
@@ -543,6 +544,13 @@ class Event:
 
 def device_synchronize():
     _check(lib().HCORE_hip_device_synchronize(), "device_synchronize")
+
+
+def mem_info():
+    """(free, total) device memory in bytes."""
+    f, t = C.c_size_t(0), C.c_size_t(0)
+    _check(lib().HCORE_hip_mem_info(C.byref(f), C.byref(t)), "HCORE_hip_mem_info")
+    return f.value, t.value
 
 
 def device_info():

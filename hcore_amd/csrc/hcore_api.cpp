@@ -2642,6 +2642,16 @@ int HCORE_hip_event_elapsed_ms(void* start, void* stop, float* ms) {
   HCHECK(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
   return HCORE_HIP_OK;
 }
+int HCORE_hip_mem_info(size_t* free_bytes, size_t* total_bytes) {
+  int st = ctx_ensure();
+  if (st) return st;
+  size_t f = 0, t = 0;
+  HCHECK(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return HCORE_HIP_OK;
+}
+
 int HCORE_hip_device_info(char* name, int len, int* num_cu) {
   int st = ctx_ensure();
   if (st) return st;

@@ -48,8 +48,10 @@ class TrailingStep:
     """
 
     def __init__(self, grid=64, nb=1024, decay=10.0, key="trailing", diag=True,
-                 max_updates=None):
+                 max_updates=None, rank_of=None):
         self.grid, self.nb, self.decay, self.key = grid, nb, decay, key
+        if rank_of is None:  # rank by distance from the diagonal
+            rank_of = lambda i, j, role: cholesky_rank(i - j)
         self.pairs = [(i, j) for i in range(1, grid) for j in range(1, i)]
         if max_updates is not None:
             self.pairs = self.pairs[:max_updates]
@@ -67,12 +69,22 @@ class TrailingStep:
         from concurrent.futures import ThreadPoolExecutor
         from threadpoolctl import threadpool_limits
         with threadpool_limits(1), ThreadPoolExecutor(max(1, min(16, os.cpu_count() or 1))) as ex:
-            pan = list(ex.map(lambda i: lowrank_tile(nb, cholesky_rank(i), decay,
+            pan = list(ex.map(lambda i: lowrank_tile(nb, rank_of(i, 0, "panel"), decay,
                                                      (key, i, 0, "panel")), rows))
-            self.C = list(ex.map(lambda ij: lowrank_tile(nb, cholesky_rank(ij[0] - ij[1]), decay,
+            self.C = list(ex.map(lambda ij: lowrank_tile(nb, rank_of(ij[0], ij[1], "C"), decay,
                                                          (key, ij[0], ij[1], "C")), self.pairs))
             self.D = list(ex.map(dense, self.diag_idx))
         self.panel = dict(zip(rows, pan))
+
+    @classmethod
+    def batch_512(cls, key="cfg3"):
+        """512 updates on nb = 2048 tiles with ranks drawn uniformly from
+        [32, 256] per tile (keyed), sigma_k = 10^(-12 k / r): stacked ranks up
+        to 512.  Panel tiles of a 34-tile column, the first 512 pairs."""
+        def rank(i, j, role):
+            return int(rng(key, i, j, role, "rank").integers(32, 257))
+        return cls(grid=34, nb=2048, decay=12.0, key=key, diag=False, max_updates=512,
+                   rank_of=rank)
 
     def ranks(self):
         ra = np.array([self.panel[i][0].shape[1] for (i, j) in self.pairs], dtype=np.int32)

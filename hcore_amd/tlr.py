@@ -28,7 +28,7 @@ class TlrMatrix:
         self.flops = FlopCounter()
 
     def tile_ptr(self, buf, i, k):
-        return buf[k].ptr + 8 * (i - k - 1) * self.cap * self.nb
+        return int(buf[int(k)].ptr) + 8 * (int(i) - int(k) - 1) * self.cap * self.nb
 
     def pointer_tables(self):
         nt = self.nt
@@ -84,10 +84,30 @@ class TlrMatrix:
         return U @ V.T
 
     def factors(self, i, k):
+        """Host copies of tile (i, k)'s factors (i > k), its rank columns only."""
+        from . import lib
         r = int(self.ranks[i, k])
-        off = (i - k - 1) * self.cap
-        return (np.asfortranarray(self.Ucol[k].numpy()[:, off:off + r]),
-                np.asfortranarray(self.Vcol[k].numpy()[:, off:off + r]))
+        out = []
+        for buf in (self.Ucol, self.Vcol):
+            x = np.zeros((self.nb, r), order="F")
+            if r:
+                _check(lib().HCORE_hip_memcpy_d2h(x.ctypes.data, self.tile_ptr(buf, i, k),
+                                                  x.nbytes), "d2h")
+            out.append(x)
+        return out[0], out[1]
+
+    def row(self, g):
+        """Row g of the lower triangle (after factor(): of L), from the
+        factors of the tiles in its tile row."""
+        nb = self.nb
+        i, li = divmod(int(g), nb)
+        out = np.zeros(self.bigM)
+        for k in range(i):
+            U, V = self.factors(i, k)
+            out[k * nb:(k + 1) * nb] = V @ U[li]
+        d = self.D[i].numpy()
+        out[i * nb:i * nb + li + 1] = d[li, :li + 1]
+        return out
 
     def dense_lower(self, lower_diag=True):
         """The lower triangle as one dense host matrix (diagonal tiles

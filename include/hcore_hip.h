@@ -240,6 +240,8 @@ int HCORE_hip_event_record(void *event, void *stream);
 int HCORE_hip_event_elapsed_ms(void *start, void *stop, float *ms);
 /* Device name and number of compute units of the current device. */
 int HCORE_hip_device_info(char *name, int len, int *num_cu);
+/* Free and total device memory of the library's GPU (bytes). */
+int HCORE_hip_mem_info(size_t *free_bytes, size_t *total_bytes);
 
 #ifdef __cplusplus
 }

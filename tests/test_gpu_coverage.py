@@ -360,3 +360,24 @@ def test_concurrent_single_calls_match_sequential(hc):
         ref = seq[u % len(data)]
         assert got[2] == ref[2]
         assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+
+
+def test_single_calls_hold_no_growing_state(hc):
+    """Thousands of small HCORE_dgemm calls (host pointers, as a runtime's
+    worker threads make them) leave host RSS and free device memory flat:
+    a call creates no events or streams and allocates nothing once the
+    thread's buffers have grown to the tile size."""
+    import psutil
+    from test_oracle import _random_case
+    AU, AV, BU, BV, CU, CV = _random_case(7, 16, 3, 3, 3)
+    _say("case: 3000 nb 16 calls, host RSS and device memory")
+    for _ in range(200):
+        hc.dgemm(-1.0, AU, AV, BU, BV, 1.0, CU, CV, maxrk=16, acc=1e-8)
+    proc = psutil.Process()
+    rss0, free0 = proc.memory_info().rss, hc.mem_info()[0]
+    for _ in range(3000):
+        hc.dgemm(-1.0, AU, AV, BU, BV, 1.0, CU, CV, maxrk=16, acc=1e-8)
+    rss1, free1 = proc.memory_info().rss, hc.mem_info()[0]
+    _say("   RSS %+d KiB, device free %+d KiB" % ((rss1 - rss0) >> 10, (free1 - free0) >> 10))
+    assert rss1 - rss0 < 8 << 20
+    assert free0 - free1 < 8 << 20
