@@ -643,7 +643,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
           q.push_back({Us, W + o_tA[u], W + o_TA[u], SA, K});
           q.push_back({Vs, W + o_tB[u], W + o_TB[u], SA + al((size_t)K * K), K});
           recQ(q);
-        } else if (g_ctx.qr_reg && M <= kQrRegMaxNb) {  // register panels + in-place trailing updates
+        } else if (g_ctx.qr_reg && M <= kQrFusedMaxNb) {  // register panels + in-place trailing updates
           d.qrp.push_back({Us, W + o_tA[u], W + o_TA[u], nullptr, K});
           d.qrp.push_back({Vs, W + o_tB[u], W + o_TB[u], nullptr, K});
           recQ(d.qrp);

@@ -167,9 +167,10 @@ void launch_geqrf(const QrProb<T>* probs, int n, int nb, hipStream_t s);
 // false (and launches nothing) for a shape beyond those limits.
 constexpr int kQrRegMaxK = 32;
 constexpr int kQrRegMaxNb = 1024;  // rows held by the largest register QR shape (512 x 2)
+constexpr int kQrFusedMaxNb = 2048;  // rows of the largest fused-QR shape (512 x 4)
 template <typename T>
 bool launch_qr_reg(const QrProb<T>* probs, int n, int nb, int kmax, hipStream_t s);
-// whole-stack QR in one workgroup (nb <= kQrRegMaxNb): register panels of 16
+// whole-stack QR in one workgroup (nb <= kQrFusedMaxNb): register panels of 16
 // columns (32 at nb <= 256), each applied to the trailing columns in place;
 // false for nb beyond the limit
 template <typename T>
@@ -206,7 +207,7 @@ void launch_core(const CoreProb<T>* probs, int n, int variant, int kmax, int rk,
 template <typename T>
 void launch_apply(const ApplyProb<T>* probs, int n, int nb, hipStream_t s);
 // blocked back substitution of the compact-WY solve, 64 columns per workgroup
-constexpr int kWyMaxK = 256;
+constexpr int kWyMaxK = 512;
 template <typename T>
 void launch_wy_solve(const WySolveProb<T>* probs, int n, int kmax, int rmax, hipStream_t s);
 

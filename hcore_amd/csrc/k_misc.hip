@@ -152,50 +152,67 @@ void launch_apply(const ApplyProb<T>* probs, int n, int nb, hipStream_t s) {
 // for the panels b from last to first,
 //   M_b = T_bb (B_b - S(b, >b) M_{>b}),  S = Y^T Y,  B = Y1^T Z (in M on entry),
 // which is M = T B for the forward compact-WY T (T^{-1} = diag(1/tau) +
-// striu(S)).  A workgroup holds 64 columns of M in LDS; wave w computes rows
-// 2w, 2w+1 of each panel, lane j column j.  The panel's rows of S (contiguous
-// in the stored lower triangle) and its kPB x kPB factor are staged in LDS by
-// coalesced loads; the next panel's are fetched into registers while the
-// current one is solved.
+// striu(S)).  A workgroup holds CW columns of M in LDS (64 for K <= 256, 16
+// for K <= 512, where 64 columns of K rows would not fit); wave w computes
+// rows 2w, 2w+1 of each panel, lane j column j % CW, the 64 / CW lanes of a
+// column splitting the panel's dot products and summing them by shuffles.
+// The panel's rows of S (contiguous in the stored lower triangle) and its
+// kPB x kPB factor are staged in LDS by coalesced loads; the next panel's are
+// fetched into registers while the current one is solved.
 // ------------------------------------------------------------------------------
-constexpr int kWyLd = 65;  // row stride of M in LDS: column-wise fills spread over the banks
-template <typename T>
+template <int CW>
+struct WyShape {
+  static constexpr int LD = CW + 1;                  // row stride of M in LDS
+  static constexpr int KMAX = CW == 64 ? 256 : 512;  // rows the staging covers
+  static constexpr int SPT = KMAX / kThreads;        // S entries per thread and row
+};
+
+template <typename T, int CW>
 __global__ __launch_bounds__(kThreads) void k_wy_solve(const WySolveProb<T>* __restrict__ probs,
                                                        int ncb) {
-  static_assert(kPB == 8 && kThreads == 256 && kWyMaxK <= kThreads,
-                "two rows of a panel per wavefront; one S element per thread and row");
+  using Sh = WyShape<CW>;
+  constexpr int LD = Sh::LD, KM = Sh::KMAX, SPT = Sh::SPT, SPLIT = 64 / CW;
+  static_assert(kPB == 8 && kThreads == 256 && KM % kThreads == 0 && 64 % CW == 0,
+                "two rows of a panel per wavefront");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const WySolveProb<T> P = probs[blockIdx.x / ncb];
   if (*P.status != ST_OK) return;
   const int r = *P.rank;
-  const int j0 = (blockIdx.x % ncb) * 64;
+  const int j0 = (blockIdx.x % ncb) * CW;
   if (j0 >= r) return;
   const int K = P.K;
-  const int nc = min(64, r - j0);
-  T* Mb = reinterpret_cast<T*>(smem_raw);  // K x 64, row c at Mb[c * kWyLd]
-  T* RH = Mb + (size_t)K * kWyLd;           // kPB x 64 right-hand sides
-  T* Sb = RH + kPB * 64;                    // kPB x kWyMaxK: S(hi + i, base + a) at Sb[a * kWyMaxK + i]
-  T* Tl = Sb + kPB * kWyMaxK;               // kPB x kPB panel factor
+  const int nc = min(CW, r - j0);
+  T* Mb = reinterpret_cast<T*>(smem_raw);  // K x CW, row c at Mb[c * LD]
+  T* RH = Mb + (size_t)K * LD;              // kPB x CW right-hand sides
+  T* Sb = RH + kPB * CW;                    // kPB x KM: S(hi + i, base + a) at Sb[a * KM + i]
+  T* Tl = Sb + kPB * KM;                    // kPB x kPB panel factor
   const int tid = threadIdx.x;
   const int lane = tid & 63;
+  const int col = lane % CW, part = lane / CW;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  // columns j0 .. j0 + 63 of M, read along its columns (consecutive rows per thread)
-  for (int e = tid; e < K * 64; e += kThreads) {
+  // columns j0 .. j0 + CW - 1 of M, read along its columns (consecutive rows per thread)
+  for (int e = tid; e < K * CW; e += kThreads) {
     const int c = e % K, j = e / K;
-    Mb[c * kWyLd + j] = j < nc ? P.M[c + (size_t)(j0 + j) * K] : T(0);
+    Mb[c * LD + j] = j < nc ? P.M[c + (size_t)(j0 + j) * K] : T(0);
   }
   const int nblk = (K + kPB - 1) / kPB;
-  T pre[kPB], pret = T(0);
+  T pre[kPB][SPT], pret = T(0);
   auto fetch = [&](int b) {
     const int base = b * kPB, hi = min(K, base + kPB), len = K - hi;
 #pragma unroll
     for (int a = 0; a < kPB; ++a)
-      pre[a] = (tid < len && base + a < hi) ? P.S[(size_t)(base + a) * K + hi + tid] : T(0);
+#pragma unroll
+      for (int q = 0; q < SPT; ++q) {
+        const int t = tid + q * kThreads;
+        pre[a][q] = (t < len && base + a < hi) ? P.S[(size_t)(base + a) * K + hi + t] : T(0);
+      }
     if (tid < kPB * kPB) pret = P.Tp[(size_t)b * kPB * kPB + tid];
   };
   auto stash = [&]() {
 #pragma unroll
-    for (int a = 0; a < kPB; ++a) Sb[a * kWyMaxK + tid] = pre[a];
+    for (int a = 0; a < kPB; ++a)
+#pragma unroll
+      for (int q = 0; q < SPT; ++q) Sb[a * KM + tid + q * kThreads] = pre[a][q];
     if (tid < kPB * kPB) Tl[tid] = pret;
   };
   fetch(nblk - 1);
@@ -206,33 +223,43 @@ __global__ __launch_bounds__(kThreads) void k_wy_solve(const WySolveProb<T>* __r
     if (b > 0) fetch(b - 1);  // in flight while this panel is solved
     {
       const int a0 = 2 * w, a1 = 2 * w + 1;
-      const T* S0 = Sb + a0 * kWyMaxK;
-      const T* S1 = Sb + a1 * kWyMaxK;
+      const T* S0 = Sb + a0 * KM;
+      const T* S1 = Sb + a1 * KM;
       T p0 = T(0), p1 = T(0), q0 = T(0), q1 = T(0);
-      int i = 0;
-      for (; i + 1 < len; i += 2) {
-        const T m0 = Mb[(hi + i) * kWyLd + lane], m1 = Mb[(hi + i + 1) * kWyLd + lane];
+      int i = part;
+      for (; i + SPLIT < len; i += 2 * SPLIT) {
+        const T m0 = Mb[(hi + i) * LD + col], m1 = Mb[(hi + i + SPLIT) * LD + col];
         p0 = fma(S0[i], m0, p0);
         q0 = fma(S1[i], m0, q0);
-        p1 = fma(S0[i + 1], m1, p1);
-        q1 = fma(S1[i + 1], m1, q1);
+        p1 = fma(S0[i + SPLIT], m1, p1);
+        q1 = fma(S1[i + SPLIT], m1, q1);
       }
       if (i < len) {
-        const T m0 = Mb[(hi + i) * kWyLd + lane];
+        const T m0 = Mb[(hi + i) * LD + col];
         p0 = fma(S0[i], m0, p0);
         q0 = fma(S1[i], m0, q0);
       }
-      if (base + a0 < hi) RH[a0 * 64 + lane] = Mb[(base + a0) * kWyLd + lane] - (p0 + p1);
-      if (base + a1 < hi) RH[a1 * 64 + lane] = Mb[(base + a1) * kWyLd + lane] - (q0 + q1);
+      T ps = p0 + p1, qs = q0 + q1;
+#pragma unroll
+      for (int off = CW; off < 64; off <<= 1) {  // the SPLIT partial sums of a column
+        ps += __shfl_xor(ps, off, 64);
+        qs += __shfl_xor(qs, off, 64);
+      }
+      if (part == 0) {
+        if (base + a0 < hi) RH[a0 * CW + col] = Mb[(base + a0) * LD + col] - ps;
+        if (base + a1 < hi) RH[a1 * CW + col] = Mb[(base + a1) * LD + col] - qs;
+      }
     }
     __syncthreads();
+    if (part == 0) {
 #pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      const int a = 2 * w + e, c = base + a;
-      if (c < hi) {
-        T v = T(0);
-        for (int cc = a; cc < hi - base; ++cc) v = fma(Tl[a + cc * kPB], RH[cc * 64 + lane], v);
-        Mb[c * kWyLd + lane] = v;
+      for (int e = 0; e < 2; ++e) {
+        const int a = 2 * w + e, c = base + a;
+        if (c < hi) {
+          T v = T(0);
+          for (int cc = a; cc < hi - base; ++cc) v = fma(Tl[a + cc * kPB], RH[cc * CW + col], v);
+          Mb[c * LD + col] = v;
+        }
       }
     }
     __syncthreads();
@@ -243,26 +270,36 @@ __global__ __launch_bounds__(kThreads) void k_wy_solve(const WySolveProb<T>* __r
   }
   for (int e = tid; e < K * nc; e += kThreads) {
     const int c = e % K, j = e / K;
-    P.M[c + (size_t)(j0 + j) * K] = Mb[c * kWyLd + j];
+    P.M[c + (size_t)(j0 + j) * K] = Mb[c * LD + j];
   }
 }
 
-template <typename T>
+template <typename T, int CW>
 static size_t wy_smem(int kmax) {
-  return sizeof(T) * ((size_t)kmax * kWyLd + kPB * 64 + kPB * kWyMaxK + kPB * kPB);
+  using Sh = WyShape<CW>;
+  return sizeof(T) * ((size_t)kmax * Sh::LD + kPB * CW + kPB * Sh::KMAX + kPB * kPB);
+}
+
+template <typename T, int CW>
+static void launch_wy_solve_cw(const WySolveProb<T>* probs, int n, int kmax, int rmax,
+                               hipStream_t s) {
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wy_solve<T, CW>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)wy_smem<T, CW>(WyShape<CW>::KMAX));
+    attr = true;
+  }
+  if (n <= 0 || rmax <= 0) return;
+  const int ncb = (rmax + CW - 1) / CW;
+  const size_t sm = wy_smem<T, CW>(kmax);
+  hipLaunchKernelGGL((k_wy_solve<T, CW>), dim3(n * ncb), dim3(kThreads), sm, s, probs, ncb);
 }
 
 template <typename T>
 void launch_wy_solve(const WySolveProb<T>* probs, int n, int kmax, int rmax, hipStream_t s) {
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wy_solve<T>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)wy_smem<T>(kWyMaxK));
-    attr = true;
-  }
-  if (n <= 0 || rmax <= 0) return;
-  const int ncb = (rmax + 63) / 64;
-  hipLaunchKernelGGL(k_wy_solve<T>, dim3(n * ncb), dim3(kThreads), wy_smem<T>(kmax), s, probs, ncb);
+  if (kmax <= WyShape<64>::KMAX) launch_wy_solve_cw<T, 64>(probs, n, kmax, rmax, s);
+  else launch_wy_solve_cw<T, 16>(probs, n, kmax, rmax, s);
 }
 
 // ------------------------------------------------------------------------------
@@ -292,6 +329,7 @@ void launch_convert(const ConvProb* probs, int n, int nmax, bool to_double, hipS
 template <typename T>
 void prepare_kernels() {
   launch_wy_solve<T>(nullptr, 0, 1, 1, nullptr);
+  launch_wy_solve<T>(nullptr, 0, kWyMaxK, 1, nullptr);
   launch_geqrf<T>(nullptr, 0, 128, nullptr);
   launch_geqrf<T>(nullptr, 0, 256, nullptr);
   launch_geqrf<T>(nullptr, 0, 512, nullptr);

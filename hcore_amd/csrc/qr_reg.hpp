@@ -339,13 +339,14 @@ bool launch_qr_reg(const QrProb<T>* probs, int n, int nb, int kmax, hipStream_t 
 template <typename T>
 bool launch_qr_fused(const QrProb<T>* probs, int n, int nb, hipStream_t s) {
   if (n <= 0) return true;
-  if (nb > kQrRegMaxNb) return false;
+  if (nb > kQrFusedMaxNb) return false;
   constexpr int NT = kQrfThreads, PB = kQrfPanel;
+  static_assert(kQrFusedMaxNb <= 4 * NT, "fused QR shapes");
   const int rpt = (nb + NT - 1) / NT;
-  // nb <= 2 NT (the plans route stacks of at most 1024 rows here)
   if (nb <= 256) hipLaunchKernelGGL((k_qr_fused<T, 256, 1, 32>), dim3(n), dim3(256), 0, s, probs, nb);
   else if (rpt <= 1) hipLaunchKernelGGL((k_qr_fused<T, NT, 1, PB>), dim3(n), dim3(NT), 0, s, probs, nb);
-  else hipLaunchKernelGGL((k_qr_fused<T, NT, 2, PB>), dim3(n), dim3(NT), 0, s, probs, nb);
+  else if (rpt <= 2) hipLaunchKernelGGL((k_qr_fused<T, NT, 2, PB>), dim3(n), dim3(NT), 0, s, probs, nb);
+  else hipLaunchKernelGGL((k_qr_fused<T, NT, 4, PB>), dim3(n), dim3(NT), 0, s, probs, nb);
   return true;
 }
 
