@@ -306,12 +306,11 @@ struct hcore_hip_plan {
   };
   struct Lane {
     int n = 0;
-    size_t o_stack = 0, o_qr = 0, o_apply = 0;
-    int n_qr = 0, n_apply = 0;
+    size_t o_stack = 0, o_qr = 0;
+    int n_qr = 0;
     hcore::TaskRef tF, tG, tT, tZ, tB, tX;  // batched GEMM phases
     int stack_cols = 0;  // largest 2rc+ra of the lane
     int kmax = 0;        // largest stacked rank of the lane
-    bool wy = false;     // reassembly by the compact-WY solve (kmax <= kWyMaxK)
     size_t o_wy = 0, o_qrr16 = 0, o_qrr = 0, o_qrp = 0;
     int n_qrr16 = 0, n_qrr = 0, n_qrp = 0;  // register QR for K <= 16, 17..32; panels
     bool gram = false;                      // T phase with the stacks' Gram products
@@ -402,14 +401,9 @@ static int plan_finalize_t(hcore_hip_plan* p) {
   // variants that need the K x K global scratch (work matrix, or the R^T
   // transposition of the register-resident kernels)
   auto global_core = core_needs_scratch;
-  // reassembly per lane: the compact-WY solve and GEMMs up to kWyMaxK,
-  // the per-panel reflector application beyond
-  bool lane_wy[3];
-  for (int l = 0; l < 3; ++l) {
-    int kmax = 0;
-    for (int u : lane_members[l]) kmax = std::max(kmax, Kv[u]);
-    lane_wy[l] = kmax <= kWyMaxK;
-  }
+  // reassembly: the compact-WY solve and GEMMs (every stacked rank the core
+  // kernels accept)
+  static_assert(kWyMaxK >= kCoreMaxK, "compact-WY reassembly for every stacked rank");
   // lane index as executed (empty lanes are dropped, so renumber)
   p->lane_of.assign(n, 0);
   {
@@ -502,7 +496,6 @@ static int plan_finalize_t(hcore_hip_plan* p) {
       TaskList tF, tG, tT, tZ, tB, tX;
       std::vector<WySolveProb<T>> wy;
       int kmax = 0;
-      bool wy_lane = false;
       std::vector<QrProb<T>> qr, qrr16, qrr, qrp;  // blocked / register single pass / register panels
       bool gram = false;  // the T phase holds k = nb Gram products
       std::vector<std::vector<CoreProb<T>>> core_b;
@@ -510,7 +503,6 @@ static int plan_finalize_t(hcore_hip_plan* p) {
       std::vector<ConvProb> cv_in, cv_out;
       int cv_max = 0;
       std::vector<int> core_kmax;
-      std::vector<ApplyProb<T>> ap;
     };
     constexpr int NV = CORE_NVARIANTS;
     std::vector<LaneDesc> ld;
@@ -584,8 +576,8 @@ static int plan_finalize_t(hcore_hip_plan* p) {
         add_tasks(d.tZ, (int)gp.size(), g); gp.push_back(g);
         recK({HCORE_GF(m), HCORE_GF(n), HCORE_GF(k), HCORE_GF(lda), HCORE_GF(ldb), HCORE_GF(ldc)});
         // register-resident QR (it also yields the Gram S and the panel factors)
-        const bool qreg = lane_wy[l] && g_ctx.qr_reg && K <= kQrRegMaxK && M <= kQrRegMaxNb;
-        if (lane_wy[l]) {
+        const bool qreg = g_ctx.qr_reg && K <= kQrRegMaxK && M <= kQrRegMaxNb;
+        {
           T* SA = W + o_S[u];
           T* SB = SA + al((size_t)K * K);
           T* MA = W + o_M[u];
@@ -678,19 +670,8 @@ static int plan_finalize_t(hcore_hip_plan* p) {
           if (dvr) rec(d.core_b[b], d.core_b[b].size() - 1, offsetof(CoreProb<T>, K), PK_K, u);
         }
         d.core_kmax[b] = std::max(d.core_kmax[b], K);
-        T* CUo = (T*)(p->CUo[u] ? p->CUo[u] : p->CU[u]);
-        T* CVo = (T*)(p->CVo[u] ? p->CVo[u] : p->CV[u]);
-        if (!lane_wy[l]) {
-          d.ap.push_back({Us, W + o_TA[u], W + o_Zu[u], CUo, d_rank + u, d_status + u, K, p->ldco});
-          d.ap.push_back({Vs, W + o_TB[u], W + o_Zv[u], CVo, d_rank + u, d_status + u, K, p->ldco});
-          if (dvr) {
-            rec(d.ap, d.ap.size() - 2, offsetof(ApplyProb<T>, K), PK_K, u);
-            rec(d.ap, d.ap.size() - 1, offsetof(ApplyProb<T>, K), PK_K, u);
-          }
-        }
         d.kmax = std::max(d.kmax, K);
       }
-      d.wy_lane = lane_wy[l];
     }
     // syrk (hcore_dsyrk.c:103, 144, 165-181)
     std::vector<GemmProb<T>> sp;
@@ -738,12 +719,10 @@ static int plan_finalize_t(hcore_hip_plan* p) {
       L.o_qrr16 = addv(d.qrr16); L.n_qrr16 = (int)d.qrr16.size();
       L.o_qrr = addv(d.qrr); L.n_qrr = (int)d.qrr.size();
       L.o_qrp = addv(d.qrp); L.n_qrp = (int)d.qrp.size();
-      L.o_apply = addv(d.ap); L.n_apply = (int)d.ap.size();
       L.tB = pk.add(d.tB);
       L.tX = pk.add(d.tX);
       L.o_wy = addv(d.wy); L.n_wy = (int)d.wy.size();
       L.kmax = d.kmax;
-      L.wy = d.wy_lane;
       L.core.clear();
       for (int b = 0; b < NV; ++b) {
         if (!d.core_b[b].empty())
@@ -995,20 +974,12 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
     phase(l, 7, ls, true, [&] { gemm(gp, L.tZ, ls); });
     }
     if (second) {
-    if (L.wy) {
-      phase(l, 6, ls, true, [&] {
-        gemm(gp, L.tB, ls);
-        launch_wy_solve<T>(reinterpret_cast<const WySolveProb<T>*>(D(L.o_wy)), L.n_wy, L.kmax,
-                           L.kmax, ls);
-      });
-      phase(l, 8, ls, true, [&] {
-        gemm(gp, L.tX, ls);
-      });
-    } else {
-      phase(l, 8, ls, true, [&] {
-        launch_apply<T>(reinterpret_cast<const ApplyProb<T>*>(D(L.o_apply)), L.n_apply, M, ls);
-      });
-    }
+    phase(l, 6, ls, true, [&] {
+      gemm(gp, L.tB, ls);
+      launch_wy_solve<T>(reinterpret_cast<const WySolveProb<T>*>(D(L.o_wy)), L.n_wy, L.kmax,
+                         L.kmax, ls);
+    });
+    phase(l, 8, ls, true, [&] { gemm(gp, L.tX, ls); });
     }
     if (l == syrk_lane && part == 0) syrk(ls);
     if (own) (void)hipEventRecord(L.ev_done, ls);

@@ -115,18 +115,6 @@ struct ConvProb {
   int n;  // elements
 };
 
-// ---- reassembly: X = Q [Z; 0] written into the caller's factor ---------------
-template <typename T>
-struct ApplyProb {
-  const T* A;    // factored stack (reflectors below the diagonal)
-  const T* Tp;   // compact-WY factors of the stack's panels
-  const T* Z;    // K x r, ld K
-  T* X;          // nb x r output, leading dimension ldx
-  const int* rank;
-  const int* status;
-  int K, ldx;
-};
-
 constexpr int kThreads = 256;
 constexpr int kPB = 8;  // Householder panel width
 
@@ -204,9 +192,8 @@ constexpr int kMaxSweeps = 40;  // Jacobi sweep limit of every core variant
 template <typename T>
 void launch_core(const CoreProb<T>* probs, int n, int variant, int kmax, int rk, int maxrk,
                  T acc, hipStream_t s);
-template <typename T>
-void launch_apply(const ApplyProb<T>* probs, int n, int nb, hipStream_t s);
-// blocked back substitution of the compact-WY solve, 64 columns per workgroup
+// blocked back substitution of the compact-WY solve (64 columns per workgroup
+// up to K = 256, 16 beyond)
 constexpr int kWyMaxK = 512;
 template <typename T>
 void launch_wy_solve(const WySolveProb<T>* probs, int n, int kmax, int rmax, hipStream_t s);

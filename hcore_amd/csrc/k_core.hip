@@ -83,13 +83,18 @@ __device__ void jacobi_sweep(T* A, int ld, int ncol, int K, T tol, T thr2, T* nr
         apq = fma(x[i], y[i], apq);
       }
       apq = gsum<G>(apq);
-      const T app = nrm[a], aqq = nrm[b];
-      if (fmin(app, aqq) > thr2 && fabs(apq) > tol * sqrt(app * aqq)) {
-        const T zeta = (aqq - app) / (T(2) * apq);
+      const T app = nrm[a], aqq = nrm[b], pq = app * aqq;
+      if (fmin(app, aqq) > thr2 && pq > T(0) && fabs(apq) * fast_rsq(pq) > tol) {
+        // rotation parameters through the refined hardware reciprocals
+        const T zeta = (aqq - app) * (T(0.5) * fast_rcp(apq));
         T t;
-        if (fabs(zeta) > T(1e30)) t = T(0.5) / zeta;
-        else t = (zeta >= T(0) ? T(1) : T(-1)) / (fabs(zeta) + sqrt(fma(zeta, zeta, T(1))));
-        const T cs = T(1) / sqrt(fma(t, t, T(1)));
+        if (fabs(zeta) > T(1e30)) {
+          t = T(0.5) * fast_rcp(zeta);
+        } else {
+          const T h2 = fma(zeta, zeta, T(1));
+          t = (zeta >= T(0) ? T(1) : T(-1)) * fast_rcp(fabs(zeta) + h2 * fast_rsq(h2));
+        }
+        const T cs = fast_rsq(fma(t, t, T(1)));
         const T sn = cs * t;
 #pragma unroll
         for (int i = 0; i < RPL; ++i) {
@@ -184,10 +189,11 @@ __device__ int qrcp(T* A, int K, int ld, T thr2, T* cn, T* rdiag, int* piv, T* g
     const T alpha = cs_[s];
     T beta = alpha, tau = T(0), scal = T(0);
     if (xn2 > T(0)) {
-      const T nrm = sqrt(fma(alpha, alpha, xn2));
+      const T h = fma(alpha, alpha, xn2);
+      const T nrm = h > T(1e-200) ? h * fast_rsq(h) : sqrt(h);  // the seed needs a normal h
       beta = alpha >= T(0) ? -nrm : nrm;
-      tau = (beta - alpha) / beta;
-      scal = T(1) / (alpha - beta);
+      tau = (beta - alpha) * fast_rcp(beta);
+      scal = fast_rcp(alpha - beta);
     }
     // this lane's rows of the reflector v (v_s = 1, v_i = A(i,s)*scal, 0 above s)
     T v[RPL];

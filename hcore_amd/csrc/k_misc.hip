@@ -1,4 +1,4 @@
-// k_misc.hip — reassembly (reflector application, compact-WY solve) and conversion kernels.
+// k_misc.hip — reassembly (compact-WY solve) and conversion kernels.
 
 #include <hip/hip_runtime.h>
 
@@ -11,141 +11,6 @@
 #include "hcore_wave.hpp"
 
 namespace hcore {
-
-// ------------------------------------------------------------------------------
-// reassembly X = Q [Z; 0]: panels applied last to first, X <- (I - Y T Y^T) X
-// ------------------------------------------------------------------------------
-template <typename T, int NBW, int PB, int CB>
-__global__ __launch_bounds__(kThreads) void k_apply(const ApplyProb<T>* __restrict__ probs, int nb) {
-  constexpr int NBP = 64 * NBW;
-  constexpr int NW = PB * CB;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* Yp = reinterpret_cast<T*>(smem_raw);  // PB x NBP
-  T* Tt = Yp + PB * NBP;                   // PB x PB
-  T* scr = Tt + PB * PB;                   // 4 x 2 x NW
-
-  const ApplyProb<T> P = probs[blockIdx.x];
-  if (*P.status != ST_OK) return;
-  const int r = *P.rank;
-  const int K = P.K;
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  T* __restrict__ X = P.X;
-  const size_t ldx = P.ldx;
-  T* wscr0 = scr + wid * 2 * NW;
-  T* wscr1 = wscr0 + NW;
-  const T* Yl = Yp + lane;  // this lane's rows: Yl[c * NBP + 64 * i]
-
-  #pragma unroll 1
-
-  for (int e = tid; e < r * nb; e += kThreads) {
-    const int col = e / nb, row = e - col * nb;
-    X[row + col * ldx] = row < K ? P.Z[row + (size_t)col * K] : T(0);
-  }
-  __syncthreads();
-
-  const int npan = (K + PB - 1) / PB;
-  const int nch = (r + CB - 1) / CB;
-  for (int p = npan - 1; p >= 0; --p) {
-    const int j0 = p * PB;
-    const int pb = min(PB, K - j0);
-    #pragma unroll 1
-    for (int e = tid; e < PB * NBP; e += kThreads) {
-      const int c = e / NBP, row = e - c * NBP;
-      T v = T(0);
-      if (c < pb && row < nb) {
-        if (row == j0 + c) v = T(1);
-        else if (row > j0 + c) v = P.A[row + (size_t)(j0 + c) * nb];
-      }
-      Yp[e] = v;
-    }
-    if (tid < PB * PB) Tt[tid] = P.Tp[(size_t)p * PB * PB + tid];
-    __syncthreads();
-    for (int q = wid; q < nch; q += 4) {
-      const int k0 = q * CB;
-      const int cb = min(CB, r - k0);
-      T Xr[NBW][CB];
-#pragma unroll
-      for (int i = 0; i < NBW; ++i) {
-        const int row = lane + 64 * i;
-#pragma unroll
-        for (int kk = 0; kk < CB; ++kk)
-          Xr[i][kk] = (row >= j0 && row < nb && kk < cb) ? X[row + (k0 + kk) * ldx] : T(0);
-      }
-      T W[NW];
-#pragma unroll
-      for (int c = 0; c < PB; ++c) {
-        T y[NBW];
-#pragma unroll
-        for (int i = 0; i < NBW; ++i) y[i] = Yl[c * NBP + 64 * i];
-#pragma unroll
-        for (int kk = 0; kk < CB; ++kk) {
-          T acc = T(0);
-#pragma unroll
-          for (int i = 0; i < NBW; ++i) acc = fma(y[i], Xr[i][kk], acc);
-          W[c * CB + kk] = acc;
-        }
-        __builtin_amdgcn_sched_barrier(0);
-      }
-      reduce_scatter<NW>(W, lane);
-      wscr0[reduce_scatter_index<NW>(lane)] = W[0];
-      wave_sync();
-      if (lane < NW) {  // W2 = T W (T upper triangular)
-        const int c = lane / CB, kk = lane - c * CB;
-        T s = T(0);
-        for (int c2 = c; c2 < PB; ++c2) s = fma(Tt[c + c2 * PB], wscr0[c2 * CB + kk], s);
-        wscr1[lane] = s;
-      }
-      wave_sync();
-#pragma unroll
-      for (int c = 0; c < PB; ++c) {
-        T y[NBW];
-#pragma unroll
-        for (int i = 0; i < NBW; ++i) y[i] = Yl[c * NBP + 64 * i];
-#pragma unroll
-        for (int kk = 0; kk < CB; ++kk) {
-          const T w2 = wscr1[c * CB + kk];
-#pragma unroll
-          for (int i = 0; i < NBW; ++i) Xr[i][kk] = fma(-y[i], w2, Xr[i][kk]);
-        }
-        __builtin_amdgcn_sched_barrier(0);
-      }
-#pragma unroll
-      for (int i = 0; i < NBW; ++i) {
-        const int row = lane + 64 * i;
-        if (row >= j0 && row < nb) {
-#pragma unroll
-          for (int kk = 0; kk < CB; ++kk)
-            if (kk < cb) X[row + (k0 + kk) * ldx] = Xr[i][kk];
-        }
-      }
-      wave_sync();
-    }
-    __syncthreads();
-  }
-}
-
-template <typename T, int NBW>
-static void launch_apply_nbw(const ApplyProb<T>* probs, int n, int nb, hipStream_t s) {
-  constexpr int CB = qr_cb<NBW>();
-  const size_t sm = sizeof(T) * ((size_t)kPB * 64 * NBW + kPB * kPB + 4 * 2 * kPB * CB);
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply<T, NBW, kPB, CB>),
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-    attr = true;
-  }
-  if (n <= 0) return;
-  hipLaunchKernelGGL((k_apply<T, NBW, kPB, CB>), dim3(n), dim3(kThreads), sm, s, probs, nb);
-}
-
-template <typename T>
-void launch_apply(const ApplyProb<T>* probs, int n, int nb, hipStream_t s) {
-  if (nb <= 128) launch_apply_nbw<T, 2>(probs, n, nb, s);
-  else if (nb <= 256) launch_apply_nbw<T, 4>(probs, n, nb, s);
-  else if (nb <= 512) launch_apply_nbw<T, 8>(probs, n, nb, s);
-  else if (nb <= 1024) launch_apply_nbw<T, 16>(probs, n, nb, s);
-  else launch_apply_nbw<T, 32>(probs, n, nb, s);
-}
 
 // ------------------------------------------------------------------------------
 // compact-WY solve M = T Y1^T Z by blocked back substitution (K <= kWyMaxK):
@@ -335,18 +200,12 @@ void prepare_kernels() {
   launch_geqrf<T>(nullptr, 0, 512, nullptr);
   launch_geqrf<T>(nullptr, 0, 1024, nullptr);
   launch_geqrf<T>(nullptr, 0, 2048, nullptr);
-  launch_apply<T>(nullptr, 0, 128, nullptr);
-  launch_apply<T>(nullptr, 0, 256, nullptr);
-  launch_apply<T>(nullptr, 0, 512, nullptr);
-  launch_apply<T>(nullptr, 0, 1024, nullptr);
-  launch_apply<T>(nullptr, 0, 2048, nullptr);
   launch_core<T>(nullptr, 0, CORE_K16, 1, 0, 1, T(0), nullptr);
   launch_core<T>(nullptr, 0, CORE_R64, 1, 0, 1, T(0), nullptr);
 }
 
 // ---- explicit instantiations ---------------------------------------------------
 #define HCORE_INSTANTIATE(T) \
-  template void launch_apply<T>(const ApplyProb<T>*, int, int, hipStream_t); \
   template void launch_wy_solve<T>(const WySolveProb<T>*, int, int, int, hipStream_t); \
   template void prepare_kernels<T>();
 HCORE_INSTANTIATE(double)
