@@ -11,7 +11,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "hcore_d.h"
@@ -37,10 +39,19 @@ extern "C" int HCORE_hip_tlr_dpotrf(int nt, int nb, double* const* D, double* co
   int st = ensure_ready();
   if (st != HCORE_HIP_OK) return st;
   flop_counter fl{};
+  // HCORE_TLR_PROFILE=1: host-side seconds per part of the stream on stderr
+  static const bool prof = getenv("HCORE_TLR_PROFILE") != nullptr;
+  using clk = std::chrono::steady_clock;
+  double t_potrf = 0, t_trsm = 0, t_build = 0, t_run = 0, t_free = 0;
+  auto since = [](clk::time_point t0) {
+    return std::chrono::duration<double>(clk::now() - t0).count();
+  };
   for (int k = 0; k < nt; ++k) {
     int pinfo = 0;
     set_last_status(HCORE_HIP_OK);
+    auto t0 = clk::now();
     HCORE_dpotrf(HCORE_Lower, nb, D[k], nb, &pinfo, &fl);
+    t_potrf += since(t0);
     if ((st = HCORE_hip_last_status()) != HCORE_HIP_OK) break;
     if (pinfo) {
       if (info) *info = k * nb + pinfo;
@@ -48,6 +59,7 @@ extern "C" int HCORE_hip_tlr_dpotrf(int nt, int nb, double* const* D, double* co
     }
     if (k == nt - 1) break;
     // panel: V_ik <- L_kk^{-1} V_ik for i > k (A_ik L_kk^{-T} = U_ik (L_kk^{-1} V_ik)^T)
+    t0 = clk::now();
     bool contiguous = true;
     for (int i = k + 2; i < nt; ++i)
       contiguous &= V[i + k * nt] == V[k + 1 + k * nt] + (size_t)(i - k - 1) * cap * nb;
@@ -62,8 +74,10 @@ extern "C" int HCORE_hip_tlr_dpotrf(int nt, int nb, double* const* D, double* co
         st = HCORE_hip_last_status();
       }
     }
+    t_trsm += since(t0);
     if (st != HCORE_HIP_OK) break;
     // trailing matrix: dsyrk on D_i (i > k), recompression of A_ij (i > j > k)
+    t0 = clk::now();
     std::vector<void*> sAU, sAV, sCD;
     std::vector<int> sK;
     for (int i = k + 1; i < nt; ++i) {
@@ -92,6 +106,8 @@ extern "C" int HCORE_hip_tlr_dpotrf(int nt, int nb, double* const* D, double* co
       st = HCORE_hip_plan_add_dsyrk(p, (int)sAU.size(), sAU.data(), sAV.data(), sK.data(), nb,
                                     sCD.data(), nb, -1.0);
     if (st == HCORE_HIP_OK) st = HCORE_hip_plan_finalize(p);
+    t_build += since(t0);
+    t0 = clk::now();
     if (st == HCORE_HIP_OK) st = HCORE_hip_plan_execute(p, stream);
     std::vector<int> nr(std::max(n, 1)), ns(std::max(n, 1));
     if (st == HCORE_HIP_OK && n) st = HCORE_hip_plan_get_ranks(p, nr.data());
@@ -103,7 +119,10 @@ extern "C" int HCORE_hip_tlr_dpotrf(int nt, int nb, double* const* D, double* co
       fl.update += upd;
       fl.syrk += syrk;
     }
+    t_run += since(t0);
+    t0 = clk::now();
     if (p) HCORE_hip_plan_destroy(p);
+    t_free += since(t0);
     if (st != HCORE_HIP_OK) break;
     int rmax = 0;
     for (int u = 0; u < n; ++u) {
@@ -117,6 +136,11 @@ extern "C" int HCORE_hip_tlr_dpotrf(int nt, int nb, double* const* D, double* co
     if (step_ranks_max) step_ranks_max[k] = rmax;
     if (st != HCORE_HIP_OK) break;
   }
+  if (prof)
+    fprintf(stderr,
+            "HCORE_hip_tlr_dpotrf: potrf %.3f s, trsm %.3f s, plan build %.3f s, execute %.3f s, "
+            "plan release %.3f s\n",
+            t_potrf, t_trsm, t_build, t_run, t_free);
   if (flops) {
     flops->potrf += fl.potrf;
     flops->trsm += fl.trsm;
