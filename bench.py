@@ -71,15 +71,18 @@ def graph_ms(plan, stream, steps, warmup):
 
 
 def reduced_precision_leg(step, stream, steps, warmup, acc, sample):
-    """FP64 and mixed-precision plans of the same gemm batch at one accuracy:
-    step times and the worst relative error of sampled updates against the
-    FP64 dense update of the original FP64 tiles."""
+    """FP64, mixed-precision (FP32 tiles) and split-precision (FP64 tiles,
+    BF16-digit products, digit counts from the slice autotuner) plans of the
+    same gemm batch at one accuracy: step times and the worst relative error
+    of sampled updates against the FP64 dense update of the original FP64
+    tiles."""
     import hcore_amd as hc
     nb = step.nb
     ra, rb, rc = step.ranks()
     K = ra + rc
     out = {"acc": acc}
-    for name, prec, dt in (("fp64", hc.FP64, np.float64), ("mixed", hc.MIXED, np.float32)):
+    for name, prec, dt in (("fp64", hc.FP64, np.float64), ("mixed", hc.MIXED, np.float32),
+                           ("split", hc.SPLIT, np.float64)):
         B = lambda x, cap=None: hc.DeviceBuffer.from_numpy(np.asfortranarray(x.astype(dt)), cap)
         panel = {i: (B(u), B(v)) for i, (u, v) in step.panel.items()}
         CUd = [B(c[0], k) for c, k in zip(step.C, K)]
@@ -95,6 +98,11 @@ def reduced_precision_leg(step, stream, steps, warmup, acc, sample):
         plan.finalize()
         plan.execute(stream)
         stream.synchronize()
+        tuned = None
+        if prec == hc.SPLIT:  # digits per lane: the smallest keeping sigma within 0.1 acc
+            tuned = plan.autotune_slices(stream)
+            plan.execute(stream)
+            stream.synchronize()
         assert np.all(plan.status() == 0)
         ranks = plan.ranks()
         out.setdefault("_ranks", {})[name] = ranks.copy()
@@ -111,14 +119,19 @@ def reduced_precision_leg(step, stream, steps, warmup, acc, sample):
         ms = graph_ms(plan, stream, steps, warmup)
         out[name] = {"ms_per_step": ms, "value": len(step.pairs) / (ms / 1e3),
                      "worst_rel_err": worst, "mean_rank": float(np.mean(ranks))}
+        if tuned is not None:
+            out[name]["slices_per_lane"] = [int(t[1]) for t in tuned]
+            out[name]["sigma_dev_over_acc"] = [float(t[2]) for t in tuned]
         plan.destroy()
         for t in CUd + CVd + OUd + OVd + [x for pr in panel.values() for x in pr]:
             t.free()
-    out["mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
     rk = out.pop("_ranks")
-    d = np.abs(rk["mixed"].astype(int) - rk["fp64"].astype(int))
-    out["rank_drift_mixed_vs_fp64"] = {"max": int(d.max()), "updates_differing": int(np.count_nonzero(d))}
-    out["meets_acc"] = {k: bool(out[k]["worst_rel_err"] <= 10 * acc) for k in ("fp64", "mixed")}
+    for k in ("mixed", "split"):
+        out["%s_over_fp64" % k] = out[k]["value"] / out["fp64"]["value"]
+        d = np.abs(rk[k].astype(int) - rk["fp64"].astype(int))
+        out["rank_drift_%s_vs_fp64" % k] = {"max": int(d.max()),
+                                            "updates_differing": int(np.count_nonzero(d))}
+    out["meets_acc"] = {k: bool(out[k]["worst_rel_err"] <= 10 * acc) for k in ("fp64", "mixed", "split")}
     return out
 
 
@@ -194,10 +207,21 @@ def single_tile_leg(reps):
     return out
 
 
+def _factored_rel_diff(XU, XV, YU, YV):
+    """||XU XV^T - YU YV^T||_F / ||YU YV^T||_F without forming the tile."""
+    Q, R = np.linalg.qr(np.hstack([XU, -YU]))
+    num = np.linalg.norm(R @ np.hstack([XV, YV]).T)
+    Q2, R2 = np.linalg.qr(YU)
+    return float(num / np.linalg.norm(R2 @ YV.T))
+
+
 def batch2048_leg(steps, warmup):
     """The 512-update batch on nb = 2048 tiles (ranks 32-256 per tile, stacked
-    ranks to 506, acc 1e-8, maxrank 512): graph replays of one plan, the
-    reference's flop accounting with the plan's ranks."""
+    ranks to 506, acc 1e-8, maxrank 512): graph replays of one FP64 plan, the
+    reference's flop accounting with the plan's ranks; then the same batch as
+    a split-precision plan (BF16-digit products, digit counts from the slice
+    autotuner) with its error per tile on a sample: against the FP64 plan's
+    factors, and against the dense FP64 update for the largest stacked ranks."""
     import hcore_amd as hc
     from hcore_amd.tiles import TrailingStep
     t = time.time()
@@ -242,14 +266,60 @@ def batch2048_leg(steps, warmup):
          int(np.sum(22 * Kl ** 3)) + int(np.sum(r * Kl)) +
          sum(fc('o', nb, int(x), int(k), 1) + fc('o', int(x), nb, int(k), 2) for x, k in zip(r, Kl)))
     plan.destroy()
-    for b in CUd + CVd + OUd + OVd + [x for pr in panel.values() for x in pr]:
-        b.free()
     res = {"updates": len(step.pairs), "nb": nb, "K_min": int(K.min()), "K_max": int(K.max()),
            "ms_per_step": ms, "updates_per_s": len(step.pairs) / (ms / 1e3),
            "tflops": f / (ms / 1e3) / 1e12, "flops_per_step": f, "generate_s": t_gen,
            "phase_ms_serialized": phases}
     log("batch 512 x nb 2048 (K %d-%d): %.2f ms/step, %.0f updates/s, %.2f TF/s" %
         (K.min(), K.max(), ms, res["updates_per_s"], res["tflops"]))
+    # ---- split precision on the same batch ------------------------------------
+    order = np.argsort(-K, kind="stable")
+    sample = sorted(set(range(0, len(K), 8)) | set(int(u) for u in order[:4]))
+    ref = {u: (OUd[u].numpy(int(r[u])), OVd[u].numpy(int(r[u]))) for u in sample}
+    for b in OUd + OVd:
+        b.zero()
+    sp = hc.Plan(nb, hc.SPLIT)
+    sp.add_dgemm([panel[i][0].ptr for i, j in step.pairs], [panel[i][1].ptr for i, j in step.pairs],
+                 ra, nb, [panel[j][0].ptr for i, j in step.pairs],
+                 [panel[j][1].ptr for i, j in step.pairs], rb, nb,
+                 [t.ptr for t in CUd], [t.ptr for t in CVd], rc, nb, -1.0, 1.0, 0, 512, 1e-8,
+                 CUout=[t.ptr for t in OUd], CVout=[t.ptr for t in OVd], ldco=nb)
+    sp.finalize()
+    sp.execute(stream)
+    stream.synchronize()
+    tuned = sp.autotune_slices(stream)
+    ms_s = graph_ms(sp, stream, steps, warmup)
+    assert np.all(sp.status() == 0)
+    rs = sp.ranks()
+    errs = np.array([_factored_rel_diff(OUd[u].numpy(int(rs[u])), OVd[u].numpy(int(rs[u])),
+                                        *ref[u]) for u in sample])
+    dense = []
+    for u in (int(x) for x in order[:4]):
+        i, j = step.pairs[u]
+        AU, AV = step.panel[i]
+        BU, BV = step.panel[j]
+        CU, CV = step.C[u]
+        E = CU @ CV.T - (AU @ AV.T) @ (BU @ BV.T).T
+        got = OUd[u].numpy(int(rs[u])) @ OVd[u].numpy(int(rs[u])).T
+        dense.append(float(np.linalg.norm(got - E) / np.linalg.norm(E)))
+    sp.destroy()
+    for b in CUd + CVd + OUd + OVd + [x for pr in panel.values() for x in pr]:
+        b.free()
+    drift = np.abs(rs.astype(int) - r.astype(int))
+    res["split"] = {"ms_per_step": ms_s, "updates_per_s": len(step.pairs) / (ms_s / 1e3),
+                    "split_over_fp64": ms / ms_s,
+                    "slices_per_lane": [int(t[1]) for t in tuned],
+                    "sigma_dev_over_acc": [float(t[2]) for t in tuned],
+                    "sample_tiles": len(sample),
+                    "rel_diff_vs_fp64_per_tile": {"max": float(errs.max()),
+                                                  "median": float(np.median(errs))},
+                    "rel_err_vs_dense_largest_K": dense,
+                    "rank_drift_vs_fp64": {"max": int(drift.max()),
+                                           "updates_differing": int(np.count_nonzero(drift))}}
+    log("batch 512 x nb 2048 split: %.2f ms/step (split/fp64 %.2fx), digits per lane %s, "
+        "per-tile rel diff vs fp64 max %.2e (%d tiles), vs dense (4 largest K) max %.2e, "
+        "rank drift %d" % (ms_s, ms / ms_s, [int(t[1]) for t in tuned], errs.max(), len(sample),
+                           max(dense), drift.max()))
     return res
 
 
@@ -519,15 +589,18 @@ def main():
             roof["traffic_kernel"] = tr["kernel"]
             roof["traffic_source"] = tr["source"]
 
-    # ---- reduced precision: FP64 vs mixed plans of the gemm batch at acc 1e-6 -------
-    # (the mixed plan's FP32 tiles alone carry a relative rounding error of about
-    # 3e-8, so it is measured at the accuracy it can meet)
+    # ---- reduced precision: FP64 vs mixed (FP32 tiles) and split (BF16-digit
+    # products) plans of the gemm batch at acc 1e-8 and 1e-6 --------------------------
     mixed = None
     if not args.no_mixed:
         mixed = {}
         for acc in (1e-8, 1e-6):
             m = reduced_precision_leg(step, stream, args.steps, args.warmup, acc, sample)
             mixed["acc_%g" % acc] = m
+            log("acc %g: split %.3f ms (worst %.2e, digits per lane %s), split/fp64 %.2fx, rank drift %d" %
+                (acc, m["split"]["ms_per_step"], m["split"]["worst_rel_err"],
+                 m["split"]["slices_per_lane"], m["split_over_fp64"],
+                 m["rank_drift_split_vs_fp64"]["max"]))
             log("acc %g: fp64 %.3f ms (worst %.2e), mixed %.3f ms (worst %.2e), mixed/fp64 %.2fx, "
                 "rank drift %d" % (acc, m["fp64"]["ms_per_step"], m["fp64"]["worst_rel_err"],
                                    m["mixed"]["ms_per_step"], m["mixed"]["worst_rel_err"],

@@ -22,7 +22,7 @@ INCLUDE_DIR = os.path.join(ROOT, "include")
 # status codes (include/hcore_hip.h)
 OK, ERR_ZERO_RANK, ERR_SUM_RANKS, ERR_K_GT_MAXRK, ERR_R_GT_MAXRK, ERR_NUMERIC, \
     ERR_RUNTIME, ERR_ARG = range(8)
-FP64, FP32, MIXED = 0, 1, 2
+FP64, FP32, MIXED, SPLIT = 0, 1, 2, 3
 ON_ERROR_EXIT, ON_ERROR_RETURN = 0, 1
 
 
@@ -92,6 +92,10 @@ def lib():
         for fn in ("HCORE_dgemm_qr_svd", "HCORE_dgemm_qr_svd_b_dense", "HCORE_dgemm_ormqr"):
             getattr(L, fn).restype = None
         L.HCORE_hip_gemm.argtypes = [i, i, i, i, i, i, d, _vp, i, _vp, i, d, _vp, i, i, _vp]
+        L.HCORE_hip_gemm_split.argtypes = [i, i, i, i, i, i, d, _vp, i, _vp, i, d, _vp, i, _vp]
+        L.HCORE_hip_plan_set_slices.argtypes = [_vp, i]
+        L.HCORE_hip_plan_get_slices.argtypes = [_vp, _ip, i]
+        L.HCORE_hip_plan_autotune_slices.argtypes = [_vp, _vp, d, _dp, i]
         L.flop_counts.argtypes = [C.c_char, C.c_ulong, C.c_ulong, C.c_ulong, C.c_ulong]
         L.flop_counts.restype = C.c_ulong
         L.HCORE_hip_init.argtypes = [i]
@@ -429,6 +433,25 @@ def dgemm_split(alpha, AU, AV, BU, BV, beta, CU, CV, rk=0, maxrk=None, acc=1e-8,
     return CUw[:, :r].copy(order="F"), CVw[:, :r].copy(order="F"), r
 
 
+def gemm_split(A, B, C_in, slices, alpha=1.0, beta=0.0, transA=False, transB=False):
+    """HCORE_hip_gemm_split on device copies of float64 numpy arrays (slices
+    BF16 digits per operand; 0: the same tiles on the FP64 matrix cores);
+    returns the new C."""
+    a = DeviceBuffer.from_numpy(np.asfortranarray(A, dtype=np.float64))
+    b = DeviceBuffer.from_numpy(np.asfortranarray(B, dtype=np.float64))
+    c = DeviceBuffer.from_numpy(np.asfortranarray(C_in, dtype=np.float64))
+    m, n = C_in.shape
+    k = A.shape[0] if transA else A.shape[1]
+    st = lib().HCORE_hip_gemm_split(slices, 112 if transA else 111, 112 if transB else 111, m, n,
+                                    k, alpha, a.ptr, A.shape[0], b.ptr, B.shape[0], beta, c.ptr,
+                                    m, None)
+    _check(st, "HCORE_hip_gemm_split")
+    out = c.numpy()
+    for x in (a, b, c):
+        x.free()
+    return out
+
+
 def gemm(A, B, C_in, alpha=1.0, beta=0.0, transA=False, transB=False, path=0):
     """HCORE_hip_gemm on device copies of numpy arrays (float64 or float32);
     returns the new C."""
@@ -687,6 +710,28 @@ class Plan:
         if n < 0:
             raise RuntimeError("autotune failed: %d" % -n)
         return [tuple(rep[6 * k:6 * k + 6]) for k in range(min(n, 32))]
+
+    def set_slices(self, slices):
+        """HCORE_hip_plan_set_slices (split-precision plans; 0: from acc)."""
+        _check(self.L.HCORE_hip_plan_set_slices(self.h, slices), "HCORE_hip_plan_set_slices")
+
+    def slices(self):
+        """HCORE_hip_plan_get_slices: digit count per lane."""
+        out = np.zeros(8, dtype=np.int32)
+        n = self.L.HCORE_hip_plan_get_slices(self.h, out.ctypes.data_as(_ip), 8)
+        if n < 0:
+            raise RuntimeError("HCORE_hip_plan_get_slices failed: %d" % -n)
+        return out[:n].copy()
+
+    def autotune_slices(self, stream=None, tol=0.1):
+        """HCORE_hip_plan_autotune_slices: list of (lane, chosen digits,
+        deviation / acc at that count, deviation / acc with one digit fewer)."""
+        rep = np.zeros(4 * 8)
+        n = self.L.HCORE_hip_plan_autotune_slices(self.h, C.c_void_p(stream.h if stream else None),
+                                                  tol, rep.ctypes.data_as(_dp), 8)
+        if n < 0:
+            raise RuntimeError("autotune_slices failed: %d" % -n)
+        return [tuple(rep[4 * k:4 * k + 4]) for k in range(min(n, 8))]
 
     def add_dgemm_dev(self, AU, AV, ra_ptrs, ra_cap, lda, BU, BV, rb_ptrs, rb_cap, ldb, CU, CV,
                       rc_ptrs, rc_cap, ldc, alpha, beta, rk, maxrk, acc, CUout=None, CVout=None,

@@ -250,6 +250,7 @@ static void tune_store(int M, bool fp64, int base, int choice) {
 struct hcore_hip_plan {
   int M = 0;
   int prec = HCORE_HIP_FP64;
+  int slices_set = 0;  // split precision: digits per operand requested (0: from acc)
   bool finalized = false;
   // single-call plans: device memory from a per-thread arena (or a fixed
   // buffer), set up asynchronously on `fin_stream`, no events or lane streams
@@ -317,6 +318,7 @@ struct hcore_hip_plan {
     size_t o_cvin = 0, o_cvout = 0;  // mixed: FP32 <-> FP64 conversions around the core
     int n_cv = 0, cv_max = 0;
     int n_wy = 0;
+    int slices = 0;  // split precision: digits per operand of the sliced products
     std::vector<CoreLaunch> core;  // one launch per K bucket
     hipStream_t stream = nullptr;  // lanes > 0 (lane 0 uses the caller's stream)
     hipEvent_t ev_done = nullptr;
@@ -347,7 +349,30 @@ struct hcore_hip_plan {
   int phase_count[kMaxLanes][HCORE_HIP_NPHASES] = {};
 };
 
+// A captured graph holds the launches as they were: capture again on the next
+// replay (after the autotuners change a launch).
+static void drop_graph(hcore_hip_plan* p) {
+  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  p->gexec = nullptr;
+  p->graph = nullptr;
+}
+
+// FP64 storage (plain FP64 and split precision) or FP32 (FP32, mixed).
+static bool plan_double(const hcore_hip_plan* p) {
+  return p->prec == HCORE_HIP_FP64 || p->prec == HCORE_HIP_SPLIT;
+}
+
 namespace hcore {
+
+// Digits per operand of a split-precision plan's products for tolerance acc:
+// 8 bits each, enough for about 10 bits below acc relative to the operands'
+// row and column scales (acc 1e-8: 5, 1e-6: 4, 1e-4: 3).
+static int default_slices(double acc) {
+  if (!(acc > 0)) return kOzMaxSlices;
+  const int s = (int)std::ceil((-std::log2(acc) + 10.0) / 8.0);
+  return std::min(kOzMaxSlices, std::max(kOzMinSlices, s));
+}
 
 // Streams of the lanes after the first (default priority, non-blocking).
 static hipError_t lane_stream_create(hipStream_t* s) {
@@ -360,6 +385,9 @@ static int plan_finalize_t(hcore_hip_plan* p) {
   const int n = p->ng;
   // mixed precision: FP32 everywhere except the core SVD and truncation (FP64)
   const bool mixed = p->prec == HCORE_HIP_MIXED;
+  // split precision: the products through BF16 digit slices (GF_SLICED), the
+  // Householder QR (with its Gram matrices), the core and the truncation FP64
+  const int sl = p->prec == HCORE_HIP_SPLIT ? GF_SLICED : 0;
   const int KL = mixed ? core_kl<double>() : core_kl<T>();
   const size_t es = sizeof(T);
   auto al = [](size_t x) { return (x + 63) & ~size_t(63); };  // elements, 512 B for FP64
@@ -529,7 +557,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
           // F = AV^T BV   (hcore_dgemm.c:175-178)
           g.A = (const T*)p->AV[u]; g.B = (const T*)p->BV[u]; g.C = W + o_F[u];
           g.m = ra; g.n = rb; g.k = M; g.lda = p->lda; g.ldb = p->ldb; g.ldc = ra;
-          g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_BETA_ZERO;
+          g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_BETA_ZERO | sl;
           add_tasks(d.tF, (int)gp.size(), g); gp.push_back(g);
           if (dvr) {
             rec(gp, gp.size() - 1, HCORE_GF(m), PK_RA, u);
@@ -540,7 +568,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
           g = GemmProb<T>{};
           g.A = (const T*)p->BU[u]; g.B = W + o_F[u]; g.C = Vs + (size_t)rc * M;
           g.m = M; g.n = ra; g.k = rb; g.lda = p->ldb; g.ldb = ra; g.ldc = M;
-          g.alpha = T(1); g.beta = T(0); g.flags = GF_B_TRANS | GF_BETA_ZERO;
+          g.alpha = T(1); g.beta = T(0); g.flags = GF_B_TRANS | GF_BETA_ZERO | sl;
           add_tasks(d.tG, (int)gp.size(), g); gp.push_back(g);
           if (dvr) {
             rec(gp, gp.size() - 1, HCORE_GF(n), PK_RA, u);
@@ -552,7 +580,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
           // G = B AV into Vs(:, rc:K) for a dense B   (hcore_dgemm.c:255-262)
           g.A = (const T*)p->BD[u]; g.B = (const T*)p->AV[u]; g.C = Vs + (size_t)rc * M;
           g.m = M; g.n = ra; g.k = M; g.lda = p->ldbd; g.ldb = p->lda; g.ldc = M;
-          g.alpha = T(1); g.beta = T(0); g.flags = GF_BETA_ZERO;
+          g.alpha = T(1); g.beta = T(0); g.flags = GF_BETA_ZERO | sl;
           add_tasks(d.tG, (int)gp.size(), g); gp.push_back(g);
         }
         // T = R_A R_B^T   (hcore_dgemm.c:384-386)
@@ -560,7 +588,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
         g.A = Us; g.B = Vs; g.C = W + o_Tc[u];
         g.m = K; g.n = K; g.k = K; g.lda = M; g.ldb = M; g.ldc = K;
         g.alpha = T(1); g.beta = T(0);
-        g.flags = GF_A_UPPER | GF_B_TRANS | GF_B_UPPER | GF_BETA_ZERO;
+        g.flags = GF_A_UPPER | GF_B_TRANS | GF_B_UPPER | GF_BETA_ZERO | sl;
         add_tasks(d.tT, (int)gp.size(), g); gp.push_back(g);
         auto recK = [&](std::initializer_list<size_t> fields) {
           if (dvr)
@@ -571,7 +599,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
         g = GemmProb<T>{};
         g.A = W + o_Tc[u]; g.B = W + o_Zu[u]; g.C = W + o_Zv[u];
         g.m = K; g.n = K; g.k = K; g.lda = K; g.ldb = K; g.ldc = K;
-        g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_BETA_ZERO;
+        g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_BETA_ZERO | sl;
         g.n_dev = d_rank + u; g.skip = d_status + u;
         add_tasks(d.tZ, (int)gp.size(), g); gp.push_back(g);
         recK({HCORE_GF(m), HCORE_GF(n), HCORE_GF(k), HCORE_GF(lda), HCORE_GF(ldb), HCORE_GF(ldc)});
@@ -604,7 +632,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
             g = GemmProb<T>{};
             g.A = Ys[side]; g.B = Zs[side]; g.C = Ms[side];
             g.m = K; g.n = K; g.k = K; g.lda = M; g.ldb = K; g.ldc = K;
-            g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_A_UNIT_LOWER | GF_BETA_ZERO;
+            g.alpha = T(1); g.beta = T(0); g.flags = GF_A_TRANS | GF_A_UNIT_LOWER | GF_BETA_ZERO | sl;
             g.n_dev = d_rank + u; g.skip = d_status + u;
             add_tasks(side == 0 ? d.tZ : d.tB, (int)gp.size(), g); gp.push_back(g);
             recK({HCORE_GF(m), HCORE_GF(n), HCORE_GF(k), HCORE_GF(ldb), HCORE_GF(ldc)});
@@ -612,7 +640,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
             g = GemmProb<T>{};
             g.A = Ys[side]; g.B = Ms[side]; g.C = Xo[side];
             g.m = M; g.n = K; g.k = K; g.lda = M; g.ldb = K; g.ldc = p->ldco;
-            g.alpha = T(-1); g.beta = T(1); g.flags = GF_A_UNIT_LOWER | GF_C0;
+            g.alpha = T(-1); g.beta = T(1); g.flags = GF_A_UNIT_LOWER | GF_C0 | sl;
             g.C0 = Zs[side]; g.ldc0 = K; g.m0 = K;
             g.n_dev = d_rank + u; g.skip = d_status + u;
             add_tasks(d.tX, (int)gp.size(), g); gp.push_back(g);
@@ -723,6 +751,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
       L.tX = pk.add(d.tX);
       L.o_wy = addv(d.wy); L.n_wy = (int)d.wy.size();
       L.kmax = d.kmax;
+      L.slices = sl ? (p->slices_set ? p->slices_set : default_slices(p->acc)) : 0;
       L.core.clear();
       for (int b = 0; b < NV; ++b) {
         if (!d.core_b[b].empty())
@@ -893,8 +922,9 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
   // lane the longest chain.
   const int syrk_lane = nl >= 2 ? nl - 2 : 0;
   // one GEMM phase: its 64-tile and 32-tile launches
-  auto gemm = [&](const GemmProb<T>* probs, const TaskRef& r, hipStream_t st, bool split = false) {
-    launch_tasks<T>(probs, reinterpret_cast<const char*>(p->desc), r, st, split);
+  auto gemm = [&](const GemmProb<T>* probs, const TaskRef& r, hipStream_t st, bool split = false,
+                  int slices = 0) {
+    launch_tasks<T>(probs, reinterpret_cast<const char*>(p->desc), r, st, split, slices);
   };
   // phases whose products reduce over the nb rows of the tiles (F = AV^T BV,
   // the stacks' Gram matrices next to T) share each tile's k range
@@ -943,8 +973,8 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
       launch_stack<T>(reinterpret_cast<const StackProb<T>*>(D(L.o_stack)), L.n, M, L.stack_cols,
                       T(p->alpha), T(p->beta), ls);
     });
-    phase(l, 1, ls, true, [&] { gemm(gp, L.tF, ls, split_k); });
-    phase(l, 2, ls, true, [&] { gemm(gp, L.tG, ls); });
+    phase(l, 1, ls, true, [&] { gemm(gp, L.tF, ls, split_k, L.slices); });
+    phase(l, 2, ls, true, [&] { gemm(gp, L.tG, ls, false, L.slices); });
     bool shapes_ok = true;
     phase(l, 3, ls, true, [&] {
       shapes_ok &= launch_qr_fused<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qrp)), L.n_qrp, M, ls);
@@ -956,7 +986,7 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
       fprintf(stderr, "hcore_amd: register QR routed a stack of %d rows beyond its shapes\n", M);
       return HCORE_HIP_ERR_ARG;
     }
-    phase(l, 4, ls, true, [&] { gemm(gp, L.tT, ls, split_k && L.gram); });
+    phase(l, 4, ls, true, [&] { gemm(gp, L.tT, ls, split_k && L.gram, L.slices); });
     // the lane's K buckets, largest first (each bucket a kernel variant with its
     // own LDS size); at most 4 streams per plan, one per hardware queue
     phase(l, 5, ls, !L.core.empty(), [&] {
@@ -971,15 +1001,15 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
       }
       launch_convert(reinterpret_cast<const ConvProb*>(D(L.o_cvout)), L.n_cv, L.cv_max, false, ls);
     });
-    phase(l, 7, ls, true, [&] { gemm(gp, L.tZ, ls); });
+    phase(l, 7, ls, true, [&] { gemm(gp, L.tZ, ls, false, L.slices); });
     }
     if (second) {
     phase(l, 6, ls, true, [&] {
-      gemm(gp, L.tB, ls);
+      gemm(gp, L.tB, ls, false, L.slices);
       launch_wy_solve<T>(reinterpret_cast<const WySolveProb<T>*>(D(L.o_wy)), L.n_wy, L.kmax,
                          L.kmax, ls);
     });
-    phase(l, 8, ls, true, [&] { gemm(gp, L.tX, ls); });
+    phase(l, 8, ls, true, [&] { gemm(gp, L.tX, ls, false, L.slices); });
     }
     if (l == syrk_lane && part == 0) syrk(ls);
     if (own) (void)hipEventRecord(L.ev_done, ls);
@@ -2157,6 +2187,44 @@ int HCORE_hip_gemm(int precision, int transA, int transB, int m, int n, int k, d
   return precision == HCORE_HIP_FP64 ? run(0.0) : run(0.0f);
 }
 
+// The split-precision product on its own: C = alpha op(A) op(B) + beta C for
+// FP64 operands through `slices` BF16 digits (0: the same tiles on the FP64
+// matrix cores).
+int HCORE_hip_gemm_split(int slices, int transA, int transB, int m, int n, int k, double alpha,
+                         const double* A, int lda, const double* B, int ldb, double beta, double* C,
+                         int ldc, void* stream) {
+  if (slices != 0 && (slices < kOzMinSlices || slices > kOzMaxSlices)) return HCORE_HIP_ERR_ARG;
+  if (m < 0 || n < 0 || k < 0) return HCORE_HIP_ERR_ARG;
+  const bool ta = transA == kTrans || transA == kConjTrans, tb = transB == kTrans || transB == kConjTrans;
+  if (lda < std::max(1, ta ? k : m) || ldb < std::max(1, tb ? n : k) || ldc < std::max(1, m))
+    return HCORE_HIP_ERR_ARG;
+  if (m == 0 || n == 0) return HCORE_HIP_OK;
+  int st = ctx_ensure();
+  if (st) return st;
+  ThreadCtx* tc = thread_ctx();
+  if (!tc) return HCORE_HIP_ERR_RUNTIME;
+  hipStream_t s = stream ? (hipStream_t)stream : tc->stream;
+  GemmProb<double> g{};
+  g.A = A; g.B = B; g.C = C;
+  g.m = m; g.n = n; g.k = k; g.lda = lda; g.ldb = ldb; g.ldc = ldc;
+  g.alpha = alpha; g.beta = beta;
+  g.flags = (ta ? GF_A_TRANS : 0) | (tb ? GF_B_TRANS : 0) | (beta == 0.0 ? GF_BETA_ZERO : 0) | GF_SLICED;
+  TaskList tl;
+  add_tasks(tl, 0, g);
+  Packer pk;
+  const size_t o_gp = pk.add(std::vector<GemmProb<double>>{g});
+  const TaskRef ref = pk.add(tl);
+  st = tc->chain.reserve(pk.buf.size() + 256);
+  if (st) return st;
+  char* d = tc->chain.p;
+  tc->desc.swap(pk.buf);
+  HCHECK(hipMemcpyAsync(d, tc->desc.data(), tc->desc.size(), hipMemcpyHostToDevice, s));
+  launch_tasks<double>(reinterpret_cast<const GemmProb<double>*>(d + o_gp), d, ref, s, false, slices);
+  HCHECK(hipGetLastError());
+  HCHECK(hipStreamSynchronize(s));
+  return HCORE_HIP_OK;
+}
+
 // ---- context -----------------------------------------------------------------
 int HCORE_hip_init(int device) {
   std::lock_guard<std::mutex> lk(g_ctx.mu);
@@ -2185,7 +2253,8 @@ int HCORE_hip_last_status(void) { return t_last_status; }
 // ---- plans -----------------------------------------------------------------------
 int HCORE_hip_plan_create(hcore_hip_plan** plan, int M, int precision) {
   if (!plan || M <= 0) return HCORE_HIP_ERR_ARG;
-  if (precision != HCORE_HIP_FP64 && precision != HCORE_HIP_FP32 && precision != HCORE_HIP_MIXED)
+  if (precision != HCORE_HIP_FP64 && precision != HCORE_HIP_FP32 && precision != HCORE_HIP_MIXED &&
+      precision != HCORE_HIP_SPLIT)
     return HCORE_HIP_ERR_ARG;
   int st = ctx_ensure();
   if (st) return st;
@@ -2283,13 +2352,13 @@ int HCORE_hip_plan_add_dsyrk(hcore_hip_plan* p, int n, void* const* AU, void* co
 
 int HCORE_hip_plan_finalize(hcore_hip_plan* p) {
   if (!p || p->finalized) return HCORE_HIP_ERR_ARG;
-  return p->prec == HCORE_HIP_FP64 ? plan_finalize_t<double>(p) : plan_finalize_t<float>(p);
+  return plan_double(p) ? plan_finalize_t<double>(p) : plan_finalize_t<float>(p);
 }
 
 int HCORE_hip_plan_execute(hcore_hip_plan* p, void* stream) {
   if (!p || !p->finalized) return HCORE_HIP_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
-  return p->prec == HCORE_HIP_FP64 ? plan_execute_t<double>(p, s) : plan_execute_t<float>(p, s);
+  return plan_double(p) ? plan_execute_t<double>(p, s) : plan_execute_t<float>(p, s);
 }
 
 int HCORE_hip_plan_execute_graph(hcore_hip_plan* p, void* stream) {
@@ -2299,8 +2368,8 @@ int HCORE_hip_plan_execute_graph(hcore_hip_plan* p, void* stream) {
     hipStream_t cs = nullptr;
     HCHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     HCHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    int st = p->prec == HCORE_HIP_FP64 ? plan_execute_t<double>(p, cs, true)
-                                       : plan_execute_t<float>(p, cs, true);
+    int st = plan_double(p) ? plan_execute_t<double>(p, cs, true)
+                            : plan_execute_t<float>(p, cs, true);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(cs, &g);
     (void)hipStreamDestroy(cs);
@@ -2311,6 +2380,106 @@ int HCORE_hip_plan_execute_graph(hcore_hip_plan* p, void* stream) {
   }
   HCHECK(hipGraphLaunch(p->gexec, s));
   return HCORE_HIP_OK;
+}
+
+static int plan_ranks_in(hcore_hip_plan* p, std::vector<int>& r3);
+
+int HCORE_hip_plan_set_slices(hcore_hip_plan* p, int slices) {
+  if (!p || p->prec != HCORE_HIP_SPLIT) return HCORE_HIP_ERR_ARG;
+  if (slices != 0 && (slices < kOzMinSlices || slices > kOzMaxSlices)) return HCORE_HIP_ERR_ARG;
+  p->slices_set = slices;
+  if (p->finalized) {
+    const int s = slices ? slices : default_slices(p->acc);
+    for (auto& L : p->lanes) L.slices = s;
+    drop_graph(p);
+  }
+  return HCORE_HIP_OK;
+}
+
+int HCORE_hip_plan_get_slices(hcore_hip_plan* p, int* out, int nlanes) {
+  if (!p || !p->finalized) return -HCORE_HIP_ERR_ARG;
+  const int nl = (int)p->lanes.size();
+  if (out)
+    for (int l = 0; l < std::min(nl, nlanes); ++l) out[l] = p->lanes[l].slices;
+  return nl;
+}
+
+// Split-precision slice autotuner: the plan's first part (up to the core SVD;
+// it reads the inputs and writes only workspace, ranks and statuses) with the
+// sliced products through the FP64 matrix cores, then with 2..6 digits; per
+// lane the smallest count whose core singular values stay within tol * acc of
+// the FP64 ones.  Ranks, statuses and sweep counts are restored afterwards.
+int HCORE_hip_plan_autotune_slices(hcore_hip_plan* p, void* stream, double tol, double* report,
+                                   int nrep) {
+  if (!p || !p->finalized || p->prec != HCORE_HIP_SPLIT || p->arena || p->fixed)
+    return -HCORE_HIP_ERR_ARG;
+  if (!(tol > 0)) tol = 0.1;
+  hipStream_t s = (hipStream_t)stream;
+  const int n = p->ng, nl = (int)p->lanes.size();
+  if (n == 0 || nl == 0) return 0;
+  std::vector<int> r3;
+  int st = plan_ranks_in(p, r3);
+  if (st) return -st;
+  // the host copy of every update's singular values after one part-1 run
+  std::vector<size_t> so(n + 1, 0);
+  for (int u = 0; u < n; ++u) so[u + 1] = so[u] + (size_t)(r3[3 * u] + r3[3 * u + 2]);
+  auto sigmas = [&](std::vector<double>& out) -> int {
+    int e = plan_execute_t<double>(p, s, false, 1);
+    if (e) return e;
+    HCHECK(hipStreamSynchronize(s));
+    out.assign(so[n], 0.0);
+    for (int u = 0; u < n; ++u)
+      if (so[u + 1] > so[u])
+        HCHECK(hipMemcpy(out.data() + so[u], reinterpret_cast<double*>(p->ws) + p->sigma_off[u],
+                         sizeof(double) * (so[u + 1] - so[u]), hipMemcpyDeviceToHost));
+    return HCORE_HIP_OK;
+  };
+  int* save = nullptr;
+  HCHECK(hipDeviceSynchronize());
+  HCHECK(hipMalloc(&save, sizeof(int) * 3 * n));
+  HCHECK(hipMemcpy(save, p->d_rank, sizeof(int) * 3 * n, hipMemcpyDeviceToDevice));
+  std::vector<int> keep(nl);
+  for (int l = 0; l < nl; ++l) keep[l] = p->lanes[l].slices;
+  const double thr = tol * (p->acc > 0 ? p->acc : 1e-12);
+  std::vector<double> ref, got;
+  // dev[l][c]: the lane's largest deviation with c digits
+  std::vector<std::vector<double>> dev(nl, std::vector<double>(kOzMaxSlices + 1, -1.0));
+  for (auto& L : p->lanes) L.slices = 0;
+  st = sigmas(ref);
+  for (int c = kOzMinSlices; c <= kOzMaxSlices && !st; ++c) {
+    for (auto& L : p->lanes) L.slices = c;
+    st = sigmas(got);
+    if (st) break;
+    for (int u = 0; u < n; ++u) {
+      double d = 0.0;
+      for (size_t i = so[u]; i < so[u + 1]; ++i) d = std::max(d, std::fabs(got[i] - ref[i]));
+      if (!(d == d)) d = INFINITY;  // NaN from a non-finite input: never within tolerance
+      double& e = dev[p->lane_of[u]][c];
+      e = std::max(e, d);
+    }
+  }
+  int nr = 0;
+  for (int l = 0; l < nl; ++l) {
+    int pick = kOzMaxSlices;
+    for (int c = kOzMinSlices; c <= kOzMaxSlices; ++c)
+      if (dev[l][c] >= 0 && dev[l][c] <= thr) { pick = c; break; }
+    p->lanes[l].slices = st ? keep[l] : pick;
+    if (report && nr < nrep) {
+      double* q = report + 4 * nr;
+      q[0] = l;
+      q[1] = p->lanes[l].slices;
+      q[2] = dev[l][p->lanes[l].slices] / (p->acc > 0 ? p->acc : 1.0);
+      q[3] = p->lanes[l].slices > kOzMinSlices
+                 ? dev[l][p->lanes[l].slices - 1] / (p->acc > 0 ? p->acc : 1.0)
+                 : -1.0;
+    }
+    ++nr;
+  }
+  HCHECK(hipMemcpy(p->d_rank, save, sizeof(int) * 3 * n, hipMemcpyDeviceToDevice));
+  HCHECK(hipDeviceSynchronize());
+  (void)hipFree(save);
+  drop_graph(p);
+  return st ? -st : nr;
 }
 
 }  // extern "C"
@@ -2369,11 +2538,7 @@ static int plan_autotune_t(hcore_hip_plan* p, hipStream_t s, int reps, double* r
   (void)hipFree(save);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  // a captured graph holds the old launches: capture again on the next replay
-  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
-  if (p->graph) (void)hipGraphDestroy(p->graph);
-  p->gexec = nullptr;
-  p->graph = nullptr;
+  drop_graph(p);
   return nr;
 }
 
@@ -2383,7 +2548,7 @@ extern "C" {
 
 int HCORE_hip_plan_autotune(hcore_hip_plan* p, void* stream, int reps, double* report, int nrep) {
   if (!p || !p->finalized || p->arena || p->fixed || reps < 1) return -HCORE_HIP_ERR_ARG;
-  const int r = p->prec == HCORE_HIP_FP64
+  const int r = plan_double(p)
                     ? plan_autotune_t<double>(p, (hipStream_t)stream, reps, report, nrep)
                     : plan_autotune_t<float>(p, (hipStream_t)stream, reps, report, nrep);
   return r;

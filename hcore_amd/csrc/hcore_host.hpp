@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "hcore_hip.h"
@@ -78,14 +79,15 @@ struct PinnedArena {
 // ------------------------------------------------------------------------------
 // Output tiles of one batched GEMM phase: 64 x 64 tiles, or 32 x 32 tiles
 // (one wavefront each) for problems with m or n <= 32, which would otherwise
-// compute mostly padding.
+// compute mostly padding; the split-precision products (GF_SLICED) as 32 x 32
+// tiles with one wavefront (k <= 64) or four sharing the k range.
 struct TaskList {
-  std::vector<GemmTask> big, small, tall;
-  size_t size() const { return big.size() + small.size() + tall.size(); }
+  std::vector<GemmTask> big, small, tall, oz1, oz4;
+  size_t size() const { return big.size() + small.size() + tall.size() + oz1.size() + oz4.size(); }
 };
 struct TaskRef {
-  size_t ob = 0, os = 0, ot = 0;
-  int nb = 0, ns = 0, nt = 0;
+  size_t ob = 0, os = 0, ot = 0, o1 = 0, o4 = 0;
+  int nb = 0, ns = 0, nt = 0, n1 = 0, n4 = 0;
 };
 
 struct Packer {
@@ -105,6 +107,10 @@ struct Packer {
     r.ns = (int)tl.small.size();
     r.ot = add(tl.tall);
     r.nt = (int)tl.tall.size();
+    r.o1 = add(tl.oz1);
+    r.n1 = (int)tl.oz1.size();
+    r.o4 = add(tl.oz4);
+    r.n4 = (int)tl.oz4.size();
     return r;
   }
 };
@@ -115,6 +121,12 @@ struct Packer {
 template <typename T>
 void add_tasks(TaskList& tl, int prob, const GemmProb<T>& g) {
   const int m = g.m, n = g.n;
+  if (g.flags & GF_SLICED) {
+    auto& v = g.k > 64 ? tl.oz4 : tl.oz1;
+    for (int tj = 0; tj < (n + 31) / 32; ++tj)
+      for (int ti = 0; ti < (m + 31) / 32; ++ti) v.push_back({prob, ti, tj});
+    return;
+  }
   const bool tall = !(g.flags & (GF_A_TRANS | GF_LOWER_ZERO_UPPER | GF_LOWER_ONLY | GF_UPPER_ONLY)) &&
                     m >= kTallMinM &&
                     n <= kTallMaxN && g.k <= kTallMaxK;
@@ -133,13 +145,26 @@ void add_tasks(TaskList& tl, int prob, const GemmProb<T>& g) {
     for (int ti = 0; ti < (m + t - 1) / t; ++ti) v.push_back({prob, ti, tj});
 }
 
-// The launches of one task list: 64-tiles, 32-tiles, tall-skinny blocks.
+// The launches of one task list: 64-tiles, 32-tiles, tall-skinny blocks, and
+// the split-precision tiles with `slices` digits (0: the same tiles through
+// the FP64 matrix-core kernels, a plan's FP64 reference for the autotuner).
 template <typename T>
 void launch_tasks(const GemmProb<T>* probs, const char* base, const TaskRef& r,
-                         hipStream_t s, bool split) {
+                         hipStream_t s, bool split, int slices = 0) {
   launch_gemm<T>(probs, reinterpret_cast<const GemmTask*>(base + r.ob), r.nb, 64, s, split);
   launch_gemm<T>(probs, reinterpret_cast<const GemmTask*>(base + r.os), r.ns, 32, s, split);
   launch_gemm_tall<T>(probs, reinterpret_cast<const GemmTask*>(base + r.ot), r.nt, s);
+  if constexpr (std::is_same<T, double>::value) {
+    const auto* t1 = reinterpret_cast<const GemmTask*>(base + r.o1);
+    const auto* t4 = reinterpret_cast<const GemmTask*>(base + r.o4);
+    if (slices > 0) {
+      launch_gemm_oz(probs, t1, r.n1, slices, 1, s);
+      launch_gemm_oz(probs, t4, r.n4, slices, 4, s);
+    } else {
+      launch_gemm<T>(probs, t1, r.n1, 32, s, false);
+      launch_gemm<T>(probs, t4, r.n4, 32, s, true);
+    }
+  }
 }
 
 // ---- services of the single-call entry points (hcore_api.cpp) ----------------

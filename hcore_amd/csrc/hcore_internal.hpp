@@ -35,6 +35,7 @@ enum : int {
   GF_C0 = 256,              // the beta term reads C0 (rows < m0; 0 below) instead of C
   GF_LOWER_ONLY = 512,      // write i >= j only; the strict upper triangle of C is not touched
   GF_UPPER_ONLY = 1024,     // write i <= j only; the strict lower triangle of C is not touched
+  GF_SLICED = 2048,         // FP64 product through BF16 digit slices (k_gemm_oz)
 };
 
 template <typename T>
@@ -144,6 +145,12 @@ constexpr int kTallRows = 256, kTallCols = 32, kTallK = 64;
 constexpr int kTallMinM = 256, kTallMaxN = 128, kTallMaxK = 128;
 template <typename T>
 void launch_gemm_tall(const GemmProb<T>* probs, const GemmTask* tasks, int ntasks, hipStream_t s);
+// split-precision FP64 products on the BF16 matrix cores: 32 x 32 output tiles
+// (tasks as for tile 32), `slices` digits per operand (kOzMinSlices..
+// kOzMaxSlices), ks = 1 or 4 wavefronts sharing a tile's k range
+constexpr int kOzMinSlices = 2, kOzMaxSlices = 6;
+void launch_gemm_oz(const GemmProb<double>* probs, const GemmTask* tasks, int ntasks, int slices,
+                    int ks, hipStream_t s);
 constexpr int kStackCols = 4;  // stacked columns per workgroup of k_stack
 template <typename T>
 void launch_stack(const StackProb<T>* probs, int n, int nb, int maxcols, T alpha, T beta,

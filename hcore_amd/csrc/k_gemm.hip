@@ -16,7 +16,8 @@ namespace hcore {
 // batched GEMM on the matrix cores.  Every product of the path (the k = nb
 // reductions F = AV^T BV and Y^T Y, the small cores T = R_A R_B^T and T^T Zu,
 // the tall products G = BU F^T and X = [Z; 0] - Y M, and dsyrk's W, X and CD)
-// runs through v_mfma_f64_16x16x4_f64 (FP32 plans: v_mfma_f32_16x16x4_f32):
+// runs through v_mfma_f64_16x16x4_f64 (FP32 plans: v_mfma_f32_16x16x4_f32;
+// the products of split-precision plans: BF16 digit slices, k_gemm_oz below):
 // one wavefront owns a 32 x 32 (or 16-row-block x 32) piece of the output as
 // 16 x 16 accumulator blocks; operands are staged through LDS (2-D tiles) or
 // read straight from HBM with coalesced 16-row segments (tall products).
@@ -389,6 +390,234 @@ __global__ __launch_bounds__(256) void k_gemm_tall(const GemmProb<T>* __restrict
 #pragma unroll
       for (int v = 0; v < 4; ++v)
         gemm_store1(P, rw + 16 * a + li, col0 + 16 * b + Mfma<T>::m_of(lane, v), n, acc[a][b][v]);
+}
+
+// ------------------------------------------------------------------------------
+// Split-precision products (plans of precision HCORE_HIP_SPLIT): FP64 operands
+// multiplied on the BF16 matrix cores (v_mfma_f32_16x16x32_bf16, FP32
+// accumulation), Ozaki style.  For every k chunk of 32 (one matrix-core step)
+// each row i of op(A) and column j of op(B) gets a power of two 2^E just above
+// its largest magnitude in the chunk; the scaled values x in (-1, 1) are cut
+// into S integer digits of at most 8 bits, d_1 = rint(256 x), then the
+// rounded remainders, each exact in BF16:
+//   x = sum_{s=1..S} d_s 2^(-8 s) + rho,  |rho| <= 2^(-8 S - 1).
+// One chunk's digit products are integers whose sums stay below 2^21, so the
+// FP32 accumulation is exact; the pairs (s, t) with s + t <= S + 1 are
+// summed per level s + t and added to FP64 accumulators with the scale
+// 2^(E_i + F_j - 8 (s + t)).  The error of a product term relative to the
+// chunk's row and column maxima is about S 2^(-8 S).
+// One workgroup per 32 x 32 output tile; KS wavefronts share its k range
+// (chunks g, g + KS, ...), their FP64 tiles summed in LDS in a fixed order.
+// ------------------------------------------------------------------------------
+using OzAcc = float __attribute__((ext_vector_type(4)));
+using OzBf8 = __bf16 __attribute__((ext_vector_type(8)));
+using OzU4 = unsigned __attribute__((ext_vector_type(4)));
+
+// Eight integer-valued doubles with |d| <= 256 as BF16: the float of such an
+// integer has at most 8 significant bits, so its upper half is the BF16.
+__device__ __forceinline__ OzBf8 oz_pack(const double (&d)[8]) {
+  OzU4 w;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    w[q] = (__float_as_uint((float)d[2 * q]) >> 16) |
+           (__float_as_uint((float)d[2 * q + 1]) & 0xffff0000u);
+  return __builtin_bit_cast(OzBf8, w);
+}
+
+// The S digits of 8 values scaled by 2^-e.
+template <int S>
+__device__ __forceinline__ void oz_digits(const double (&v)[8], int e, OzBf8 (&out)[S]) {
+  double x[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) x[q] = ldexp(v[q], -e);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    double d[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const double t = x[q] * 256.0;  // exact
+      d[q] = rint(t);
+      x[q] = t - d[q];                // exact: the rounded-off part of t
+    }
+    out[s] = oz_pack(d);
+  }
+}
+
+// Exponent E with max|v| < 2^E over a row (or column) whose 32 chunk values
+// are spread over the four lanes li, li + 16, li + 32, li + 48 (0 when all
+// are zero).
+__device__ __forceinline__ int oz_exponent(const double (&v)[8]) {
+  double m = 0.0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) m = fmax(m, fabs(v[q]));
+  int e = 0;
+  (void)frexp(m, &e);
+  if (m == 0.0) e = -1100;  // below every other exponent: the max picks the rest
+  e = max(e, __shfl_xor(e, 16));
+  e = max(e, __shfl_xor(e, 32));
+  return e == -1100 ? 0 : e;
+}
+
+// One staged k chunk (32 wide) into the wavefront's FP64 32 x 32 accumulators:
+// acc[rb][cb][v] = C(i = 16 rb + lane % 16, j = 16 cb + 4 (lane / 16) + v).
+template <int S, int LDT>
+__device__ __forceinline__ void oz_chunk(const double (*As)[LDT], const double (*Bs)[LDT],
+                                         double (&acc)[2][2][4], int lane) {
+  const int li = lane & 15, lg = lane >> 4;
+  // op(B)'s two 16-column blocks: digits kept for both row blocks of op(A)
+  OzBf8 sb[2][S];
+  int eb[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    double vb[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) vb[q] = Bs[8 * lg + q][16 * h + li];  // op(B)(k = 8 lg + q, j)
+    eb[h] = oz_exponent(vb);
+    oz_digits<S>(vb, eb[h], sb[h]);
+  }
+  // column 16 cb + 4 lg + v's exponent is held by lane 4 lg + v
+  int ej[2][4];
+#pragma unroll
+  for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) ej[cb][v] = __shfl(eb[cb], 4 * lg + v);
+#pragma unroll 1
+  for (int rb = 0; rb < 2; ++rb) {
+    double va[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) va[q] = As[8 * lg + q][16 * rb + li];  // op(A)(i, k = 8 lg + q)
+    const int ea = oz_exponent(va);
+    OzBf8 sa[S];
+    oz_digits<S>(va, ea, sa);
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+      OzAcc lv[S];
+#pragma unroll
+      for (int L = 0; L < S; ++L) {  // level L: the digit pairs (s, L - s), 0-based
+        lv[L] = OzAcc{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s <= L; ++s)
+          lv[L] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sb[cb][L - s], sa[s], lv[L], 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        // sum_L lv[L] 2^(-8 (L + 2)), Horner in FP64
+        double sum = (double)lv[S - 1][v];
+#pragma unroll
+        for (int L = S - 2; L >= 0; --L) sum = fma(sum, 0x1p-8, (double)lv[L][v]);
+        const double c = ldexp(sum, ea + ej[cb][v] - 16);
+        if (rb == 0) acc[0][cb][v] += c;
+        else acc[1][cb][v] += c;
+      }
+    }
+  }
+}
+
+template <int S, int KS>
+__global__ __launch_bounds__(64 * KS) void k_gemm_oz(const GemmProb<double>* __restrict__ probs,
+                                                     const GemmTask* __restrict__ tasks) {
+  constexpr int TILE = 32, NG = 64, GKS = 32;
+  constexpr int LD = TILE * GKS / NG;  // elements of each operand per thread per chunk
+  constexpr int STG = 2 * GKS * (TILE + 1);
+  constexpr int RED = (KS / 2) * TILE * TILE;
+  constexpr int SM = KS * STG > RED ? KS * STG : RED;
+  static_assert(KS == 1 || KS == 2 || KS == 4, "split-k groups");
+  __shared__ double sm[SM];
+  const GemmTask tk = tasks[blockIdx.x];
+  const GemmProb<double> P = probs[tk.prob];
+  if (P.skip && *P.skip) return;
+  int n = P.n;
+  if (P.n_dev) n = min(n, *P.n_dev);
+  const int row0 = tk.ti * TILE, col0 = tk.tj * TILE;
+  if (row0 >= P.m || col0 >= n) return;
+  const int grp = threadIdx.x / NG, lane = threadIdx.x % NG;
+  if (((P.flags & GF_LOWER_ONLY) && row0 + TILE - 1 < col0) ||
+      ((P.flags & GF_UPPER_ONLY) && col0 + TILE - 1 < row0))
+    return;  // the tile lies in the triangle that is not written
+  if ((P.flags & GF_LOWER_ZERO_UPPER) && row0 + TILE - 1 < col0) {
+    for (int e = threadIdx.x; e < TILE * TILE; e += NG * KS) {
+      const int i = row0 + (e % TILE), j = col0 + (e / TILE);
+      if (i < P.m && j < n) P.C[i + (size_t)j * P.ldc] = 0.0;
+    }
+    return;
+  }
+  const bool ta = P.flags & GF_A_TRANS, tb = P.flags & GF_B_TRANS;
+  const bool ua = P.flags & GF_A_UPPER, ub = P.flags & GF_B_UPPER;
+  double (*As)[TILE + 1] = reinterpret_cast<double (*)[TILE + 1]>(sm + grp * STG);
+  double (*Bs)[TILE + 1] = reinterpret_cast<double (*)[TILE + 1]>(sm + grp * STG + GKS * (TILE + 1));
+  double acc[2][2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[a][b][v] = 0.0;
+  const int nsteps = (P.k + GKS * KS - 1) / (GKS * KS);
+  for (int st = 0; st < nsteps; ++st) {
+    {
+      double ra[LD], rb[LD];
+      gemm_fetch<double, TILE, NG, LD, GKS>(P, n, row0, col0, (st * KS + grp) * GKS, ta, tb, ua,
+                                            ub, ra, rb, lane);
+      gemm_stage<double, TILE, NG, LD, GKS, TILE + 1>(As, Bs, ra, rb, ta, tb, lane);
+    }
+    __syncthreads();
+    oz_chunk<S, TILE + 1>(As, Bs, acc, lane);
+    __syncthreads();
+  }
+  const int li = lane & 15, lg = lane >> 4;
+#pragma unroll
+  for (int h = KS / 2; h >= 1; h /= 2) {  // pairwise sum of the groups' tiles
+    if (grp >= h && grp < 2 * h) {
+      double* buf = sm + (size_t)(grp - h) * TILE * TILE;
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) buf[(16 * a + li) + (16 * b + 4 * lg + v) * TILE] = acc[a][b][v];
+    }
+    __syncthreads();
+    if (grp < h) {
+      const double* buf = sm + (size_t)grp * TILE * TILE;
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[a][b][v] += buf[(16 * a + li) + (16 * b + 4 * lg + v) * TILE];
+    }
+    __syncthreads();
+  }
+  if (grp == 0) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          gemm_store1(P, row0 + 16 * a + li, col0 + 16 * b + 4 * lg + v, n, acc[a][b][v]);
+  }
+}
+
+template <int S>
+static void launch_gemm_oz_s(const GemmProb<double>* probs, const GemmTask* tasks, int ntasks,
+                             int ks, hipStream_t s) {
+  if (ks == 4)
+    hipLaunchKernelGGL((k_gemm_oz<S, 4>), dim3(ntasks), dim3(256), 0, s, probs, tasks);
+  else
+    hipLaunchKernelGGL((k_gemm_oz<S, 1>), dim3(ntasks), dim3(64), 0, s, probs, tasks);
+}
+
+void launch_gemm_oz(const GemmProb<double>* probs, const GemmTask* tasks, int ntasks, int slices,
+                    int ks, hipStream_t s) {
+  if (ntasks <= 0) return;
+  switch (slices) {
+    case 2: launch_gemm_oz_s<2>(probs, tasks, ntasks, ks, s); break;
+    case 3: launch_gemm_oz_s<3>(probs, tasks, ntasks, ks, s); break;
+    case 4: launch_gemm_oz_s<4>(probs, tasks, ntasks, ks, s); break;
+    case 5: launch_gemm_oz_s<5>(probs, tasks, ntasks, ks, s); break;
+    default: launch_gemm_oz_s<6>(probs, tasks, ntasks, ks, s); break;
+  }
 }
 
 template <typename T>

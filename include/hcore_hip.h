@@ -36,6 +36,10 @@ extern "C" {
 #define HCORE_HIP_FP32 1   /* everything in FP32 (HCORE_sgemm) */
 #define HCORE_HIP_MIXED 2  /* FP32 tiles, products, QR and reassembly; the core
                               SVD and the truncation test against acc in FP64 */
+#define HCORE_HIP_SPLIT 3  /* FP64 tiles, QR, core SVD and truncation; the
+                              products F, G, T, T^T Zu and the reassembly as
+                              BF16 digit slices on the matrix cores with FP32
+                              accumulation (Ozaki-style split precision) */
 
 /* error policy of the drop-in HCORE_* entry points */
 #define HCORE_HIP_ON_ERROR_EXIT 0    /* reference behaviour: message + exit(-1) */
@@ -130,6 +134,34 @@ int HCORE_hip_plan_execute_graph(hcore_hip_plan *plan, void *stream);
 int HCORE_hip_plan_autotune(hcore_hip_plan *plan, void *stream, int reps, double *report,
                             int nreport);
 
+/*
+ * Split-precision plans (HCORE_HIP_SPLIT; no counterpart in the reference).
+ * Digits per operand of the sliced products, 2..6 (each adds 8 bits; the
+ * error of a product term relative to its chunk's row and column maxima is
+ * about slices * 2^(-8 slices)), for every lane; 0 restores the default,
+ * chosen from acc at finalize (acc 1e-8: 5, 1e-6: 4).  Before or after
+ * finalize (a captured graph is captured again).  HCORE_HIP_ERR_ARG for
+ * another precision form or a count outside 0, 2..6.
+ */
+int HCORE_hip_plan_set_slices(hcore_hip_plan *plan, int slices);
+/* The digit count of each lane after finalize (up to nlanes entries);
+ * returns the number of lanes, or -HCORE_HIP_* on error. */
+int HCORE_hip_plan_get_slices(hcore_hip_plan *plan, int *slices_host, int nlanes);
+/*
+ * Slice autotuner (no counterpart in the reference).  After at least one
+ * execution: runs the plan once with its sliced products through the FP64
+ * matrix cores and once per digit count 2..6, and keeps for each lane (a
+ * stacked-rank bucket) the smallest count whose singular values of every
+ * update's core differ from the FP64 ones by at most tol * acc (tol <= 0:
+ * 0.1).  The plan's outputs are those of its last execution afterwards
+ * (execute again to refresh them).  report (may be NULL) receives up to
+ * nreport entries of 4 doubles per lane: lane, chosen count, its largest
+ * singular-value deviation / acc, the same for one digit fewer (-1 if none).
+ * Returns the number of lanes tuned, or -HCORE_HIP_* on error.
+ */
+int HCORE_hip_plan_autotune_slices(hcore_hip_plan *plan, void *stream, double tol,
+                                   double *report, int nreport);
+
 /* Device array of the n new ranks (int), valid after execution. */
 const int *HCORE_hip_plan_ranks_device(const hcore_hip_plan *plan);
 /* Copy new ranks / per-update status (HCORE_HIP_*) to host; synchronizes. */
@@ -185,6 +217,17 @@ int HCORE_hip_plan_phase_times(hcore_hip_plan *plan, double *ms, int *count);
 int HCORE_hip_gemm(int precision, int transA, int transB, int m, int n, int k, double alpha,
                    const void *A, int lda, const void *B, int ldb, double beta, void *C, int ldc,
                    int path, void *stream);
+/*
+ * The split-precision product of HCORE_HIP_SPLIT plans on its own (no
+ * counterpart in the reference): C = alpha op(A) op(B) + beta C for FP64
+ * operands, each k chunk of 32 cut into `slices` (2..6) BF16 digits per row of
+ * op(A) and column of op(B) under a power-of-two scale, digit products on the
+ * matrix cores with FP32 accumulation, FP64 sums; 0 runs the same output
+ * tiles on the FP64 matrix cores.  Synchronous.
+ */
+int HCORE_hip_gemm_split(int slices, int transA, int transB, int m, int n, int k, double alpha,
+                         const double *A, int lda, const double *B, int ldb, double beta,
+                         double *C, int ldc, void *stream);
 
 /* Tile generation and compression in batches (all pointers device memory,
  * tiles of one size m x nn, max(m, nn) <= 512).  Tile i of the covariance

@@ -242,7 +242,7 @@ def _factored_rel_diff(XU, XV, YU, YV):
     return num / max(np.linalg.norm(R2 @ YV.T), 1e-300)
 
 
-def _run_batch(hc, step, prec, acc, dt):
+def _run_batch(hc, step, prec, acc, dt, tune=None):
     nb = step.nb
     ra, rb, rc = step.ranks()
     K = ra + rc
@@ -265,6 +265,10 @@ def _run_batch(hc, step, prec, acc, dt):
                     [step.panel[i][0].shape[1] for i in step.diag_idx], nb,
                     [t.ptr for t in Dd], nb, -1.0)
     p.finalize()
+    if tune is not None:  # after one execution, then the graph of the tuned plan
+        p.execute()
+        hc.device_synchronize()
+        tune(p)
     p.execute(graph=True)
     hc.device_synchronize()
     status, ranks = p.status(), p.ranks()
@@ -276,7 +280,17 @@ def _run_batch(hc, step, prec, acc, dt):
     return status, ranks, outs, D
 
 
+_ORACLE_BATCH = {}
+
+
 def _oracle_batch(orc, step, acc, dt):
+    key = (id(step), acc, np.dtype(dt).name)
+    if key not in _ORACLE_BATCH:
+        _ORACLE_BATCH[key] = _oracle_batch_run(orc, step, acc, dt)
+    return _ORACLE_BATCH[key]
+
+
+def _oracle_batch_run(orc, step, acc, dt):
     nb = step.nb
     ra, rb, rc = step.ranks()
     K = ra + rc
@@ -342,6 +356,25 @@ def test_full_trailing_step_mixed_acc1e6_vs_oracle(hc, orc, full_step):
     assert np.all(status == 0)
     oranks, oouts = _oracle_batch(orc, step, 1e-6, np.float32)
     _compare_batch(step, ranks, outs, oranks, oouts, 1e-6, "mixed")
+
+
+@pytest.mark.timeout(900)
+def test_full_trailing_step_split_acc1e8_vs_oracle(hc, orc, full_step):
+    """The split-precision plan (FP64 tiles, Householder QR, core SVD and
+    truncation; the products F, G, T, T^T Zu and the reassembly as BF16 digit
+    slices on the matrix cores, digit counts per lane from the slice
+    autotuner) on the 1953-update gemm batch at acc 1e-8: every product within
+    10*acc of the oracle's on the same FP64 tiles, ranks within one."""
+    step = full_step
+    _say("case: full trailing step nb 1024 split acc 1e-8")
+    picked = []
+    status, ranks, outs, _ = _run_batch(hc, step, hc.SPLIT, 1e-8, np.float64,
+                                        tune=lambda p: picked.extend(p.autotune_slices()))
+    _say("   slices per lane:", [(int(t[0]), int(t[1]), round(t[2], 4)) for t in picked])
+    assert np.all(status == 0)
+    assert all(2 <= t[1] <= 6 and t[2] <= 0.1 for t in picked)
+    oranks, oouts = _oracle_batch(orc, step, 1e-8, np.float64)
+    _compare_batch(step, ranks, outs, oranks, oouts, 1e-8, "split")
 
 
 def test_concurrent_single_calls_match_sequential(hc):
