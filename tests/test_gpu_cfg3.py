@@ -4,9 +4,10 @@ every update of one plan against the oracle.
 
 Bars: status 0 everywhere; each update's rank within one of the oracle's
 and its product within 10*acc (relative Frobenius) of the oracle's product;
-a sample within 10*acc of the dense update.  This exercises the blocked
-Householder QR (nb > 1024), the core SVD beyond the register-resident
-shapes (K 193-506) and the per-panel reassembly (K > 256) at scale.
+a sample within 10*acc of the dense update.  This exercises the fused
+register QR at nb = 2048 (four rows per thread), the core SVD beyond the
+register-resident shapes (K 193-506) and the compact-WY reassembly with the
+16-column solve (K > 256) at scale.
 """
 import numpy as np
 import pytest

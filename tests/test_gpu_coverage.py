@@ -1,5 +1,5 @@
 """GPU parity beyond the basic shapes: large stacked ranks (the global-memory
-core SVD variants and the per-panel reassembly), zero singular values,
+core SVD variants and the 16-column compact-WY solve), zero singular values,
 argument handling of the drop-in and plan entry points, and the full nb=1024
 trailing step (1953 HCORE_dgemm + 63 HCORE_dsyrk) in FP64 and in the mixed
 form, every update against the oracle.
@@ -43,8 +43,8 @@ def _check_single(hc, orc, case, device=False):
 
 
 # stacked ranks K = ra + rc beyond 192: the global-memory core variants (K <= 256
-# and K <= 512), the compact-WY reassembly up to K = 256 and the per-panel
-# reflector application beyond it
+# and K <= 512), the compact-WY solve with 64 columns per workgroup up to K = 256
+# and with 16 beyond it
 LARGE_K = [  # (seed, M, ra, rb, rc, rk, acc)
     (51, 1024, 100, 90, 93, 0, 1e-8),    # K = 193
     (52, 1024, 128, 100, 128, 0, 1e-8),  # K = 256
@@ -72,7 +72,7 @@ def test_register_qr_two_rows_k16(hc, orc, case):
 
 def test_large_stacked_rank_plan(hc, orc):
     """Four updates with K = 193, 256, 300 and 512 at nb = 2048 in one plan:
-    one lane, core variants by K, per-panel reassembly for the whole lane."""
+    one lane, core variants by K, the 16-column compact-WY solve for the lane."""
     M = 2048
     cases = [(61, 100, 90, 93), (62, 128, 100, 128), (63, 150, 120, 150), (64, 256, 200, 256)]
     _say("case: large K plan nb", M)
