@@ -621,7 +621,8 @@ static int plan_finalize_t(hcore_hip_plan* p) {
             g.A = Ys[side]; g.B = Ys[side]; g.C = Ss[side];
             g.m = K; g.n = K; g.k = M; g.lda = M; g.ldb = M; g.ldc = K;
             g.alpha = T(1); g.beta = T(0);
-            g.flags = GF_A_TRANS | GF_A_UNIT_LOWER | GF_B_UNIT_LOWER | GF_BETA_ZERO;
+            // only its strict lower triangle is read (the compact-WY solve)
+            g.flags = GF_A_TRANS | GF_A_UNIT_LOWER | GF_B_UNIT_LOWER | GF_BETA_ZERO | GF_LOWER_ONLY;
             if (!qreg) {
               add_tasks(d.tT, (int)gp.size(), g);
               gp.push_back(g);

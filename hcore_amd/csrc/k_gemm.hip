@@ -97,6 +97,33 @@ __device__ __forceinline__ void gemm_fetch(const GemmProb<T>& P, int n, int row0
   }
 }
 
+// The k range [lo, hi) of the output tile (TILE rows from row0, TILE columns
+// from col0) outside of which op(A)(i, k) or op(B)(k, j) vanish by the
+// triangular flags (upper: stored (p, q) zero for p > q; unit lower: zero for
+// p < q), lo rounded down to a multiple of the chunk GKX: T = R_A R_B^T needs
+// k >= max(i, j), the compact-WY products Y1^T Z k >= i.
+template <int TILE, int GKX>
+__device__ __forceinline__ void gemm_k_range(int flags, int row0, int col0, int k, int& lo,
+                                             int& hi) {
+  const bool ta = (flags & GF_A_TRANS) != 0, tb = (flags & GF_B_TRANS) != 0;
+  lo = 0;
+  hi = k;
+  if (flags & GF_A_UPPER) {
+    if (ta) hi = min(hi, row0 + TILE); else lo = max(lo, row0);
+  }
+  if (flags & GF_A_UNIT_LOWER) {
+    if (ta) lo = max(lo, row0); else hi = min(hi, row0 + TILE);
+  }
+  if (flags & GF_B_UPPER) {
+    if (tb) lo = max(lo, col0); else hi = min(hi, col0 + TILE);
+  }
+  if (flags & GF_B_UNIT_LOWER) {
+    if (tb) hi = min(hi, col0 + TILE); else lo = max(lo, col0);
+  }
+  lo = (lo / GKX) * GKX;
+  if (hi < lo) hi = lo;
+}
+
 // Registers -> LDS: As[kk][i] = op(A)(i, k0 + kk), Bs[kk][j] = op(B)(k0 + kk, j).
 template <typename T, int TILE, int NT, int LD, int GKX, int LDT>
 __device__ __forceinline__ void gemm_stage(T (*As)[LDT], T (*Bs)[LDT], const T (&ra)[LD],
@@ -205,12 +232,14 @@ __global__ __launch_bounds__((TILE / 4) * (TILE / 4)) void k_gemm(
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = V{T(0), T(0), T(0), T(0)};
 
+  int klo, khi;
+  gemm_k_range<TILE, GKT>(P.flags, row0, col0, P.k, klo, khi);
   T ra[LD], rb[LD];
-  gemm_fetch<T, TILE, NT, LD, GKT>(P, n, row0, col0, 0, ta, tb, ua, ub, ra, rb, t);
-  for (int k0 = 0; k0 < P.k; k0 += GKT) {
+  gemm_fetch<T, TILE, NT, LD, GKT>(P, n, row0, col0, klo, ta, tb, ua, ub, ra, rb, t);
+  for (int k0 = klo; k0 < khi; k0 += GKT) {
     gemm_stage<T, TILE, NT, LD, GKT, TILE + 1>(As, Bs, ra, rb, ta, tb, t);
     __syncthreads();
-    if (k0 + GKT < P.k)
+    if (k0 + GKT < khi)
       gemm_fetch<T, TILE, NT, LD, GKT>(P, n, row0, col0, k0 + GKT, ta, tb, ua, ub, ra, rb, t);
     gemm_mfma_chunk<T, GKT, TILE + 1>(As, Bs, acc, r0w, c0w, lane);
     __syncthreads();
@@ -266,14 +295,16 @@ __global__ __launch_bounds__(64 * KS) void k_gemm_sk(const GemmProb<T>* __restri
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = V{T(0), T(0), T(0), T(0)};
-  const int nsteps = (P.k + GKS * KS - 1) / (GKS * KS);
+  int klo, khi;
+  gemm_k_range<TILE, GKS>(P.flags, row0, col0, P.k, klo, khi);
+  const int nsteps = (khi - klo + GKS * KS - 1) / (GKS * KS);
   T ra[LD], rb[LD];
-  gemm_fetch<T, TILE, NG, LD, GKS>(P, n, row0, col0, grp * GKS, ta, tb, ua, ub, ra, rb, t);
+  gemm_fetch<T, TILE, NG, LD, GKS>(P, n, row0, col0, klo + grp * GKS, ta, tb, ua, ub, ra, rb, t);
   for (int st = 0; st < nsteps; ++st) {
     gemm_stage<T, TILE, NG, LD, GKS, TILE + 1>(As, Bs, ra, rb, ta, tb, t);
     __syncthreads();
     if (st + 1 < nsteps)
-      gemm_fetch<T, TILE, NG, LD, GKS>(P, n, row0, col0, ((st + 1) * KS + grp) * GKS, ta, tb, ua,
+      gemm_fetch<T, TILE, NG, LD, GKS>(P, n, row0, col0, klo + ((st + 1) * KS + grp) * GKS, ta, tb, ua,
                                        ub, ra, rb, t);
     gemm_mfma_chunk<T, GKS, TILE + 1>(As, Bs, acc, 0, 0, lane);
     __syncthreads();
@@ -552,11 +583,13 @@ __global__ __launch_bounds__(64 * KS) void k_gemm_oz(const GemmProb<double>* __r
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int v = 0; v < 4; ++v) acc[a][b][v] = 0.0;
-  const int nsteps = (P.k + GKS * KS - 1) / (GKS * KS);
+  int klo, khi;
+  gemm_k_range<TILE, GKS>(P.flags, row0, col0, P.k, klo, khi);
+  const int nsteps = (khi - klo + GKS * KS - 1) / (GKS * KS);
   for (int st = 0; st < nsteps; ++st) {
     {
       double ra[LD], rb[LD];
-      gemm_fetch<double, TILE, NG, LD, GKS>(P, n, row0, col0, (st * KS + grp) * GKS, ta, tb, ua,
+      gemm_fetch<double, TILE, NG, LD, GKS>(P, n, row0, col0, klo + (st * KS + grp) * GKS, ta, tb, ua,
                                             ub, ra, rb, lane);
       gemm_stage<double, TILE, NG, LD, GKS, TILE + 1>(As, Bs, ra, rb, ta, tb, lane);
     }
