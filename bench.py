@@ -326,47 +326,57 @@ def batch2048_leg(steps, warmup):
 def tlr_leg(bigM, nb):
     """Generation, compression and TLR Cholesky of the covariance problem
     (bigM Morton-sorted points, exp(-d/0.1) + 1e-4 I, acc 1e-8, compression
-    maxrank nb/2, recompression maxrank nb), all on the GPU; the factor is
-    checked on sampled entries of L L^T against the matrix (numpy, from the
-    library's points)."""
+    maxrank nb/2, recompression maxrank nb), all on the GPU, with FP64
+    recompressions and again (regenerated) with split-precision ones; each
+    factor is checked on sampled entries of L L^T against the matrix (numpy,
+    from the library's points)."""
     import hcore_amd as hc
     from hcore_amd.tlr import TlrMatrix
     acc = 1e-8
-    M = TlrMatrix(bigM, nb, 2024, maxrank=nb // 2, acc=acc, cap=nb)
-    t = time.time()
-    M.generate()
-    t_gen = time.time() - t
-    r0 = M.ranks.copy()
-    nt = M.nt
-    low = [(i, k) for k in range(nt) for i in range(k + 1, nt)]
-    t = time.time()
-    st, info, smax = M.factor(maxrk=nb, acc=acc)
-    t_fac = time.time() - t
-    assert st == 0 and info == 0, (st, info)
-    fl = M.flops
     rng = np.random.default_rng(7)
-    log("tlr cholesky %d points nb %d: generate %.2f s, factor %.2f s" % (bigM, nb, t_gen, t_fac))
     rows = sorted(int(g) for g in rng.choice(bigM, 6, replace=False))
     P = hc.covariance_points(bigM, 2024)
-    Lr = {g: M.row(g) for g in rows}
-    worst = 0.0
-    for a in rows:
-        for b in rows:
-            d = np.hypot(*(P[a] - P[b]))
-            ref = np.exp(-d / 0.1) + (1e-4 if a == b else 0.0)
-            worst = max(worst, abs(Lr[a] @ Lr[b] - ref))
-    M.free()
-    res = {"points": bigM, "nb": nb, "tiles": nt, "gemm_updates": nt * (nt - 1) * (nt - 2) // 6,
-           "syrk_updates": nt * (nt - 1) // 2, "generate_s": t_gen, "factor_s": t_fac,
-           "factor_gflops": (fl.update + fl.syrk + fl.trsm + fl.potrf) / t_fac / 1e9,
-           "rank_max_generated": int(max(r0[i, k] for i, k in low)),
-           "rank_mean_generated": float(np.mean([r0[i, k] for i, k in low])),
-           "rank_max_per_step": [int(x) for x in smax[:-1]],
-           "sampled_LLt_max_abs_err": worst}
-    log("tlr cholesky %d points nb %d: generate %.2f s, factor %.2f s (%.0f GF/s), ranks mean %.1f "
-        "max %d, sampled |LL^T - A| %.2e" % (bigM, nb, t_gen, t_fac, res["factor_gflops"],
-                                             res["rank_mean_generated"],
-                                             res["rank_max_generated"], worst))
+    res = None
+    for form, prec in (("fp64", hc.FP64), ("split", hc.SPLIT)):
+        M = TlrMatrix(bigM, nb, 2024, maxrank=nb // 2, acc=acc, cap=nb)
+        t = time.time()
+        M.generate()
+        t_gen = time.time() - t
+        r0 = M.ranks.copy()
+        nt = M.nt
+        low = [(i, k) for k in range(nt) for i in range(k + 1, nt)]
+        t = time.time()
+        st, info, smax = M.factor(maxrk=nb, acc=acc, precision=prec)
+        t_fac = time.time() - t
+        assert st == 0 and info == 0, (st, info)
+        fl = M.flops
+        Lr = {g: M.row(g) for g in rows}
+        worst = 0.0
+        for a in rows:
+            for b in rows:
+                d = np.hypot(*(P[a] - P[b]))
+                ref = np.exp(-d / 0.1) + (1e-4 if a == b else 0.0)
+                worst = max(worst, abs(Lr[a] @ Lr[b] - ref))
+        M.free()
+        if form == "fp64":
+            res = {"points": bigM, "nb": nb, "tiles": nt,
+                   "gemm_updates": nt * (nt - 1) * (nt - 2) // 6,
+                   "syrk_updates": nt * (nt - 1) // 2, "generate_s": t_gen, "factor_s": t_fac,
+                   "factor_gflops": (fl.update + fl.syrk + fl.trsm + fl.potrf) / t_fac / 1e9,
+                   "rank_max_generated": int(max(r0[i, k] for i, k in low)),
+                   "rank_mean_generated": float(np.mean([r0[i, k] for i, k in low])),
+                   "rank_max_per_step": [int(x) for x in smax[:-1]],
+                   "sampled_LLt_max_abs_err": worst}
+            log("tlr cholesky %d points nb %d: generate %.2f s, factor %.2f s (%.0f GF/s), ranks "
+                "mean %.1f max %d, sampled |LL^T - A| %.2e" %
+                (bigM, nb, t_gen, t_fac, res["factor_gflops"], res["rank_mean_generated"],
+                 res["rank_max_generated"], worst))
+        else:
+            res["split"] = {"factor_s": t_fac, "split_over_fp64": res["factor_s"] / t_fac,
+                            "sampled_LLt_max_abs_err": worst,
+                            "rank_max_per_step": [int(x) for x in smax[:-1]]}
+            log("tlr cholesky split recompressions: factor %.2f s (split/fp64 %.2fx), sampled "
+                "|LL^T - A| %.2e" % (t_fac, res["factor_s"] / t_fac, worst))
     return res
 
 

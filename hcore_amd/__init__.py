@@ -88,6 +88,8 @@ def lib():
         L.HCORE_hip_compress_batch.argtypes = [i, i, i, _pp, i, _pp, i, _pp, i, _vp, i, d, _vp]
         L.HCORE_hip_set_covariance.argtypes = [d, d]
         L.HCORE_hip_tlr_dpotrf.argtypes = [i, i, _pp, _pp, _pp, _ip, i, i, d, _vp, _ip, fl, _ip]
+        L.HCORE_hip_tlr_dpotrf_form.argtypes = [i, i, i, i, _pp, _pp, _pp, _ip, i, i, d, _vp, _ip,
+                                                fl, _ip]
         L.HCORE_hip_covariance_points.argtypes = [i, C.c_ulonglong, _dp]
         for fn in ("HCORE_dgemm_qr_svd", "HCORE_dgemm_qr_svd_b_dense", "HCORE_dgemm_ormqr"):
             getattr(L, fn).restype = None
@@ -346,19 +348,27 @@ def compress_batch(m, n, A, lda, AU, ldu, AV, ldv, rank_ptr, maxrank=None, tol=1
                                           tol, stream)
 
 
-def tlr_dpotrf(nb, D, U, V, ranks, cap, maxrk, acc, stream=None, flops=None):
-    """HCORE_hip_tlr_dpotrf: D (nt device addresses), U, V (nt*nt addresses,
-    tile (i, j) at i + j*nt, 0 above the diagonal), ranks (nt x nt ints,
-    column-major, updated in place).  Returns (status, info, step_max)."""
+def tlr_dpotrf(nb, D, U, V, ranks, cap, maxrk, acc, stream=None, flops=None, precision=FP64,
+               slices=0):
+    """HCORE_hip_tlr_dpotrf (HCORE_hip_tlr_dpotrf_form for precision SPLIT):
+    D (nt device addresses), U, V (nt*nt addresses, tile (i, j) at i + j*nt, 0
+    above the diagonal), ranks (nt x nt ints, column-major, updated in
+    place).  Returns (status, info, step_max)."""
     nt = len(D)
     rk = np.asfortranarray(ranks, dtype=np.int32)
     flat = np.ascontiguousarray(rk.reshape(-1, order="F"))
     info = C.c_int(0)
     smax = np.zeros(nt, dtype=np.int32)
     fc = flops if flops is not None else FlopCounter()
-    st = lib().HCORE_hip_tlr_dpotrf(nt, nb, _parr(D), _parr(U), _parr(V),
-                                    flat.ctypes.data_as(_ip), cap, maxrk, acc, stream,
-                                    C.byref(info), C.byref(fc), smax.ctypes.data_as(_ip))
+    if precision == FP64:
+        st = lib().HCORE_hip_tlr_dpotrf(nt, nb, _parr(D), _parr(U), _parr(V),
+                                        flat.ctypes.data_as(_ip), cap, maxrk, acc, stream,
+                                        C.byref(info), C.byref(fc), smax.ctypes.data_as(_ip))
+    else:
+        st = lib().HCORE_hip_tlr_dpotrf_form(precision, slices, nt, nb, _parr(D), _parr(U),
+                                             _parr(V), flat.ctypes.data_as(_ip), cap, maxrk, acc,
+                                             stream, C.byref(info), C.byref(fc),
+                                             smax.ctypes.data_as(_ip))
     ranks[...] = flat.reshape(nt, nt, order="F")
     return st, info.value, smax
 

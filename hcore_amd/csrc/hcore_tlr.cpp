@@ -4,6 +4,9 @@
 // tiles below, HCORE_dgemm recompression of every lower tile of the
 // trailing matrix), run here as one batched plan per step on the GPU.
 //
+// HCORE_hip_tlr_dpotrf_form runs the recompressions as FP64 or as
+// split-precision plans (BF16-digit products); potrf, trsm and dsyrk stay FP64.
+//
 // Storage (all device memory): dense diagonal tiles D[k] (nb x nb, ld nb);
 // off-diagonal tiles (i > j) as U[i + j nt], V[i + j nt] (nb x cap, ld nb)
 // with host int ranks rank[i + j nt].  Panel tiles whose V factors lie cap
@@ -20,14 +23,17 @@
 #include "hcore_hip.h"
 #include "hcore_host.hpp"
 
-extern "C" int HCORE_hip_tlr_dpotrf(int nt, int nb, double* const* D, double* const* U,
-                                    double* const* V, int* rank, int cap, int maxrk, double acc,
-                                    void* stream, int* info, flop_counter* flops,
-                                    int* step_ranks_max) {
+extern "C" int HCORE_hip_tlr_dpotrf_form(int precision, int slices, int nt, int nb,
+                                         double* const* D, double* const* U, double* const* V,
+                                         int* rank, int cap, int maxrk, double acc, void* stream,
+                                         int* info, flop_counter* flops, int* step_ranks_max) {
   using namespace hcore;
   if (info) *info = 0;
   if (nt < 1 || nb < 1 || !D || !rank || cap < 1 || maxrk < 1 || maxrk > cap ||
       (nt > 1 && (!U || !V)))
+    return HCORE_HIP_ERR_ARG;
+  if (precision != HCORE_HIP_FP64 && precision != HCORE_HIP_SPLIT) return HCORE_HIP_ERR_ARG;
+  if (precision == HCORE_HIP_SPLIT && slices != 0 && (slices < kOzMinSlices || slices > kOzMaxSlices))
     return HCORE_HIP_ERR_ARG;
   for (int j = 0; j < nt; ++j) {
     if (!D[j]) return HCORE_HIP_ERR_ARG;
@@ -96,7 +102,9 @@ extern "C" int HCORE_hip_tlr_dpotrf(int nt, int nb, double* const* D, double* co
         tile.push_back(i + j * nt);
       }
     hcore_hip_plan* p = nullptr;
-    st = HCORE_hip_plan_create(&p, nb, HCORE_HIP_FP64);
+    st = HCORE_hip_plan_create(&p, nb, precision);
+    if (st == HCORE_HIP_OK && precision == HCORE_HIP_SPLIT && slices)
+      st = HCORE_hip_plan_set_slices(p, slices);
     const int n = (int)AU.size();
     if (st == HCORE_HIP_OK && n)
       st = HCORE_hip_plan_add_dgemm(p, n, AU.data(), AV.data(), ra.data(), nb, BU.data(),
@@ -149,4 +157,12 @@ extern "C" int HCORE_hip_tlr_dpotrf(int nt, int nb, double* const* D, double* co
   }
   set_last_status(st);
   return st;
+}
+
+extern "C" int HCORE_hip_tlr_dpotrf(int nt, int nb, double* const* D, double* const* U,
+                                    double* const* V, int* rank, int cap, int maxrk, double acc,
+                                    void* stream, int* info, flop_counter* flops,
+                                    int* step_ranks_max) {
+  return HCORE_hip_tlr_dpotrf_form(HCORE_HIP_FP64, 0, nt, nb, D, U, V, rank, cap, maxrk, acc,
+                                   stream, info, flops, step_ranks_max);
 }

@@ -10,7 +10,7 @@ HCORE_hip_tlr_dpotrf (potrf, trsm, dsyrk and batched recompression per step).
 """
 import numpy as np
 
-from . import DeviceBuffer, gytlr_batch, tlr_dpotrf, FlopCounter, _check
+from . import DeviceBuffer, gytlr_batch, tlr_dpotrf, FlopCounter, _check, FP64
 
 
 class TlrMatrix:
@@ -65,13 +65,14 @@ class TlrMatrix:
         rank.free()
         return st
 
-    def factor(self, maxrk=None, acc=None, stream=None):
-        """HCORE_hip_tlr_dpotrf in place.  Returns (status, info, largest
-        new rank per step)."""
+    def factor(self, maxrk=None, acc=None, stream=None, precision=FP64, slices=0):
+        """HCORE_hip_tlr_dpotrf in place (the recompressions as split-precision
+        plans for precision SPLIT).  Returns (status, info, largest new rank
+        per step)."""
         D, U, V = self.pointer_tables()
         return tlr_dpotrf(self.nb, D, U, V, self.ranks, self.cap,
                           self.cap if maxrk is None else maxrk,
-                          self.acc if acc is None else acc, stream, self.flops)
+                          self.acc if acc is None else acc, stream, self.flops, precision, slices)
 
     def tile(self, i, k):
         """Host copy of tile (i, k), i >= k: dense diagonal, else U V^T."""

@@ -264,6 +264,17 @@ int HCORE_hip_covariance_points(int bigM, unsigned long long seed, double *xy);
 int HCORE_hip_tlr_dpotrf(int nt, int nb, double *const *D, double *const *U, double *const *V,
                          int *rank, int cap, int maxrk, double acc, void *stream, int *info,
                          flop_counter *flops, int *step_ranks_max);
+/*
+ * The same stream with the recompressions in a chosen precision form:
+ * HCORE_HIP_FP64 (exactly HCORE_hip_tlr_dpotrf) or HCORE_HIP_SPLIT (the
+ * products as BF16 digit slices; slices 0: the count from acc, else 2..6);
+ * potrf, trsm and the dsyrk updates stay FP64.  HCORE_HIP_ERR_ARG for another
+ * form or slice count.  No counterpart in the reference.
+ */
+int HCORE_hip_tlr_dpotrf_form(int precision, int slices, int nt, int nb, double *const *D,
+                              double *const *U, double *const *V, int *rank, int cap, int maxrk,
+                              double acc, void *stream, int *info, flop_counter *flops,
+                              int *step_ranks_max);
 
 /* Device memory and stream helpers (the library's own HIP runtime; callers
  * that share device buffers with the library allocate them here). */

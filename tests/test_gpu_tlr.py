@@ -112,3 +112,38 @@ def test_tlr_cholesky_not_spd_reports_info(hc):
         M.free()
     finally:
         hc.set_covariance(0.1, 1e-4)
+
+
+def test_tlr_cholesky_split_precision_vs_fp64(hc, orc):
+    """The stream with split-precision recompressions (BF16-digit products,
+    HCORE_hip_tlr_dpotrf_form) on the same generated matrix as the FP64
+    stream: residual ||L L^T - A||_F / ||A||_F within 2x of the FP64 stream's
+    (plus 1e-12), final ranks within 3, the factor within 1e-5 of the dense
+    Cholesky; an unknown form or slice count is rejected."""
+    from hcore_amd.tlr import TlrMatrix
+    bigM, nb, seed, acc = 2048, 256, 2024, 1e-8
+    P = orc.covariance_points(bigM, seed)
+    A = orc.covariance_tile(bigM, seed, 0, 0, bigM, bigM, points=P)
+    nA = np.linalg.norm(A)
+    out = {}
+    for name, prec in (("fp64", hc.FP64), ("split", hc.SPLIT)):
+        M = TlrMatrix(bigM, nb, seed, maxrank=nb // 2, acc=acc, cap=nb)
+        M.generate()
+        _say("case: TLR factor", name)
+        st, info, _ = M.factor(maxrk=nb, acc=acc, precision=prec)
+        assert st == 0 and info == 0
+        L = M.dense_lower()
+        out[name] = (np.linalg.norm(L @ L.T - A) / nA, M.ranks.copy(), L)
+        M.free()
+    _say("residual fp64 %.3e split %.3e" % (out["fp64"][0], out["split"][0]))
+    assert out["split"][0] <= 2 * out["fp64"][0] + 1e-12
+    low = np.tril_indices(bigM // nb, -1)
+    assert np.max(np.abs(out["split"][1][low].astype(int) - out["fp64"][1][low])) <= 3
+    Ld = np.linalg.cholesky(A)
+    assert np.linalg.norm(out["split"][2] - Ld) / np.linalg.norm(Ld) <= 1e-5
+    M = TlrMatrix(512, 256, seed, maxrank=128, acc=acc, cap=256)
+    M.generate()
+    for prec, sl in ((hc.MIXED, 0), (hc.SPLIT, 7)):
+        st, _, _ = M.factor(maxrk=256, acc=acc, precision=prec, slices=sl)
+        assert st == hc.ERR_ARG
+    M.free()
