@@ -70,6 +70,79 @@ def graph_ms(plan, stream, steps, warmup):
     return e0.elapsed_ms(e1) / steps
 
 
+def device_ranks_leg(step, stream, steps, host_ms, host_ranks, slack=16):
+    """The same gemm batch as a plan with device-side ranks whose capacities
+    exceed the ranks in use by `slack` per factor (lanes and workspace by
+    capacity; each core SVD routed by the rank read at execution through the
+    lane's rank queue): graph replays, each after the rank words are reset
+    (the plan publishes the new ranks to them), timed one by one with HIP
+    events; ranks compared with the host-rank plan's."""
+    import hcore_amd as hc
+    nb = step.nb
+    ra, rb, rc = step.ranks()
+    rows = sorted(step.panel)
+    n = len(step.pairs)
+    B = hc.DeviceBuffer.from_numpy
+    panel = {i: (B(u), B(v)) for i, (u, v) in step.panel.items()}
+    K = ra + rc
+    CUd = [B(c[0], k) for c, k in zip(step.C, K)]
+    CVd = [B(c[1], k) for c, k in zip(step.C, K)]
+    OUd = [hc.DeviceBuffer(nb, k) for k in K]
+    OVd = [hc.DeviceBuffer(nb, k) for k in K]
+    pos = {i: k for k, i in enumerate(rows)}
+    prk = np.array([step.panel[i][0].shape[1] for i in rows], dtype=np.int32)
+    crk = np.asarray(rc, dtype=np.int32)
+    prank = B(prk.reshape(-1, 1))
+    crank = B(crk.reshape(-1, 1))
+    pa = lambda i: prank.ptr + 4 * pos[i]
+    plan = hc.Plan(nb, hc.FP64)
+    plan.add_dgemm_dev([panel[i][0].ptr for i, j in step.pairs], [panel[i][1].ptr for i, j in step.pairs],
+                       [pa(i) for i, j in step.pairs], ra + slack, nb,
+                       [panel[j][0].ptr for i, j in step.pairs], [panel[j][1].ptr for i, j in step.pairs],
+                       [pa(j) for i, j in step.pairs], rb + slack, nb,
+                       [t.ptr for t in CUd], [t.ptr for t in CVd],
+                       [crank.ptr + 4 * u for u in range(n)], rc + slack, nb, -1.0, 1.0, 0, 512, 1e-8,
+                       CUout=[t.ptr for t in OUd], CVout=[t.ptr for t in OVd], ldco=nb)
+    plan.finalize()
+    e0, e1 = hc.Event(), hc.Event()
+    tot = 0.0
+    for rep in range(steps + 2):
+        crank.upload(crk.reshape(-1, 1))
+        e0.record(stream)
+        plan.execute(stream, graph=True)
+        e1.record(stream)
+        stream.synchronize()
+        if rep >= 2:  # capture and one warm-up replay untimed
+            tot += e0.elapsed_ms(e1)
+    ms = tot / steps
+    assert np.all(plan.status() == 0)
+    drift = int(np.max(np.abs(plan.ranks().astype(int) - host_ranks.astype(int))))
+    lanes = np.bincount(plan.lanes(), minlength=3)
+    # per-phase durations with the lanes one after another
+    crank.upload(crk.reshape(-1, 1))
+    plan.set_timing(2)
+    plan.execute(stream)
+    stream.synchronize()
+    ph_ms, ph_n = plan.phase_times()
+    plan.set_timing(False)
+    phases = {}
+    for l in range(ph_ms.shape[0]):
+        row = {PHASES[i]: float(ph_ms[l, i] / ph_n[l, i]) for i in range(len(PHASES)) if ph_n[l, i] > 0}
+        if row:
+            phases["lane%d" % l] = row
+            log("  device ranks lane %d: %s" % (l, " ".join("%s %.3f" % kv for kv in row.items())))
+    plan.destroy()
+    for t in CUd + CVd + OUd + OVd + [prank, crank] + [x for pr in panel.values() for x in pr]:
+        t.free()
+    res = {"capacity_slack_per_factor": slack, "ms_per_step": ms, "host_rank_ms_per_step": host_ms,
+           "over_host_ranks": ms / host_ms, "rank_drift_vs_host_ranks": drift,
+           "updates_per_lane_by_capacity": [int(x) for x in lanes], "phase_ms_serialized": phases}
+    log("device ranks (capacities +%d): %.3f ms/step vs host ranks %.3f ms (x%.2f), lanes by "
+        "capacity %s, rank drift %d" % (slack, ms, host_ms, ms / host_ms, res["updates_per_lane_by_capacity"],
+                                        drift))
+    return res
+
+
 def reduced_precision_leg(step, stream, steps, warmup, acc, sample):
     """FP64, mixed-precision (FP32 tiles) and split-precision (FP64 tiles,
     BF16-digit products, digit counts from the slice autotuner) plans of the
@@ -615,6 +688,11 @@ def main():
                 "rank drift %d" % (acc, m["fp64"]["ms_per_step"], m["fp64"]["worst_rel_err"],
                                    m["mixed"]["ms_per_step"], m["mixed"]["worst_rel_err"],
                                    m["mixed_over_fp64"], m["rank_drift_mixed_vs_fp64"]["max"]))
+    # ---- the same gemm batch with device-side ranks (rank queue) ---------------------
+    dev_ranks = None
+    if mixed:
+        dev_ranks = device_ranks_leg(step, stream, args.steps,
+                                     mixed["acc_1e-08"]["fp64"]["ms_per_step"], ranks)
     single = None
     if not args.no_cpu:
         single = single_tile_leg(args.single_reps)
@@ -702,6 +780,7 @@ def main():
                    "graph_capture_and_first_replay_ms": capture_ms},
         "roofline": roof,
         "reduced_precision": mixed,
+        "device_ranks": dev_ranks,
         "single_tile": single,
         "cpu_baseline": cpu,
         "tlr_cholesky": tlr,

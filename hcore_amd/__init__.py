@@ -117,6 +117,7 @@ def lib():
         L.HCORE_hip_plan_get_sweeps.argtypes = [_vp, _ip]
         L.HCORE_hip_plan_get_core_profile.argtypes = [_vp, C.POINTER(C.c_longlong), _dp]
         L.HCORE_hip_plan_get_lanes.argtypes = [_vp, _ip]
+        L.HCORE_hip_plan_get_core_queue.argtypes = [_vp, i, _ip, i]
         L.HCORE_hip_plan_workspace_bytes.argtypes = [_vp]
         L.HCORE_hip_plan_workspace_bytes.restype = C.c_longlong
         L.HCORE_hip_plan_flops.argtypes = [_vp, C.POINTER(C.c_ulong), C.POINTER(C.c_ulong)]
@@ -720,6 +721,15 @@ class Plan:
         if n < 0:
             raise RuntimeError("autotune failed: %d" % -n)
         return [tuple(rep[6 * k:6 * k + 6]) for k in range(min(n, 32))]
+
+    def core_queue(self, lane):
+        """HCORE_hip_plan_get_core_queue: the lane's update order of the last
+        execution (device-rank plans; None for a host-rank lane)."""
+        out = np.zeros(max(self.n, 1), dtype=np.int32)
+        k = self.L.HCORE_hip_plan_get_core_queue(self.h, lane, out.ctypes.data_as(_ip), len(out))
+        if k < 0:
+            raise RuntimeError("HCORE_hip_plan_get_core_queue failed: %d" % -k)
+        return out[:k].copy() if k else None
 
     def set_slices(self, slices):
         """HCORE_hip_plan_set_slices (split-precision plans; 0: from acc)."""

@@ -304,6 +304,7 @@ struct hcore_hip_plan {
     int count, variant, kmax;
     bool dbl;          // FP64 core of a mixed-precision plan
     int base_variant;  // the variant chosen by K (the autotuner may swap in its alternative)
+    bool queue = false;  // served from the lane's device-side rank queue (count: an upper bound)
   };
   struct Lane {
     int n = 0;
@@ -319,12 +320,23 @@ struct hcore_hip_plan {
     int n_cv = 0, cv_max = 0;
     int n_wy = 0;
     int slices = 0;  // split precision: digits per operand of the sliced products
+    // device-side ranks: the lane's core SVDs routed by their stacked rank at
+    // execution (queue ints at o_q, every update's descriptor at o_qprobs)
+    bool queue = false, qdbl = false, qreg = false;
+    size_t o_q = 0, o_qprobs = 0;
+    int qkl = 0;
+    // device-side ranks: the stacks' QR routed by K at execution (queue at
+    // o_qrq, both stacks of every update at o_qrall)
+    bool qrq = false;
+    size_t o_qrq = 0, o_qrall = 0;
+    int n_qrall = 0;
     std::vector<CoreLaunch> core;  // one launch per K bucket
     hipStream_t stream = nullptr;  // lanes > 0 (lane 0 uses the caller's stream)
     hipEvent_t ev_done = nullptr;
   };
   std::vector<Lane> lanes;
   std::vector<int> lane_of;  // lane index of each update (HCORE_hip_plan_get_lanes)
+  std::vector<std::vector<int>> lane_updates;  // each lane's updates in its local order
   size_t o_gp = 0, o_sp = 0;
   hcore::TaskRef tW, tXs, tC;  // dsyrk GEMM phases
   hipEvent_t ev_start = nullptr;
@@ -434,18 +446,20 @@ static int plan_finalize_t(hcore_hip_plan* p) {
   static_assert(kWyMaxK >= kCoreMaxK, "compact-WY reassembly for every stacked rank");
   // lane index as executed (empty lanes are dropped, so renumber)
   p->lane_of.assign(n, 0);
+  p->lane_updates.clear();
   {
     int li = 0;
     for (int l = 0; l < 3; ++l) {
       if (lane_members[l].empty()) continue;
       for (int u : lane_members[l]) p->lane_of[u] = li;
+      p->lane_updates.push_back(lane_members[l]);
       ++li;
     }
   }
 
   // ---- workspace layout (elements of T) ----
   std::vector<size_t> o_Us(n), o_Vs(n), o_tA(n), o_tB(n), o_TA(n), o_TB(n), o_F(n), o_Tc(n),
-      o_Tj(n), o_Zu(n), o_Zv(n), o_sg(n), o_S(n), o_M(n), o_c64(n);
+      o_Tj(n), o_Zu(n), o_Zv(n), o_sg(n), o_S(n), o_M(n), o_c64(n), o_rt(n);
   size_t off = 0;
   for (int u = 0; u < n; ++u) {
     const int K = Kv[u];
@@ -463,6 +477,7 @@ static int plan_finalize_t(hcore_hip_plan* p) {
     o_S[u] = off; off += 2 * al((size_t)K * K);   // Gram Y^T Y of both stacks
     o_M[u] = off; off += 2 * al((size_t)K * K);   // compact-WY solves of both sides
     o_sg[u] = off; off += al(K);
+    if (p->dev_ranks) { o_rt[u] = off; off += al(1); }  // QR queue: routed flags of both stacks
     if (mixed) {  // FP64 core: T, scratch, left vectors and sigma, in T-sized elements
       const size_t d = (size_t)K * K + (size_t)K * (K + 64) + (size_t)K * K + K;
       o_c64[u] = off; off += al(d * sizeof(double) / es);
@@ -525,9 +540,13 @@ static int plan_finalize_t(hcore_hip_plan* p) {
       std::vector<WySolveProb<T>> wy;
       int kmax = 0;
       std::vector<QrProb<T>> qr, qrr16, qrr, qrp;  // blocked / register single pass / register panels
+      std::vector<QrProb<T>> qr_all;               // device-side ranks: every stack (the QR queue)
       bool gram = false;  // the T phase holds k = nb Gram products
       std::vector<std::vector<CoreProb<T>>> core_b;
       std::vector<std::vector<CoreProb<double>>> core_b64;  // mixed: the FP64 core
+      // device-side ranks: every update's core in lane order (the rank queue)
+      std::vector<CoreProb<T>> core_all;
+      std::vector<CoreProb<double>> core_all64;
       std::vector<ConvProb> cv_in, cv_out;
       int cv_max = 0;
       std::vector<int> core_kmax;
@@ -623,6 +642,8 @@ static int plan_finalize_t(hcore_hip_plan* p) {
             g.alpha = T(1); g.beta = T(0);
             // only its strict lower triangle is read (the compact-WY solve)
             g.flags = GF_A_TRANS | GF_A_UNIT_LOWER | GF_B_UNIT_LOWER | GF_BETA_ZERO | GF_LOWER_ONLY;
+            if (dvr && g_ctx.qr_reg && M <= kQrRegMaxNb)  // the QR queue: a register QR wrote S
+              g.skip = reinterpret_cast<const int*>(W + o_rt[u]) + side;
             if (!qreg) {
               add_tasks(d.tT, (int)gp.size(), g);
               gp.push_back(g);
@@ -658,7 +679,16 @@ static int plan_finalize_t(hcore_hip_plan* p) {
             rec(q, q.size() - 1, offsetof(QrProb<T>, K), PK_K, u);
           }
         };
-        if (qreg) {  // one register-resident pass (also yields S)
+        if (dvr && g_ctx.qr_reg && M <= kQrRegMaxNb) {
+          // device-side ranks: the kernel chosen by the K read at execution
+          // (the register QR also writes S; the Gram products of capacities
+          // above kQrRegMaxK rewrite it for every stack)
+          T* SA = W + o_S[u];
+          int* rt = reinterpret_cast<int*>(W + o_rt[u]);
+          d.qr_all.push_back({Us, W + o_tA[u], W + o_TA[u], SA, K, rt});
+          d.qr_all.push_back({Vs, W + o_tB[u], W + o_TB[u], SA + al((size_t)K * K), K, rt + 1});
+          recQ(d.qr_all);
+        } else if (qreg) {  // one register-resident pass (also yields S)
           T* SA = W + o_S[u];
           auto& q = K <= 16 ? d.qrr16 : d.qrr;
           q.push_back({Us, W + o_tA[u], W + o_TA[u], SA, K});
@@ -682,21 +712,27 @@ static int plan_finalize_t(hcore_hip_plan* p) {
           double* sg64 = Zu64 + (size_t)K * K;
           CoreProb<double> c{Tc64, Tj64, Zu64, sg64, d_rank + u, d_status + u, d_sweeps + u,
                              d_clk ? d_clk + (size_t)u * kCoreClk : nullptr, K};
-          d.core_b64[b].push_back(c);
           d.cv_in.push_back({W + o_Tc[u], Tc64, K * K});
           d.cv_out.push_back({Zu64, W + o_Zu[u], K * K});
           d.cv_max = std::max(d.cv_max, K * K);
           if (dvr) {
-            rec(d.core_b64[b], d.core_b64[b].size() - 1, offsetof(CoreProb<double>, K), PK_K, u);
+            d.core_all64.push_back(c);
+            rec(d.core_all64, d.core_all64.size() - 1, offsetof(CoreProb<double>, K), PK_K, u);
             rec(d.cv_in, d.cv_in.size() - 1, offsetof(ConvProb, n), PK_KK, u);
             rec(d.cv_out, d.cv_out.size() - 1, offsetof(ConvProb, n), PK_KK, u);
+          } else {
+            d.core_b64[b].push_back(c);
           }
         } else {
           CoreProb<T> c{W + o_Tc[u], global_core(variant[u]) ? W + o_Tj[u] : nullptr, W + o_Zu[u],
                         W + o_sg[u], d_rank + u, d_status + u, d_sweeps + u,
                         d_clk ? d_clk + (size_t)u * kCoreClk : nullptr, K};
-          d.core_b[b].push_back(c);
-          if (dvr) rec(d.core_b[b], d.core_b[b].size() - 1, offsetof(CoreProb<T>, K), PK_K, u);
+          if (dvr) {
+            d.core_all.push_back(c);
+            rec(d.core_all, d.core_all.size() - 1, offsetof(CoreProb<T>, K), PK_K, u);
+          } else {
+            d.core_b[b].push_back(c);
+          }
         }
         d.core_kmax[b] = std::max(d.core_kmax[b], K);
         d.kmax = std::max(d.kmax, K);
@@ -748,15 +784,40 @@ static int plan_finalize_t(hcore_hip_plan* p) {
       L.o_qrr16 = addv(d.qrr16); L.n_qrr16 = (int)d.qrr16.size();
       L.o_qrr = addv(d.qrr); L.n_qrr = (int)d.qrr.size();
       L.o_qrp = addv(d.qrp); L.n_qrp = (int)d.qrp.size();
+      L.qrq = !d.qr_all.empty();
+      L.n_qrall = (int)d.qr_all.size();
+      L.o_qrall = addv(d.qr_all);
+      L.o_qrq = pk.add(std::vector<int>(kQrQueueHead + std::max(L.n_qrall, 1), 0));
       L.tB = pk.add(d.tB);
       L.tX = pk.add(d.tX);
       L.o_wy = addv(d.wy); L.n_wy = (int)d.wy.size();
       L.kmax = d.kmax;
       L.slices = sl ? (p->slices_set ? p->slices_set : default_slices(p->acc)) : 0;
       L.core.clear();
+      L.queue = dvr;
+      if (dvr) {
+        // one launch per base variant the lane's ranks can select (K up to the
+        // lane's capacity), each served from the rank queue built at execution
+        const bool dbl = mixed;
+        L.qdbl = dbl;
+        L.qreg = reg;
+        L.qkl = KL;
+        L.o_qprobs = dbl ? addv(d.core_all64) : addv(d.core_all);
+        L.o_q = pk.add(std::vector<int>(kCoreQueueHead + std::max(L.n, 1), 0));
+        std::vector<int> vmax(NV, 0);
+        for (int K = 1; K <= d.kmax; ++K) vmax[core_variant_for(K, KL, reg)] = K;
+        for (int b = 0; b < NV; ++b)
+          if (vmax[b] > 0) {
+            hcore_hip_plan::CoreLaunch c{L.o_qprobs, L.n, tuned_variant(M, dbl || sizeof(T) == 8, b),
+                                         vmax[b], dbl, b};
+            c.queue = true;
+            L.core.push_back(c);
+          }
+      }
       for (int b = 0; b < NV; ++b) {
         if (!d.core_b[b].empty())
-          L.core.push_back({addv(d.core_b[b]), (int)d.core_b[b].size(), tuned_variant(M, false, b),
+          L.core.push_back({addv(d.core_b[b]), (int)d.core_b[b].size(),
+                            tuned_variant(M, sizeof(T) == 8, b),
                             d.core_kmax[b], false, b});
         if (!d.core_b64[b].empty())
           L.core.push_back({addv(d.core_b64[b]), (int)d.core_b64[b].size(),
@@ -978,6 +1039,14 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
     phase(l, 2, ls, true, [&] { gemm(gp, L.tG, ls, false, L.slices); });
     bool shapes_ok = true;
     phase(l, 3, ls, true, [&] {
+      if (L.qrq) {
+        const auto* qa = reinterpret_cast<const QrProb<T>*>(D(L.o_qrall));
+        int* qq = reinterpret_cast<int*>(p->desc + L.o_qrq);
+        launch_qr_queue<T>(qa, L.n_qrall, qq, ls);
+        shapes_ok &= launch_qr_fused<T>(qa, L.n_qrall, M, ls, qq, 2);
+        shapes_ok &= launch_qr_reg<T>(qa, L.n_qrall, M, 32, ls, qq, 1);
+        shapes_ok &= launch_qr_reg<T>(qa, L.n_qrall, M, 16, ls, qq, 0);
+      }
       shapes_ok &= launch_qr_fused<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qrp)), L.n_qrp, M, ls);
       shapes_ok &= launch_qr_reg<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qrr)), L.n_qrr, M, 32, ls);
       shapes_ok &= launch_qr_reg<T>(reinterpret_cast<const QrProb<T>*>(D(L.o_qrr16)), L.n_qrr16, M, 16, ls);
@@ -992,13 +1061,23 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
     // own LDS size); at most 4 streams per plan, one per hardware queue
     phase(l, 5, ls, !L.core.empty(), [&] {
       launch_convert(reinterpret_cast<const ConvProb*>(D(L.o_cvin)), L.n_cv, L.cv_max, true, ls);
+      int* q = L.queue ? reinterpret_cast<int*>(p->desc + L.o_q) : nullptr;
+      if (q) {
+        if (L.qdbl)
+          launch_core_queue<double>(reinterpret_cast<const CoreProb<double>*>(D(L.o_qprobs)), L.n,
+                                    L.qkl, L.qreg, q, ls);
+        else
+          launch_core_queue<T>(reinterpret_cast<const CoreProb<T>*>(D(L.o_qprobs)), L.n, L.qkl,
+                               L.qreg, q, ls);
+      }
       for (const auto& c : L.core) {
+        const int* cq = c.queue ? q : nullptr;
         if (c.dbl)
           launch_core<double>(reinterpret_cast<const CoreProb<double>*>(D(c.off)), c.count,
-                              c.variant, c.kmax, p->rk, p->maxrk, p->acc, ls);
+                              c.variant, c.kmax, p->rk, p->maxrk, p->acc, ls, cq, c.base_variant);
         else
           launch_core<T>(reinterpret_cast<const CoreProb<T>*>(D(c.off)), c.count, c.variant,
-                         c.kmax, p->rk, p->maxrk, T(p->acc), ls);
+                         c.kmax, p->rk, p->maxrk, T(p->acc), ls, cq, c.base_variant);
       }
       launch_convert(reinterpret_cast<const ConvProb*>(D(L.o_cvout)), L.n_cv, L.cv_max, false, ls);
     });
@@ -2512,12 +2591,14 @@ static int plan_autotune_t(hcore_hip_plan* p, hipStream_t s, int reps, double* r
       for (int r = 0; r < reps; ++r)
         for (int k = 0; k < 2; ++k) {
           HCHECK(hipEventRecord(e0, s));
+          const int* cq =
+              c.queue ? reinterpret_cast<const int*>(p->desc + p->lanes[l].o_q) : nullptr;
           if (c.dbl)
             launch_core<double>(reinterpret_cast<const CoreProb<double>*>(p->desc + c.off), c.count,
-                                cand[k], c.kmax, p->rk, p->maxrk, p->acc, s);
+                                cand[k], c.kmax, p->rk, p->maxrk, p->acc, s, cq, c.base_variant);
           else
             launch_core<T>(reinterpret_cast<const CoreProb<T>*>(p->desc + c.off), c.count, cand[k],
-                           c.kmax, p->rk, p->maxrk, T(p->acc), s);
+                           c.kmax, p->rk, p->maxrk, T(p->acc), s, cq, c.base_variant);
           HCHECK(hipEventRecord(e1, s));
           HCHECK(hipEventSynchronize(e1));
           float ms = 0.f;
@@ -2571,6 +2652,19 @@ int HCORE_hip_plan_get_status(hcore_hip_plan* p, int* out) {
   HCHECK(hipDeviceSynchronize());
   if (p->ng) HCHECK(hipMemcpy(out, p->d_status, sizeof(int) * p->ng, hipMemcpyDeviceToHost));
   return HCORE_HIP_OK;
+}
+
+int HCORE_hip_plan_get_core_queue(hcore_hip_plan* p, int lane, int* out, int n) {
+  if (!p || !p->finalized || lane < 0 || lane >= (int)p->lanes.size()) return -HCORE_HIP_ERR_ARG;
+  const auto& L = p->lanes[lane];
+  if (!L.queue) return 0;
+  std::vector<int> q(kCoreQueueHead + L.n);
+  HCHECK(hipDeviceSynchronize());
+  HCHECK(hipMemcpy(q.data(), p->desc + L.o_q, sizeof(int) * q.size(), hipMemcpyDeviceToHost));
+  const auto& mem = p->lane_updates[lane];
+  if (out)
+    for (int i = 0; i < std::min(n, L.n); ++i) out[i] = mem[q[kCoreQueueHead + i]];
+  return L.n;
 }
 
 int HCORE_hip_plan_get_lanes(hcore_hip_plan* p, int* out) {

@@ -77,6 +77,7 @@ struct QrProb {
   T* Tp;   // ceil(K/PB) blocks of PB x PB compact-WY factors
   T* S;    // K x K Gram Y^T Y, lower triangle (single-panel register QR only)
   int K;
+  int* routed;  // QR queue: set to 1 when a register QR (which writes S) takes the stack
 };
 
 // ---- core SVD: one-sided Jacobi on T = R_A R_B^T -----------------------------
@@ -163,13 +164,24 @@ void launch_geqrf(const QrProb<T>* probs, int n, int nb, hipStream_t s);
 constexpr int kQrRegMaxK = 32;
 constexpr int kQrRegMaxNb = 1024;  // rows held by the largest register QR shape (512 x 2)
 constexpr int kQrFusedMaxNb = 2048;  // rows of the largest fused-QR shape (512 x 4)
+// q (optional): a device-side QR queue (launch_qr_queue), the launch serving
+// its class qc (n an upper bound on the class's size).
 template <typename T>
-bool launch_qr_reg(const QrProb<T>* probs, int n, int nb, int kmax, hipStream_t s);
+bool launch_qr_reg(const QrProb<T>* probs, int n, int nb, int kmax, hipStream_t s,
+                   const int* q = nullptr, int qc = -1);
 // whole-stack QR in one workgroup (nb <= kQrFusedMaxNb): register panels of 16
 // columns (32 at nb <= 256), each applied to the trailing columns in place;
 // false for nb beyond the limit
 template <typename T>
-bool launch_qr_fused(const QrProb<T>* probs, int n, int nb, hipStream_t s);
+bool launch_qr_fused(const QrProb<T>* probs, int n, int nb, hipStream_t s, const int* q = nullptr,
+                     int qc = -1);
+// ---- device-side QR queue (plans with device ranks) --------------------------
+// q = [start (3) | count (3) | items (n)]: the stacks by the stacked rank K in
+// their (patched) descriptors, class 0: K <= 16, 1: 17..kQrRegMaxK (the
+// register QR shapes), 2: larger (the fused QR).
+constexpr int kQrQueueHead = 6;
+template <typename T>
+void launch_qr_queue(const QrProb<T>* probs, int n, int* q, hipStream_t s);
 // core SVD variants by stacked rank K: one wavefront for K <= 64, one
 // workgroup with the core in LDS up to the LDS limit, 1024 threads with the
 // core in global memory beyond it
@@ -182,6 +194,22 @@ enum : int {
   CORE_R64W, CORE_R128W, CORE_R192W, CORE_NVARIANTS
 };
 int core_variant_for(int K, int lds_max_k, bool reg);
+// The same choice on the device (the rank queue below).
+__host__ __device__ inline int core_variant_of(int K, int kl, bool reg) {
+  if (K <= 16) return CORE_K16;
+  if (K <= 32) return CORE_K32;
+  if (reg) {
+    if (K <= 64) return CORE_R64;
+    if (K <= 128) return CORE_R128;
+    if (K <= 192) return CORE_R192;
+  }
+  if (K <= 64) return CORE_K64;
+  if (K <= 128) return CORE_K128;
+  if (K <= kl && K <= 144) return CORE_KLDS;
+  if (K <= 192) return CORE_K192;
+  if (K <= 256) return CORE_K256;
+  return CORE_K512;
+}
 // Variants whose work matrix (or R^T transposition) lives in the K x (K + 64)
 // global scratch CoreProb::Tj.
 inline bool core_needs_scratch(int v) {
@@ -193,12 +221,23 @@ int core_alt_variant(int variant);
 // the register-resident variants (CORE_R64 .. CORE_R192); launch_core forwards to it
 template <typename T>
 void launch_core_reg(const CoreProb<T>* probs, int n, int variant, int rk, int maxrk, T acc,
-                     hipStream_t s);
+                     hipStream_t s, const int* q = nullptr, int qv = -1);
 constexpr int kCoreMaxK = 512;
 constexpr int kMaxSweeps = 40;  // Jacobi sweep limit of every core variant
+// q (optional): a lane's device-side rank queue (launch_core_queue); the launch
+// then serves the updates of base variant qv in it, largest K first, n
+// being an upper bound on their number (workgroups beyond it exit at once).
 template <typename T>
 void launch_core(const CoreProb<T>* probs, int n, int variant, int kmax, int rk, int maxrk,
-                 T acc, hipStream_t s);
+                 T acc, hipStream_t s, const int* q = nullptr, int qv = -1);
+// ---- device-side rank queue of a lane's core SVDs (plans with device ranks) --
+// Built each execution from the stacked ranks K in the (patched) descriptors:
+// q = [vstart (CORE_NVARIANTS) | vcount (CORE_NVARIANTS) | items (n)], items
+// the lane's updates ordered by K, largest first, and [vstart[v], + vcount[v])
+// the slice whose K selects base variant v (core_variant_of).
+constexpr int kCoreQueueHead = 2 * CORE_NVARIANTS;
+template <typename T>
+void launch_core_queue(const CoreProb<T>* probs, int n, int kl, bool reg, int* q, hipStream_t s);
 // blocked back substitution of the compact-WY solve (64 columns per workgroup
 // up to K = 256, 16 beyond)
 constexpr int kWyMaxK = 512;

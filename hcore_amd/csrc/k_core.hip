@@ -340,7 +340,8 @@ __host__ __device__ constexpr size_t core_head_bytes(size_t elem, int kmax) {
 // IN_LDS: the work matrix lives in LDS; otherwise in the global scratch P.Tj.
 template <typename T, int NT, int G, int RPL, bool IN_LDS>
 __global__ __launch_bounds__(NT) void k_core(const CoreProb<T>* __restrict__ probs, int kmax,
-                                             int rk, int maxrk, T acc) {
+                                             int rk, int maxrk, T acc, const int* __restrict__ q,
+                                             int qv) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* sig = reinterpret_cast<T*>(smem_raw);
   T* nrm = sig + kmax;
@@ -352,7 +353,12 @@ __global__ __launch_bounds__(NT) void k_core(const CoreProb<T>* __restrict__ pro
   int* gi = piv + kmax;
   int* flags = gi + 256;  // [0] rotated, [1] r, [2] nonfinite
   static_assert(NT / G <= 256 && NT / 64 <= 16, "LDS scratch sized for 256 groups, 16 waves");
-  const CoreProb<T> P = probs[blockIdx.x];
+  int item = blockIdx.x;
+  if (q) {  // the rank queue's slice of this variant (uniform exit before any barrier)
+    if (item >= q[CORE_NVARIANTS + qv]) return;
+    item = q[kCoreQueueHead + q[qv] + item];
+  }
+  const CoreProb<T> P = probs[item];
   const int K = P.K;
   const int tid = threadIdx.x;
   T* A = IN_LDS ? reinterpret_cast<T*>(smem_raw + core_head_bytes(sizeof(T), kmax)) : P.Tj;
@@ -415,7 +421,7 @@ int core_lds_max_k(size_t elem) {
 
 template <typename T>
 void launch_core(const CoreProb<T>* probs, int n, int variant, int kmax, int rk, int maxrk, T acc,
-                 hipStream_t s) {
+                 hipStream_t s, const int* q, int qv) {
   static bool attr = false;
   if (!attr) {
 #define HCORE_SET_ATTR(V, NT, G, RPL, LDS)                                                   \
@@ -426,7 +432,7 @@ void launch_core(const CoreProb<T>* probs, int n, int variant, int kmax, int rk,
     attr = true;
   }
   if (variant >= CORE_R64) {
-    launch_core_reg<T>(probs, n, variant, rk, maxrk, acc, s);
+    launch_core_reg<T>(probs, n, variant, rk, maxrk, acc, s, q, qv);
     return;
   }
   if (n <= 0) return;
@@ -436,7 +442,7 @@ void launch_core(const CoreProb<T>* probs, int n, int variant, int kmax, int rk,
   case V: {                                                                                  \
     const size_t sm = head + (LDS ? sizeof(T) * (size_t)kmax * core_ld(kmax, G) : 0);        \
     hipLaunchKernelGGL((k_core<T, NT, G, RPL, LDS>), dim3(n), dim3(NT), sm, s, probs, kmax,  \
-                       rk, maxrk, acc);                                                      \
+                       rk, maxrk, acc, q, qv);                                               \
     break;                                                                                   \
   }
     HCORE_CORE_VARIANTS(HCORE_LAUNCH)
@@ -457,25 +463,60 @@ int core_alt_variant(int v) {
   }
 }
 
-int core_variant_for(int K, int kl, bool reg) {
-  if (K <= 16) return CORE_K16;
-  if (K <= 32) return CORE_K32;
-  if (reg) {
-    if (K <= 64) return CORE_R64;
-    if (K <= 128) return CORE_R128;
-    if (K <= 192) return CORE_R192;
+int core_variant_for(int K, int kl, bool reg) { return core_variant_of(K, kl, reg); }
+
+// One workgroup per lane: counting sort of the lane's updates by stacked rank
+// K (largest first; equal K in arrival order of the LDS atomics, which does
+// not change any result), then each base variant's slice of the order.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_core_queue(const CoreProb<T>* __restrict__ probs, int n,
+                                                     int kl, int reg, int* __restrict__ q) {
+  __shared__ int hist[kCoreMaxK + 1];
+  __shared__ int start[kCoreMaxK + 1];
+  const int tid = threadIdx.x;
+  auto kof = [&](int i) { return min(max(probs[i].K, 1), kCoreMaxK); };
+  for (int k = tid; k <= kCoreMaxK; k += 1024) hist[k] = 0;
+  __syncthreads();
+  for (int i = tid; i < n; i += 1024) atomicAdd(&hist[kof(i)], 1);
+  __syncthreads();
+  if (tid == 0) {  // start[k]: the number of updates with a larger K
+    int acc = 0;
+    for (int k = kCoreMaxK; k >= 1; --k) {
+      start[k] = acc;
+      acc += hist[k];
+    }
   }
-  if (K <= 64) return CORE_K64;
-  if (K <= 128) return CORE_K128;
-  if (K <= kl && K <= 144) return CORE_KLDS;
-  if (K <= 192) return CORE_K192;
-  if (K <= 256) return CORE_K256;
-  return CORE_K512;
+  __syncthreads();
+  if (tid < CORE_NVARIANTS) {
+    int cnt = 0, st = -1;
+    for (int k = kCoreMaxK; k >= 1; --k)
+      if (core_variant_of(k, kl, reg != 0) == tid) {
+        if (st < 0) st = start[k];
+        cnt += hist[k];
+      }
+    q[tid] = st < 0 ? 0 : st;
+    q[CORE_NVARIANTS + tid] = cnt;
+  }
+  __syncthreads();
+  for (int k = tid; k <= kCoreMaxK; k += 1024) hist[k] = 0;  // now fill counters
+  __syncthreads();
+  for (int i = tid; i < n; i += 1024) {
+    const int K = kof(i);
+    q[kCoreQueueHead + start[K] + atomicAdd(&hist[K], 1)] = i;
+  }
+}
+
+template <typename T>
+void launch_core_queue(const CoreProb<T>* probs, int n, int kl, bool reg, int* q, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_core_queue<T>, dim3(1), dim3(1024), 0, s, probs, n, kl, reg ? 1 : 0, q);
 }
 
 // ---- explicit instantiations ---------------------------------------------------
-#define HCORE_INSTANTIATE(T) \
-  template void launch_core<T>(const CoreProb<T>*, int, int, int, int, int, T, hipStream_t);
+#define HCORE_INSTANTIATE(T)                                                                  \
+  template void launch_core<T>(const CoreProb<T>*, int, int, int, int, int, T, hipStream_t,     \
+                               const int*, int);                                              \
+  template void launch_core_queue<T>(const CoreProb<T>*, int, int, bool, int*, hipStream_t);
 HCORE_INSTANTIATE(double)
 HCORE_INSTANTIATE(float)
 #undef HCORE_INSTANTIATE

@@ -261,7 +261,8 @@ struct CoreReg {
 
 template <typename T, int W, int RPL, int CPW, int B>
 __global__ __launch_bounds__(64 * W) void k_core_reg(const CoreProb<T>* __restrict__ probs, int rk,
-                                                      int maxrk, T acc) {
+                                                      int maxrk, T acc, const int* __restrict__ q,
+                                                      int qv) {
   using C = CoreReg<T, W, RPL, CPW, B>;
   constexpr int NT = C::NT, KM = C::KM;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -281,7 +282,12 @@ __global__ __launch_bounds__(64 * W) void k_core_reg(const CoreProb<T>* __restri
   int* rnk = perm + KM;                            // KM: sorted position of each column
   int* flags = rnk + KM;  // [0] r, [1] nonfinite, [2..4] rotated, [5..7] max rho (sweep mod 3)
 
-  const CoreProb<T> P = probs[blockIdx.x];
+  int item = blockIdx.x;
+  if (q) {  // the rank queue's slice of this variant (uniform exit before any barrier)
+    if (item >= q[CORE_NVARIANTS + qv]) return;
+    item = q[kCoreQueueHead + q[qv] + item];
+  }
+  const CoreProb<T> P = probs[item];
   const int K = P.K;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   T* xs = xs_all + w * C::XS;
@@ -676,7 +682,7 @@ __global__ __launch_bounds__(64 * W) void k_core_reg(const CoreProb<T>* __restri
 
 template <typename T>
 void launch_core_reg(const CoreProb<T>* probs, int n, int variant, int rk, int maxrk, T acc,
-                            hipStream_t s) {
+                            hipStream_t s, const int* q, int qv) {
   static bool attr = false;
   if (!attr) {
 #define HCORE_SET_ATTR_R(V, W, RPL, CPW, B)                                                   \
@@ -692,7 +698,7 @@ void launch_core_reg(const CoreProb<T>* probs, int n, int variant, int rk, int m
 #define HCORE_LAUNCH_R(V, W, RPL, CPW, B)                                                      \
   case V:                                                                                      \
     hipLaunchKernelGGL((k_core_reg<T, W, RPL, CPW, B>), dim3(n), dim3(64 * W),                \
-                       (CoreReg<T, W, RPL, CPW, B>::smem), s, probs, rk, maxrk, acc);          \
+                       (CoreReg<T, W, RPL, CPW, B>::smem), s, probs, rk, maxrk, acc, q, qv);  \
     break;
     HCORE_CORE_REG_VARIANTS(HCORE_LAUNCH_R)
 #undef HCORE_LAUNCH_R
@@ -701,7 +707,9 @@ void launch_core_reg(const CoreProb<T>* probs, int n, int variant, int rk, int m
 }
 
 // ---- explicit instantiations ---------------------------------------------------
-template void launch_core_reg<double>(const CoreProb<double>*, int, int, int, int, double, hipStream_t);
-template void launch_core_reg<float>(const CoreProb<float>*, int, int, int, int, float, hipStream_t);
+template void launch_core_reg<double>(const CoreProb<double>*, int, int, int, int, double, hipStream_t,
+                                      const int*, int);
+template void launch_core_reg<float>(const CoreProb<float>*, int, int, int, int, float, hipStream_t,
+                                     const int*, int);
 
 }  // namespace hcore

@@ -323,10 +323,52 @@ void launch_geqrf(const QrProb<T>* probs, int n, int nb, hipStream_t s) {
   else launch_geqrf_nbw<T, 32>(probs, n, nb, s);
 }
 
+// One workgroup: the stacks of a lane by their stacked rank K into the three
+// classes of QR kernel (K <= 16, <= kQrRegMaxK, larger); the order within a
+// class follows the LDS atomics and changes no result.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_qr_queue(const QrProb<T>* __restrict__ probs, int n,
+                                                   int* __restrict__ q) {
+  __shared__ int cnt[3], fill[3];
+  const int tid = threadIdx.x;
+  if (tid < 3) {
+    cnt[tid] = 0;
+    fill[tid] = 0;
+  }
+  __syncthreads();
+  auto cls = [&](int i) {
+    const int K = probs[i].K;
+    return K <= 16 ? 0 : (K <= kQrRegMaxK ? 1 : 2);
+  };
+  for (int i = tid; i < n; i += 1024) atomicAdd(&cnt[cls(i)], 1);
+  __syncthreads();
+  if (tid == 0) {
+    q[0] = 0;
+    q[1] = cnt[0];
+    q[2] = cnt[0] + cnt[1];
+    q[3] = cnt[0];
+    q[4] = cnt[1];
+    q[5] = cnt[2];
+  }
+  for (int i = tid; i < n; i += 1024) {
+    const int c = cls(i);
+    const int st = c == 0 ? 0 : (c == 1 ? cnt[0] : cnt[0] + cnt[1]);
+    q[kQrQueueHead + st + atomicAdd(&fill[c], 1)] = i;
+    if (probs[i].routed) *probs[i].routed = c < 2 ? 1 : 0;
+  }
+}
+
+template <typename T>
+void launch_qr_queue(const QrProb<T>* probs, int n, int* q, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_qr_queue<T>, dim3(1), dim3(1024), 0, s, probs, n, q);
+}
+
 // ---- explicit instantiations ---------------------------------------------------
 #define HCORE_INSTANTIATE(T) \
   template void launch_stack<T>(const StackProb<T>*, int, int, int, T, T, hipStream_t); \
-  template void launch_geqrf<T>(const QrProb<T>*, int, int, hipStream_t);
+  template void launch_geqrf<T>(const QrProb<T>*, int, int, hipStream_t); \
+  template void launch_qr_queue<T>(const QrProb<T>*, int, int*, hipStream_t);
 HCORE_INSTANTIATE(double)
 HCORE_INSTANTIATE(float)
 #undef HCORE_INSTANTIATE

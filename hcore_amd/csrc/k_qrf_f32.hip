@@ -2,5 +2,5 @@
 #include "qr_reg.hpp"
 
 namespace hcore {
-template bool launch_qr_fused<float>(const QrProb<float>*, int, int, hipStream_t);
+template bool launch_qr_fused<float>(const QrProb<float>*, int, int, hipStream_t, const int*, int);
 }  // namespace hcore

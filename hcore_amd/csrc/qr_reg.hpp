@@ -138,7 +138,8 @@ __device__ __forceinline__ void qr_panel_factor(const QrProb<T>& P, int K, int j
 }
 
 template <typename T, int NT, int RPT, int KMAX>
-__global__ __launch_bounds__(NT) void k_qr_reg(const QrProb<T>* __restrict__ probs, int nb, int j0) {
+__global__ __launch_bounds__(NT) void k_qr_reg(const QrProb<T>* __restrict__ probs, int nb, int j0,
+                                               const int* __restrict__ q, int qc) {
   constexpr int NW = NT / 64;
   static_assert(KMAX % 16 == 0 && KMAX <= 64 && NT >= KMAX && KMAX % kPB == 0, "register QR shape");
   __shared__ T part[2][NW][KMAX];
@@ -147,7 +148,12 @@ __global__ __launch_bounds__(NT) void k_qr_reg(const QrProb<T>* __restrict__ pro
   __shared__ T Sl[KMAX][KMAX + 1];  // Sl[j][t] = y_t^T y_j, t < j (panel columns)
   __shared__ T taus[KMAX];
   __shared__ T Tf[KMAX][KMAX + 1];  // the panel's compact-WY factor
-  const QrProb<T> P = probs[blockIdx.x];
+  int item = blockIdx.x;
+  if (q) {  // the queue's slice of this class (uniform exit before any barrier)
+    if (item >= q[3 + qc]) return;
+    item = q[kQrQueueHead + q[qc] + item];
+  }
+  const QrProb<T> P = probs[item];
   const int K = P.K;
   if (j0 >= K) return;
   const int pw = min(KMAX, K - j0);  // panel width
@@ -190,7 +196,8 @@ __global__ __launch_bounds__(NT) void k_qr_reg(const QrProb<T>* __restrict__ pro
 // rows of the stack, so the panels and the trailing columns need no
 // inter-thread ordering through global memory.
 template <typename T, int NT, int RPT, int PB>
-__global__ __launch_bounds__(NT) void k_qr_fused(const QrProb<T>* __restrict__ probs, int nb) {
+__global__ __launch_bounds__(NT) void k_qr_fused(const QrProb<T>* __restrict__ probs, int nb,
+                                                 const int* __restrict__ q, int qc) {
   constexpr int NW = NT / 64;
   constexpr int CB = 4;
   constexpr int CH = 16;
@@ -204,7 +211,12 @@ __global__ __launch_bounds__(NT) void k_qr_fused(const QrProb<T>* __restrict__ p
   __shared__ T tpart[NW][CB][PB];
   __shared__ T Wm[PB][CB];
   __shared__ T W2[PB][CB];
-  const QrProb<T> P = probs[blockIdx.x];
+  int item = blockIdx.x;
+  if (q) {  // the queue's slice of this class (uniform exit before any barrier)
+    if (item >= q[3 + qc]) return;
+    item = q[kQrQueueHead + q[qc] + item];
+  }
+  const QrProb<T> P = probs[item];
   const int K = P.K;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   for (int j0 = 0; j0 < K; j0 += PB) {
@@ -313,40 +325,46 @@ constexpr int kQrfThreads = 512;   // panels: 2 rows per thread at nb = 1024
 constexpr int kQrfPanel = 16;      // panel width of the fused QR (VGPR budget at 2 rows)
 
 template <typename T, int NT, int RPT, int KMAX>
-inline void launch_qr_reg_shape(const QrProb<T>* probs, int n, int nb, hipStream_t s) {
-  hipLaunchKernelGGL((k_qr_reg<T, NT, RPT, KMAX>), dim3(n), dim3(NT), 0, s, probs, nb, 0);
+inline void launch_qr_reg_shape(const QrProb<T>* probs, int n, int nb, hipStream_t s,
+                                const int* q, int qc) {
+  hipLaunchKernelGGL((k_qr_reg<T, NT, RPT, KMAX>), dim3(n), dim3(NT), 0, s, probs, nb, 0, q, qc);
 }
 
 // Only the shapes below are instantiated (nb <= 1024): K <= 16 at 256
 // threads x 1, 2 or 4 rows; K <= 32 at 256 x 1, 512 x 1 or 512 x 2 rows.
 template <typename T>
-bool launch_qr_reg(const QrProb<T>* probs, int n, int nb, int kmax, hipStream_t s) {
+bool launch_qr_reg(const QrProb<T>* probs, int n, int nb, int kmax, hipStream_t s, const int* q,
+                   int qc) {
   if (n <= 0) return true;
   if (nb > kQrRegMaxNb || kmax > kQrRegMaxK) return false;
   static_assert(kQr16Threads == 256 && kQr32Threads == 512, "register QR shapes");
   if (kmax <= 16) {
-    if (nb <= 256) launch_qr_reg_shape<T, 256, 1, 16>(probs, n, nb, s);
-    else if (nb <= 512) launch_qr_reg_shape<T, 256, 2, 16>(probs, n, nb, s);
-    else launch_qr_reg_shape<T, 256, 4, 16>(probs, n, nb, s);
+    if (nb <= 256) launch_qr_reg_shape<T, 256, 1, 16>(probs, n, nb, s, q, qc);
+    else if (nb <= 512) launch_qr_reg_shape<T, 256, 2, 16>(probs, n, nb, s, q, qc);
+    else launch_qr_reg_shape<T, 256, 4, 16>(probs, n, nb, s, q, qc);
   } else {
-    if (nb <= 256) launch_qr_reg_shape<T, 256, 1, 32>(probs, n, nb, s);
-    else if (nb <= 512) launch_qr_reg_shape<T, 512, 1, 32>(probs, n, nb, s);
-    else launch_qr_reg_shape<T, 512, 2, 32>(probs, n, nb, s);
+    if (nb <= 256) launch_qr_reg_shape<T, 256, 1, 32>(probs, n, nb, s, q, qc);
+    else if (nb <= 512) launch_qr_reg_shape<T, 512, 1, 32>(probs, n, nb, s, q, qc);
+    else launch_qr_reg_shape<T, 512, 2, 32>(probs, n, nb, s, q, qc);
   }
   return true;
 }
 
 template <typename T>
-bool launch_qr_fused(const QrProb<T>* probs, int n, int nb, hipStream_t s) {
+bool launch_qr_fused(const QrProb<T>* probs, int n, int nb, hipStream_t s, const int* q, int qc) {
   if (n <= 0) return true;
   if (nb > kQrFusedMaxNb) return false;
   constexpr int NT = kQrfThreads, PB = kQrfPanel;
   static_assert(kQrFusedMaxNb <= 4 * NT, "fused QR shapes");
   const int rpt = (nb + NT - 1) / NT;
-  if (nb <= 256) hipLaunchKernelGGL((k_qr_fused<T, 256, 1, 32>), dim3(n), dim3(256), 0, s, probs, nb);
-  else if (rpt <= 1) hipLaunchKernelGGL((k_qr_fused<T, NT, 1, PB>), dim3(n), dim3(NT), 0, s, probs, nb);
-  else if (rpt <= 2) hipLaunchKernelGGL((k_qr_fused<T, NT, 2, PB>), dim3(n), dim3(NT), 0, s, probs, nb);
-  else hipLaunchKernelGGL((k_qr_fused<T, NT, 4, PB>), dim3(n), dim3(NT), 0, s, probs, nb);
+  if (nb <= 256)
+    hipLaunchKernelGGL((k_qr_fused<T, 256, 1, 32>), dim3(n), dim3(256), 0, s, probs, nb, q, qc);
+  else if (rpt <= 1)
+    hipLaunchKernelGGL((k_qr_fused<T, NT, 1, PB>), dim3(n), dim3(NT), 0, s, probs, nb, q, qc);
+  else if (rpt <= 2)
+    hipLaunchKernelGGL((k_qr_fused<T, NT, 2, PB>), dim3(n), dim3(NT), 0, s, probs, nb, q, qc);
+  else
+    hipLaunchKernelGGL((k_qr_fused<T, NT, 4, PB>), dim3(n), dim3(NT), 0, s, probs, nb, q, qc);
   return true;
 }
 

@@ -167,6 +167,16 @@ const int *HCORE_hip_plan_ranks_device(const hcore_hip_plan *plan);
 /* Copy new ranks / per-update status (HCORE_HIP_*) to host; synchronizes. */
 int HCORE_hip_plan_get_ranks(hcore_hip_plan *plan, int *ranks_host);
 int HCORE_hip_plan_get_status(hcore_hip_plan *plan, int *status_host);
+/*
+ * Plans with device-side ranks route each lane's core SVDs by the stacked
+ * rank K = ra + rc read at execution (not by the capacities): a one-workgroup
+ * kernel sorts the lane's updates by K (largest first) into a device queue
+ * and each core shape's launch serves its slice of it.  This copies the
+ * order of the last execution (update indices, up to n of them; needs a
+ * prior execution) and returns the lane's update count, 0 for a lane of a
+ * host-rank plan (sorted at finalize), or -HCORE_HIP_* on error.
+ */
+int HCORE_hip_plan_get_core_queue(hcore_hip_plan *plan, int lane, int *updates_host, int n);
 /* Copy the lane each update runs in (index into the timing lanes). */
 int HCORE_hip_plan_get_lanes(hcore_hip_plan *plan, int *lanes_host);
 /* Copy the number of Jacobi sweeps each update's core SVD used (diagnostic). */

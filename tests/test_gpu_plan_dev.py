@@ -168,3 +168,76 @@ def test_autotune_keeps_results(hc):
         b = OU[u].numpy(r) @ OV[u].numpy(r).T
         assert rel_fro(b, a) <= 10 * 1e-8
     plan.destroy()
+
+
+def test_device_rank_queue_routes_by_actual_rank(hc, orc):
+    """Device-rank plan whose capacities are far above the ranks in use (one
+    capacity for every tile, so the lanes by capacity collapse to one): each
+    execution's rank queue orders the updates by the stacked rank actually
+    read (largest first, every update once) and routes each core SVD by it;
+    two steps with different ranks replay one graph, every update within
+    10*acc of the dense update and its rank within one of the oracle's."""
+    from hcore_amd.tiles import TrailingStep
+    nb = 256
+    step = TrailingStep(grid=13, nb=nb, key="dev-queue")
+    pairs, rows = step.pairs, sorted(step.panel)
+    n = len(pairs)
+    rmax = max(max(step.panel[i][0].shape[1] for i in rows), max(c[0].shape[1] for c in step.C))
+    cap = nb // 2  # stacked capacity 2 cap = nb; ranks in use up to rmax
+    assert rmax < cap
+    B = hc.DeviceBuffer
+    pU = {i: B(nb, cap) for i in rows}
+    pV = {i: B(nb, cap) for i in rows}
+    CUd = [B(nb, 2 * cap) for _ in range(n)]
+    CVd = [B(nb, 2 * cap) for _ in range(n)]
+    OU = [B(nb, 2 * cap) for _ in range(n)]
+    OV = [B(nb, 2 * cap) for _ in range(n)]
+    prank = B(len(rows), 1, np.int32)
+    crank = B(n, 1, np.int32)
+    pos = {i: k for k, i in enumerate(rows)}
+    p_addr = lambda i: prank.ptr + 4 * pos[i]
+    plan = hc.Plan(nb)
+    plan.add_dgemm_dev([pU[i].ptr for i, j in pairs], [pV[i].ptr for i, j in pairs],
+                       [p_addr(i) for i, j in pairs], [cap] * n, nb,
+                       [pU[j].ptr for i, j in pairs], [pV[j].ptr for i, j in pairs],
+                       [p_addr(j) for i, j in pairs], [cap] * n, nb,
+                       [t.ptr for t in CUd], [t.ptr for t in CVd],
+                       [crank.ptr + 4 * u for u in range(n)], [cap] * n, nb, -1.0, 1.0, 0, nb,
+                       1e-8, CUout=[t.ptr for t in OU], CVout=[t.ptr for t in OV])
+    plan.finalize()
+    lanes = plan.lanes()
+    assert np.all(lanes == 0)
+    for shrink, label in ((0, "full ranks"), (6, "ranks - 6")):
+        _say("case: device rank queue,", label)
+        prk = {i: max(1, step.panel[i][0].shape[1] - shrink) for i in rows}
+        crk = [max(1, c[0].shape[1] - shrink) for c in step.C]
+        pan = {i: (_trim(step.panel[i][0], prk[i]), _trim(step.panel[i][1], prk[i])) for i in rows}
+        C = [(_trim(step.C[u][0], crk[u]), _trim(step.C[u][1], crk[u])) for u in range(n)]
+        for i in rows:
+            pU[i].upload(pan[i][0])
+            pV[i].upload(pan[i][1])
+        for u in range(n):
+            CUd[u].upload(C[u][0])
+            CVd[u].upload(C[u][1])
+        prank.upload(np.array([prk[i] for i in rows], dtype=np.int32))
+        crank.upload(np.array(crk, dtype=np.int32))
+        plan.execute(graph=True)
+        hc.device_synchronize()
+        assert np.all(plan.status() == 0)
+        order = plan.core_queue(0)
+        K = np.array([prk[i] + crk[u] for u, (i, j) in enumerate(pairs)])
+        assert sorted(order.tolist()) == list(range(n))
+        assert np.all(np.diff(K[order]) <= 0), K[order]
+        ranks = plan.ranks()
+        for u, (i, j) in enumerate(pairs):
+            AU, AV = pan[i]
+            BU, BV = pan[j]
+            CU, CV = C[u]
+            r = int(ranks[u])
+            st, oU, oV, ro, _, _ = orc.dgemm(-1.0, AU, AV, BU, BV, 1.0, CU, CV, maxrk=nb, acc=1e-8)
+            E = _dense(AU, AV, BU, BV, CU, CV)
+            assert abs(r - ro) <= 1, (label, u, r, ro)
+            assert rel_fro(OU[u].numpy(r) @ OV[u].numpy(r).T, E) <= 10 * 1e-8, (label, u)
+    plan.destroy()
+    for t in list(pU.values()) + list(pV.values()) + CUd + CVd + OU + OV + [prank, crank]:
+        t.free()
