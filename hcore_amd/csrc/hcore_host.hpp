@@ -82,12 +82,14 @@ struct PinnedArena {
 // compute mostly padding; the split-precision products (GF_SLICED) as 32 x 32
 // tiles with one wavefront (k <= 64) or four sharing the k range.
 struct TaskList {
-  std::vector<GemmTask> big, small, tall, oz1, oz4;
-  size_t size() const { return big.size() + small.size() + tall.size() + oz1.size() + oz4.size(); }
+  std::vector<GemmTask> big, small, tall, oz1, oz4, ozt;
+  size_t size() const {
+    return big.size() + small.size() + tall.size() + oz1.size() + oz4.size() + ozt.size();
+  }
 };
 struct TaskRef {
-  size_t ob = 0, os = 0, ot = 0, o1 = 0, o4 = 0;
-  int nb = 0, ns = 0, nt = 0, n1 = 0, n4 = 0;
+  size_t ob = 0, os = 0, ot = 0, o1 = 0, o4 = 0, oT = 0;
+  int nb = 0, ns = 0, nt = 0, n1 = 0, n4 = 0, nT = 0;
 };
 
 struct Packer {
@@ -111,6 +113,8 @@ struct Packer {
     r.n1 = (int)tl.oz1.size();
     r.o4 = add(tl.oz4);
     r.n4 = (int)tl.oz4.size();
+    r.oT = add(tl.ozt);
+    r.nT = (int)tl.ozt.size();
     return r;
   }
 };
@@ -121,15 +125,20 @@ struct Packer {
 template <typename T>
 void add_tasks(TaskList& tl, int prob, const GemmProb<T>& g) {
   const int m = g.m, n = g.n;
+  const bool tall = !(g.flags & (GF_A_TRANS | GF_LOWER_ZERO_UPPER | GF_LOWER_ONLY | GF_UPPER_ONLY)) &&
+                    m >= kTallMinM &&
+                    n <= kTallMaxN && g.k <= kTallMaxK;
+  if ((g.flags & GF_SLICED) && tall) {  // the tall split-precision kernel (same tiles)
+    for (int tj = 0; tj < (n + kTallCols - 1) / kTallCols; ++tj)
+      for (int ti = 0; ti < (m + kTallRows - 1) / kTallRows; ++ti) tl.ozt.push_back({prob, ti, tj});
+    return;
+  }
   if (g.flags & GF_SLICED) {
     auto& v = g.k > 64 ? tl.oz4 : tl.oz1;
     for (int tj = 0; tj < (n + 31) / 32; ++tj)
       for (int ti = 0; ti < (m + 31) / 32; ++ti) v.push_back({prob, ti, tj});
     return;
   }
-  const bool tall = !(g.flags & (GF_A_TRANS | GF_LOWER_ZERO_UPPER | GF_LOWER_ONLY | GF_UPPER_ONLY)) &&
-                    m >= kTallMinM &&
-                    n <= kTallMaxN && g.k <= kTallMaxK;
   if (tall) {
     for (int tj = 0; tj < (n + kTallCols - 1) / kTallCols; ++tj)
       for (int ti = 0; ti < (m + kTallRows - 1) / kTallRows; ++ti) tl.tall.push_back({prob, ti, tj});
@@ -157,12 +166,15 @@ void launch_tasks(const GemmProb<T>* probs, const char* base, const TaskRef& r,
   if constexpr (std::is_same<T, double>::value) {
     const auto* t1 = reinterpret_cast<const GemmTask*>(base + r.o1);
     const auto* t4 = reinterpret_cast<const GemmTask*>(base + r.o4);
+    const auto* tT = reinterpret_cast<const GemmTask*>(base + r.oT);
     if (slices > 0) {
       launch_gemm_oz(probs, t1, r.n1, slices, 1, s);
       launch_gemm_oz(probs, t4, r.n4, slices, 4, s);
+      launch_gemm_oz(probs, tT, r.nT, slices, 0, s);
     } else {
       launch_gemm<T>(probs, t1, r.n1, 32, s, false);
       launch_gemm<T>(probs, t4, r.n4, 32, s, true);
+      launch_gemm_tall<T>(probs, tT, r.nT, s);
     }
   }
 }

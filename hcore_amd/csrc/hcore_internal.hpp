@@ -148,7 +148,8 @@ template <typename T>
 void launch_gemm_tall(const GemmProb<T>* probs, const GemmTask* tasks, int ntasks, hipStream_t s);
 // split-precision FP64 products on the BF16 matrix cores: 32 x 32 output tiles
 // (tasks as for tile 32), `slices` digits per operand (kOzMinSlices..
-// kOzMaxSlices), ks = 1 or 4 wavefronts sharing a tile's k range
+// kOzMaxSlices), ks = 1 or 4 wavefronts sharing a tile's k range; ks = 0: the
+// tall-skinny kernel on kTallRows x kTallCols tasks (as launch_gemm_tall)
 constexpr int kOzMinSlices = 2, kOzMaxSlices = 6;
 void launch_gemm_oz(const GemmProb<double>* probs, const GemmTask* tasks, int ntasks, int slices,
                     int ks, hipStream_t s);

@@ -632,9 +632,149 @@ __global__ __launch_bounds__(64 * KS) void k_gemm_oz(const GemmProb<double>* __r
   }
 }
 
+// Tall-skinny split-precision products (A not transposed, m >= kTallMinM,
+// n <= kTallMaxN, k <= kTallMaxK; the FP64 tall kernel's tasks of kTallRows x
+// kTallCols): per k chunk of 32 the workgroup cuts the 32 x 32 block of op(B)
+// into digits once (8 threads per column, column exponents by lane shuffles)
+// and keeps them in LDS; each wavefront reads its 64 rows of A straight from
+// HBM (16 consecutive rows per load) and cuts them itself.
+template <int S>
+__global__ __launch_bounds__(256) void k_gemm_oz_tall(const GemmProb<double>* __restrict__ probs,
+                                                      const GemmTask* __restrict__ tasks) {
+  constexpr int NC = kTallCols, KC = 32, RW = kTallRows / 4;  // 64 rows per wavefront
+  constexpr int BDK = KC + 8;  // k stride of a digit column (bank-conflict-free 16-byte reads)
+  static_assert(NC == 32 && RW == 64, "tall split-precision shape");
+  __shared__ double Bs[KC][NC + 1];
+  __shared__ __align__(16) __bf16 Bd[S][NC][BDK];  // rows of 80 bytes: 16-byte reads stay aligned
+  __shared__ int eB[NC];
+  const GemmTask tk = tasks[blockIdx.x];
+  const GemmProb<double> P = probs[tk.prob];
+  if (P.skip && *P.skip) return;
+  int n = P.n;
+  if (P.n_dev) n = min(n, *P.n_dev);
+  const int row0 = tk.ti * kTallRows, col0 = tk.tj * NC;
+  if (row0 >= P.m || col0 >= n) return;
+  const int nc = min(NC, n - col0);
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  const int rw = row0 + w * RW;
+  const bool tb = (P.flags & GF_B_TRANS) != 0;
+  const bool ua = (P.flags & GF_A_UPPER) != 0, ub = (P.flags & GF_B_UPPER) != 0;
+  const bool la = (P.flags & GF_A_UNIT_LOWER) != 0, lb = (P.flags & GF_B_UNIT_LOWER) != 0;
+  int klo, khi;
+  gemm_k_range<kTallRows, KC>(P.flags, row0, col0, P.k, klo, khi);
+  double acc[4][2][4];  // C(rw + 16 rb + li, col0 + 16 cb + 4 lg + v)
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[a][b][v] = 0.0;
+  for (int k0 = klo; k0 < khi; k0 += KC) {
+    for (int e = t; e < KC * NC; e += 256) {  // op(B)(k0 + kk, col0 + j)
+      int kk, j;
+      if (tb) { j = e % NC; kk = e / NC; }
+      else    { kk = e % KC; j = e / KC; }
+      double v = 0.0;
+      if (k0 + kk < P.k && j < nc) {
+        const int gk = k0 + kk, gj = col0 + j;
+        v = tb ? gemm_masked(P.B, P.ldb, gj, gk, ub, lb) : gemm_masked(P.B, P.ldb, gk, gj, ub, lb);
+      }
+      Bs[kk][j] = v;
+    }
+    __syncthreads();
+    {  // the block's digits: thread t takes column t / 8, k = 4 (t % 8) .. + 3
+      const int j = t >> 3, kq = (t & 7) * 4;
+      double v[4], m = 0.0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        v[i] = Bs[kq + i][j];
+        m = fmax(m, fabs(v[i]));
+      }
+      int e = 0;
+      (void)frexp(m, &e);
+      if (m == 0.0) e = -1100;
+      e = max(e, __shfl_xor(e, 1));
+      e = max(e, __shfl_xor(e, 2));
+      e = max(e, __shfl_xor(e, 4));
+      e = e == -1100 ? 0 : e;
+      if ((t & 7) == 0) eB[j] = e;
+      double x[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) x[i] = ldexp(v[i], -e);
+#pragma unroll
+      for (int sd = 0; sd < S; ++sd)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const double tt = x[i] * 256.0;
+          const double d = rint(tt);
+          x[i] = tt - d;
+          Bd[sd][j][kq + i] = __builtin_bit_cast(__bf16, (unsigned short)(__float_as_uint((float)d) >> 16));
+        }
+    }
+    __syncthreads();
+    OzBf8 sb[2][S];
+    int ej[2][4];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+#pragma unroll
+      for (int sd = 0; sd < S; ++sd)
+        sb[cb][sd] = *reinterpret_cast<const OzBf8*>(&Bd[sd][16 * cb + li][8 * lg]);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) ej[cb][v] = eB[16 * cb + 4 * lg + v];
+    }
+#pragma unroll 1
+    for (int rb = 0; rb < 4; ++rb) {
+      double va[8];
+      const int row = rw + 16 * rb + li;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int gk = k0 + 8 * lg + q;
+        va[q] = (gk < P.k && row < P.m) ? gemm_masked(P.A, P.lda, row, gk, ua, la) : 0.0;
+      }
+      const int ea = oz_exponent(va);
+      OzBf8 sa[S];
+      oz_digits<S>(va, ea, sa);
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) {
+        OzAcc lv[S];
+#pragma unroll
+        for (int L = 0; L < S; ++L) {
+          lv[L] = OzAcc{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int sd = 0; sd <= L; ++sd)
+            lv[L] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sb[cb][L - sd], sa[sd], lv[L], 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          double sum = (double)lv[S - 1][v];
+#pragma unroll
+          for (int L = S - 2; L >= 0; --L) sum = fma(sum, 0x1p-8, (double)lv[L][v]);
+          const double c = ldexp(sum, ea + ej[cb][v] - 16);
+#pragma unroll
+          for (int a = 0; a < 4; ++a)
+            if (a == rb) acc[a][cb][v] += c;
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        gemm_store1(P, rw + 16 * a + li, col0 + 16 * b + 4 * lg + v, n, acc[a][b][v]);
+}
+
 template <int S>
 static void launch_gemm_oz_s(const GemmProb<double>* probs, const GemmTask* tasks, int ntasks,
                              int ks, hipStream_t s) {
+  if (ks == 0) {
+    hipLaunchKernelGGL((k_gemm_oz_tall<S>), dim3(ntasks), dim3(256), 0, s, probs, tasks);
+    return;
+  }
   if (ks == 4)
     hipLaunchKernelGGL((k_gemm_oz<S, 4>), dim3(ntasks), dim3(256), 0, s, probs, tasks);
   else
