@@ -192,6 +192,18 @@ def reduced_precision_leg(step, stream, steps, warmup, acc, sample):
         ms = graph_ms(plan, stream, steps, warmup)
         out[name] = {"ms_per_step": ms, "value": len(step.pairs) / (ms / 1e3),
                      "worst_rel_err": worst, "mean_rank": float(np.mean(ranks))}
+        if prec == hc.SPLIT and acc == 1e-8:  # per-phase durations, lanes one after another
+            plan.set_timing(2)
+            plan.execute(stream)
+            stream.synchronize()
+            ph_ms, ph_n = plan.phase_times()
+            plan.set_timing(False)
+            for l in range(ph_ms.shape[0]):
+                row = {PHASES[i]: float(ph_ms[l, i] / ph_n[l, i]) for i in range(len(PHASES))
+                       if ph_n[l, i] > 0}
+                if row:
+                    out[name].setdefault("phase_ms_serialized", {})["lane%d" % l] = row
+                    log("  split lane %d: %s" % (l, " ".join("%s %.3f" % kv for kv in row.items())))
         if tuned is not None:
             out[name]["slices_per_lane"] = [int(t[1]) for t in tuned]
             out[name]["sigma_dev_over_acc"] = [float(t[2]) for t in tuned]
