@@ -170,14 +170,19 @@ def test_autotune_keeps_results(hc):
     plan.destroy()
 
 
-def test_device_rank_queue_routes_by_actual_rank(hc, orc):
+@pytest.mark.parametrize("form", ["fp64", "split", "mixed"])
+def test_device_rank_queue_routes_by_actual_rank(hc, orc, form):
     """Device-rank plan whose capacities are far above the ranks in use (one
     capacity for every tile, so the lanes by capacity collapse to one): each
     execution's rank queue orders the updates by the stacked rank actually
     read (largest first, every update once) and routes each core SVD by it;
     two steps with different ranks replay one graph, every update within
-    10*acc of the dense update and its rank within one of the oracle's."""
+    10*acc of the dense update (of the FP32-rounded tiles for the mixed form)
+    and its rank within one of the oracle's.  FP64 and split precision at
+    acc 1e-8, the mixed form (FP32 tiles, FP64 core) at 1e-5."""
     from hcore_amd.tiles import TrailingStep
+    prec, dt, acc = {"fp64": (hc.FP64, np.float64, 1e-8), "split": (hc.SPLIT, np.float64, 1e-8),
+                     "mixed": (hc.MIXED, np.float32, 1e-5)}[form]
     nb = 256
     step = TrailingStep(grid=13, nb=nb, key="dev-queue")
     pairs, rows = step.pairs, sorted(step.panel)
@@ -186,33 +191,35 @@ def test_device_rank_queue_routes_by_actual_rank(hc, orc):
     cap = nb // 2  # stacked capacity 2 cap = nb; ranks in use up to rmax
     assert rmax < cap
     B = hc.DeviceBuffer
-    pU = {i: B(nb, cap) for i in rows}
-    pV = {i: B(nb, cap) for i in rows}
-    CUd = [B(nb, 2 * cap) for _ in range(n)]
-    CVd = [B(nb, 2 * cap) for _ in range(n)]
-    OU = [B(nb, 2 * cap) for _ in range(n)]
-    OV = [B(nb, 2 * cap) for _ in range(n)]
+    pU = {i: B(nb, cap, dt) for i in rows}
+    pV = {i: B(nb, cap, dt) for i in rows}
+    CUd = [B(nb, 2 * cap, dt) for _ in range(n)]
+    CVd = [B(nb, 2 * cap, dt) for _ in range(n)]
+    OU = [B(nb, 2 * cap, dt) for _ in range(n)]
+    OV = [B(nb, 2 * cap, dt) for _ in range(n)]
     prank = B(len(rows), 1, np.int32)
     crank = B(n, 1, np.int32)
     pos = {i: k for k, i in enumerate(rows)}
     p_addr = lambda i: prank.ptr + 4 * pos[i]
-    plan = hc.Plan(nb)
+    plan = hc.Plan(nb, prec)
     plan.add_dgemm_dev([pU[i].ptr for i, j in pairs], [pV[i].ptr for i, j in pairs],
                        [p_addr(i) for i, j in pairs], [cap] * n, nb,
                        [pU[j].ptr for i, j in pairs], [pV[j].ptr for i, j in pairs],
                        [p_addr(j) for i, j in pairs], [cap] * n, nb,
                        [t.ptr for t in CUd], [t.ptr for t in CVd],
                        [crank.ptr + 4 * u for u in range(n)], [cap] * n, nb, -1.0, 1.0, 0, nb,
-                       1e-8, CUout=[t.ptr for t in OU], CVout=[t.ptr for t in OV])
+                       acc, CUout=[t.ptr for t in OU], CVout=[t.ptr for t in OV])
     plan.finalize()
     lanes = plan.lanes()
     assert np.all(lanes == 0)
+    rnd = lambda x: np.asfortranarray(x.astype(dt).astype(np.float64))
     for shrink, label in ((0, "full ranks"), (6, "ranks - 6")):
-        _say("case: device rank queue,", label)
+        _say("case: device rank queue,", form, label)
         prk = {i: max(1, step.panel[i][0].shape[1] - shrink) for i in rows}
         crk = [max(1, c[0].shape[1] - shrink) for c in step.C]
-        pan = {i: (_trim(step.panel[i][0], prk[i]), _trim(step.panel[i][1], prk[i])) for i in rows}
-        C = [(_trim(step.C[u][0], crk[u]), _trim(step.C[u][1], crk[u])) for u in range(n)]
+        pan = {i: (rnd(_trim(step.panel[i][0], prk[i])), rnd(_trim(step.panel[i][1], prk[i])))
+               for i in rows}
+        C = [(rnd(_trim(step.C[u][0], crk[u])), rnd(_trim(step.C[u][1], crk[u]))) for u in range(n)]
         for i in rows:
             pU[i].upload(pan[i][0])
             pV[i].upload(pan[i][1])
@@ -234,10 +241,11 @@ def test_device_rank_queue_routes_by_actual_rank(hc, orc):
             BU, BV = pan[j]
             CU, CV = C[u]
             r = int(ranks[u])
-            st, oU, oV, ro, _, _ = orc.dgemm(-1.0, AU, AV, BU, BV, 1.0, CU, CV, maxrk=nb, acc=1e-8)
+            st, oU, oV, ro, _, _ = orc.dgemm(-1.0, AU, AV, BU, BV, 1.0, CU, CV, maxrk=nb, acc=acc)
             E = _dense(AU, AV, BU, BV, CU, CV)
             assert abs(r - ro) <= 1, (label, u, r, ro)
-            assert rel_fro(OU[u].numpy(r) @ OV[u].numpy(r).T, E) <= 10 * 1e-8, (label, u)
+            got = OU[u].numpy(r).astype(np.float64) @ OV[u].numpy(r).astype(np.float64).T
+            assert rel_fro(got, E) <= 10 * acc, (label, u)
     plan.destroy()
     for t in list(pU.values()) + list(pV.values()) + CUd + CVd + OU + OV + [prank, crank]:
         t.free()
