@@ -39,6 +39,17 @@ __host__ __device__ constexpr int pow2_ceil(int n) {
   return p;
 }
 
+// v of lane l (a wave-uniform constant lane) in scalar registers.
+__device__ __forceinline__ double lane_read(double v, int l) {
+  const unsigned long long b = __double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, l);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), l);
+  return __longlong_as_double(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ float lane_read(float v, int l) {
+  return __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(v), l));
+}
+
 template <typename T>
 __device__ __forceinline__ bool is_finite_val(T v) {
   return fabs(v) <= std::numeric_limits<T>::max();
@@ -136,12 +147,12 @@ __device__ __forceinline__ T& jreg(T (&a)[B][RPL], T (&b)[B][RPL], int idx, int 
 }
 
 // One round of B disjoint pairs.  nrm: the wave's 2B squared column norms
-// (LDS), rot: 2B values of LDS scratch.  The lanes that hold a pair's reduced
+// (LDS); the rotations reach every lane through scalar registers.  The lanes that hold a pair's reduced
 // inner product compute its rotation and keep the largest relative
 // off-diagonal |a_pq| / sqrt(a_pp a_qq) they have seen in rho.
 template <bool CROSS, typename T, int B, int RPL>
-__device__ __forceinline__ void jacobi_round(T (&a)[B][RPL], T (&b)[B][RPL], int t, T* nrm, T* rot,
-                                             int lane, T tol, T thr2, int* rotflag, float& rho) {
+__device__ __forceinline__ void jacobi_round(T (&a)[B][RPL], T (&b)[B][RPL], int t, T* nrm, int lane,
+                                             T tol, T thr2, int* rotflag, float& rho) {
   constexpr int NB = pow2_ceil(B);
   T d[NB];
 #pragma unroll
@@ -156,35 +167,41 @@ __device__ __forceinline__ void jacobi_round(T (&a)[B][RPL], T (&b)[B][RPL], int
   }
   reduce_scatter<NB>(d, lane);
   const int idx = reduce_scatter_index<NB>(lane);
+  T cs_l = T(1), sn_l = T(0);  // pair idx's rotation, in the lane that holds its inner product
   if ((lane & ((64 / NB) - 1)) == 0 && idx < B) {
     int u, v;
     jacobi_pair<CROSS, B>(t, idx, u, v);
     const T app = nrm[u], aqq = nrm[v], apq = d[0];
-    T cs, sn, tt;
-    if (jacobi_params(app, aqq, apq, tol, thr2, cs, sn, tt, rho)) {
+    T tt;
+    if (jacobi_params(app, aqq, apq, tol, thr2, cs_l, sn_l, tt, rho)) {
       nrm[u] = fma(-tt, apq, app);
       nrm[v] = fma(tt, apq, aqq);
       *rotflag = 1;
     }
-    rot[2 * idx] = cs;
-    rot[2 * idx + 1] = sn;
   }
-  wave_sync();
+  // pair q's rotation read from lane q * 64 / NB into scalar registers (no LDS
+  // round trip); a pair left alone carries (1, 0), applied like any other (the
+  // matrix is finite here, so the identity rotation changes nothing)
+  T csq[B], snq[B];
 #pragma unroll
   for (int q = 0; q < B; ++q) {
-    const T cs = rot[2 * q], sn = rot[2 * q + 1];
-    if (sn != T(0)) {
-      int u, v;
-      jacobi_pair<CROSS, B>(t, q, u, v);
+    csq[q] = lane_read(cs_l, q * (64 / NB));
+    snq[q] = lane_read(sn_l, q * (64 / NB));
+  }
+  __builtin_amdgcn_sched_barrier(0);  // the per-lane rotation registers die here
 #pragma unroll
-      for (int i = 0; i < RPL; ++i) {
-        const T xu = jreg(a, b, u, i), xv = jreg(a, b, v, i);
-        jreg(a, b, u, i) = cs * xu - sn * xv;
-        jreg(a, b, v, i) = sn * xu + cs * xv;
-      }
+  for (int q = 0; q < B; ++q) {
+    const T cs = csq[q], sn = snq[q];
+    int u, v;
+    jacobi_pair<CROSS, B>(t, q, u, v);
+#pragma unroll
+    for (int i = 0; i < RPL; ++i) {
+      const T xu = jreg(a, b, u, i), xv = jreg(a, b, v, i);
+      jreg(a, b, u, i) = cs * xu - sn * xv;
+      jreg(a, b, v, i) = sn * xu + cs * xv;
     }
   }
-  wave_sync();
+  wave_sync();  // the squared norms (LDS) before the next round reads them
 }
 
 // Move the blocks one place along the ring (top_0 stays): through LDS, one
@@ -254,7 +271,7 @@ struct CoreReg {
   static_assert(CPW * W >= KM && NBLK * B >= KM && B % 2 == 0 && N2 <= 64 && NQ <= 64,
                 "register core shape");
   static constexpr size_t n_elem = (size_t)BIG + 2 * KM + W * XS + 2 * W * B + NBLK * B +
-                                   2 * W * B + 2 * W + 2;
+                                   2 * W + 2;
   static constexpr size_t n_int = 2 * W + 4 * KM + 8;
   static constexpr size_t smem = n_elem * sizeof(T) + n_int * sizeof(int);
 };
@@ -272,8 +289,7 @@ __global__ __launch_bounds__(64 * W) void k_core_reg(const CoreProb<T>* __restri
   T* xs_all = sig + KM;                     // W x XS
   T* nrm_all = xs_all + W * C::XS;          // W x 2B block norms
   T* nst = nrm_all + W * 2 * B;             // NBLK x B norms in transit
-  T* rot_all = nst + C::NBLK * B;           // W x 2B rotations
-  T* cand_v = rot_all + W * 2 * B;          // 2 x W pivot candidates
+  T* cand_v = nst + C::NBLK * B;            // 2 x W pivot candidates
   T* misc = cand_v + 2 * W;                 // [0] tau
   int* cand_i = reinterpret_cast<int*>(misc + 2);  // 2 x W
   int* piv = cand_i + 2 * W;                       // KM: column at each position
@@ -524,7 +540,6 @@ __global__ __launch_bounds__(64 * W) void k_core_reg(const CoreProb<T>* __restri
 
   // ---- block one-sided Jacobi on R^T ----
   T* nrm = nrm_all + w * 2 * B;
-  T* rot = rot_all + w * 2 * B;
   {
     T d[2 * B];
 #pragma unroll
@@ -561,11 +576,11 @@ __global__ __launch_bounds__(64 * W) void k_core_reg(const CoreProb<T>* __restri
         if (o == 0) {
 #pragma unroll
           for (int t = 0; t < B - 1; ++t)
-            jacobi_round<false>(a, b, t, nrm, rot, lane, tol, thr2, rf, rho);
+            jacobi_round<false>(a, b, t, nrm, lane, tol, thr2, rf, rho);
         }
 #pragma unroll
         for (int t = 0; t < B; ++t)
-          jacobi_round<true>(a, b, t, nrm, rot, lane, tol, thr2, rf, rho);
+          jacobi_round<true>(a, b, t, nrm, lane, tol, thr2, rf, rho);
         if (o == 2 * W - 2) {  // publish the sweep's maximum before the last exchange
 #pragma unroll
           for (int off = 32; off >= 1; off >>= 1) rho = fmaxf(rho, __shfl_xor(rho, off, 64));
