@@ -173,6 +173,17 @@ __device__ __forceinline__ float fast_rsq<float>(float x) {
   return fmaf(0.5f * r, fmaf(-x * r, r, 1.0f), r);
 }
 
+// v of lane l (a wave-uniform lane) in scalar registers.
+__device__ __forceinline__ double lane_read(double v, int l) {
+  const unsigned long long b = __double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, l);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), l);
+  return __longlong_as_double(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ float lane_read(float v, int l) {
+  return __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(v), l));
+}
+
 template <int NT>
 __device__ __forceinline__ void block_sync() {
   if constexpr (NT == 64) wave_sync();

@@ -39,17 +39,6 @@ __host__ __device__ constexpr int pow2_ceil(int n) {
   return p;
 }
 
-// v of lane l (a wave-uniform constant lane) in scalar registers.
-__device__ __forceinline__ double lane_read(double v, int l) {
-  const unsigned long long b = __double_as_longlong(v);
-  const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, l);
-  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), l);
-  return __longlong_as_double(((unsigned long long)hi << 32) | lo);
-}
-__device__ __forceinline__ float lane_read(float v, int l) {
-  return __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(v), l));
-}
-
 template <typename T>
 __device__ __forceinline__ bool is_finite_val(T v) {
   return fabs(v) <= std::numeric_limits<T>::max();

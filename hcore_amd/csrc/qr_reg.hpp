@@ -35,7 +35,7 @@ namespace hcore {
 template <int J, typename T, int NT, int RPT, int KMAX>
 __device__ __forceinline__ void qr_reg_col(T (&x)[RPT][KMAX], int K, int j0, int tid, int lane,
                                            int wid, T (*part)[NT / 64][KMAX], T (*rowj)[KMAX],
-                                           T (*wvw)[KMAX], T (*Sl)[KMAX + 1], T* taus) {
+                                           T (*Sl)[KMAX + 1], T* taus) {
   constexpr int NW = NT / 64;
   constexpr int CH = 16;       // values per batched wave reduction
   constexpr int BUF = J & 1;   // partials / row g buffer of this column
@@ -82,28 +82,38 @@ __device__ __forceinline__ void qr_reg_col(T (&x)[RPT][KMAX], int K, int j0, int
     scal = T(1) / (alpha - beta);
   }
   const T dotv = fma(scal, tot, rowj[BUF][t]);  // y_t^T v_g (t < J), v_g^T x_t (t > J)
-  if (lane < KMAX) wvw[wid][lane] = (lane > J && j0 + lane < K) ? tau * dotv : T(0);
+  // reflector products w_t (t > J) in lane t, read by every lane into scalar
+  // registers KC at a time (no LDS round trip)
+  const T w_l = (lane > J && lane < KMAX && j0 + lane < K) ? tau * dotv : T(0);
   if (wid == 0) {
     if (lane < J) Sl[J][lane] = dotv;
     if (lane == 0) taus[J] = tau;
   }
-  wave_sync();
+  // per row: the reflector entry v (rows below g), 1 (row g), 0 (rows above:
+  // unchanged, the matrix being finite)
+  T vr[RPT];
 #pragma unroll
   for (int r = 0; r < RPT; ++r) {
     const int row = tid + r * NT;
-    if (row > g) {
-      const T v = x[r][J] * scal;
-      x[r][J] = v;
+    const T v = x[r][J] * scal;
+    vr[r] = row > g ? v : (row == g ? T(1) : T(0));
+    x[r][J] = row > g ? v : (row == g ? beta : x[r][J]);
+  }
+  constexpr int KC = 8;
 #pragma unroll
-      for (int k = J + 1; k < KMAX; ++k) x[r][k] = fma(-v, wvw[wid][k], x[r][k]);
-    } else if (row == g) {
-      x[r][J] = beta;
+  for (int k0 = J + 1; k0 < KMAX; k0 += KC) {
+    T wk[KC];
 #pragma unroll
-      for (int k = J + 1; k < KMAX; ++k) x[r][k] -= wvw[wid][k];
-    }
+    for (int c = 0; c < KC; ++c)
+      if (k0 + c < KMAX) wk[c] = lane_read(w_l, k0 + c);
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+#pragma unroll
+      for (int c = 0; c < KC; ++c)
+        if (k0 + c < KMAX) x[r][k0 + c] = fma(-vr[r], wk[c], x[r][k0 + c]);
   }
   if constexpr (J + 1 < KMAX)
-    qr_reg_col<J + 1, T, NT, RPT, KMAX>(x, K, j0, tid, lane, wid, part, rowj, wvw, Sl, taus);
+    qr_reg_col<J + 1, T, NT, RPT, KMAX>(x, K, j0, tid, lane, wid, part, rowj, Sl, taus);
 }
 
 // The panel's compact-WY factor from its Gram entries, column by column:
@@ -144,7 +154,6 @@ __global__ __launch_bounds__(NT) void k_qr_reg(const QrProb<T>* __restrict__ pro
   static_assert(KMAX % 16 == 0 && KMAX <= 64 && NT >= KMAX && KMAX % kPB == 0, "register QR shape");
   __shared__ T part[2][NW][KMAX];
   __shared__ T rowj[2][KMAX];
-  __shared__ T wvw[NW][KMAX];       // each wavefront's copy of the reflector products
   __shared__ T Sl[KMAX][KMAX + 1];  // Sl[j][t] = y_t^T y_j, t < j (panel columns)
   __shared__ T taus[KMAX];
   __shared__ T Tf[KMAX][KMAX + 1];  // the panel's compact-WY factor
@@ -166,7 +175,7 @@ __global__ __launch_bounds__(NT) void k_qr_reg(const QrProb<T>* __restrict__ pro
     for (int k = 0; k < KMAX; ++k)
       x[r][k] = (row < nb && k < pw) ? P.A[row + (size_t)(j0 + k) * nb] : T(0);
   }
-  qr_reg_col<0, T, NT, RPT, KMAX>(x, K, j0, tid, lane, wid, part, rowj, wvw, Sl, taus);
+  qr_reg_col<0, T, NT, RPT, KMAX>(x, K, j0, tid, lane, wid, part, rowj, Sl, taus);
   __syncthreads();
   if (tid < pw) P.tau[j0 + tid] = taus[tid];
 #pragma unroll
@@ -204,7 +213,6 @@ __global__ __launch_bounds__(NT) void k_qr_fused(const QrProb<T>* __restrict__ p
   static_assert(PB % CH == 0 && PB <= 64 && NT >= PB * CB && PB % kPB == 0, "fused QR shape");
   __shared__ T part[2][NW][PB];
   __shared__ T rowj[2][PB];
-  __shared__ T wvw[NW][PB];
   __shared__ T Sl[PB][PB + 1];
   __shared__ T taus[PB];
   __shared__ T Tf[PB][PB + 1];
@@ -229,7 +237,7 @@ __global__ __launch_bounds__(NT) void k_qr_fused(const QrProb<T>* __restrict__ p
       for (int k = 0; k < PB; ++k)
         x[r][k] = (row < nb && k < pw) ? P.A[row + (size_t)(j0 + k) * nb] : T(0);
     }
-    qr_reg_col<0, T, NT, RPT, PB>(x, K, j0, tid, lane, wid, part, rowj, wvw, Sl, taus);
+    qr_reg_col<0, T, NT, RPT, PB>(x, K, j0, tid, lane, wid, part, rowj, Sl, taus);
     __syncthreads();
     if (tid < pw) P.tau[j0 + tid] = taus[tid];
 #pragma unroll
