@@ -77,6 +77,35 @@ __device__ __forceinline__ double lane_partner(double v) {
   return __longlong_as_double(((unsigned long long)hi << 32) | lo);
 }
 
+// Value of the partner lane at any OFF = 1..32 (DPP up to 8; for 16 and 32
+// the permlane swap of v with itself, whose two results hold the own and the
+// partner's value in an order set by the lane's bit OFF).
+template <int OFF>
+__device__ __forceinline__ unsigned partner_u32(unsigned v, int lane) {
+  if constexpr (OFF <= 8) {
+    return dpp_partner<OFF>(v);
+  } else {
+    static_assert(OFF == 16 || OFF == 32, "partner offset");
+    const auto r = OFF == 32 ? __builtin_amdgcn_permlane32_swap(v, v, false, false)
+                             : __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return (lane & OFF) ? r[0] : r[1];
+  }
+}
+template <int OFF>
+__device__ __forceinline__ int partner_any(int v, int lane) {
+  return (int)partner_u32<OFF>((unsigned)v, lane);
+}
+template <int OFF>
+__device__ __forceinline__ double partner_any(double v, int lane) {
+  const unsigned long long b = __double_as_longlong(v);
+  const unsigned lo = partner_u32<OFF>((unsigned)b, lane), hi = partner_u32<OFF>((unsigned)(b >> 32), lane);
+  return __longlong_as_double(((unsigned long long)hi << 32) | lo);
+}
+template <int OFF>
+__device__ __forceinline__ float partner_any(float v, int lane) {
+  return __uint_as_float(partner_u32<OFF>(__float_as_uint(v), lane));
+}
+
 // v + v(partner at OFF), for any OFF = 1..32.
 template <int OFF, typename T>
 __device__ __forceinline__ T add_partner(T v) {

@@ -233,15 +233,22 @@ __device__ __forceinline__ void qrcp_candidate(T nl, unsigned long long done_mas
                                                T* cand_v, int* cand_i) {
   T bv = lane < CPW && !((done_mask >> lane) & 1) ? nl : T(-1);
   int bi = bv >= T(0) ? lane * W + w : -1;
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    const T ov = __shfl_xor(bv, off, 64);
-    const int oi = __shfl_xor(bi, off, 64);
+  // butterfly arg-max through DPP / permlane exchanges (no LDS crossbar)
+  auto level = [&](auto off_c) {
+    constexpr int OFF = decltype(off_c)::value;
+    const T ov = partner_any<OFF>(bv, lane);
+    const int oi = partner_any<OFF>(bi, lane);
     if (oi >= 0 && (ov > bv || (ov == bv && (bi < 0 || oi < bi)))) {
       bv = ov;
       bi = oi;
     }
-  }
+  };
+  level(std::integral_constant<int, 32>{});
+  level(std::integral_constant<int, 16>{});
+  level(std::integral_constant<int, 8>{});
+  level(std::integral_constant<int, 4>{});
+  level(std::integral_constant<int, 2>{});
+  level(std::integral_constant<int, 1>{});
   if (lane == 0) {
     cand_v[w] = bv;
     cand_i[w] = bi;
