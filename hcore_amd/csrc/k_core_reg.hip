@@ -569,14 +569,23 @@ __global__ __launch_bounds__(64 * W) void k_core_reg(const CoreProb<T>* __restri
       }
 #pragma unroll 1
       for (int o = 0; o < 2 * W - 1; ++o) {
-        if (o == 0) {
+        // A column at or below the floor thr2 never rotates (jacobi_params),
+        // so a block without a column above it (the slots past the pivoted
+        // QR's numerical rank) makes every pair it takes part in a no-op:
+        // such rounds are skipped, which changes no result.
+        const T nv = lane < 2 * B ? nrm[lane] : T(0);
+        const bool live_a = __any(lane < B && nv > thr2);
+        const bool live_b = __any(lane >= B && lane < 2 * B && nv > thr2);
+        if (o == 0 && (live_a || live_b)) {
 #pragma unroll
           for (int t = 0; t < B - 1; ++t)
             jacobi_round<false>(a, b, t, nrm, lane, tol, thr2, rf, rho);
         }
+        if (live_a && live_b) {
 #pragma unroll
-        for (int t = 0; t < B; ++t)
-          jacobi_round<true>(a, b, t, nrm, lane, tol, thr2, rf, rho);
+          for (int t = 0; t < B; ++t)
+            jacobi_round<true>(a, b, t, nrm, lane, tol, thr2, rf, rho);
+        }
         if (o == 2 * W - 2) {  // publish the sweep's maximum before the last exchange
 #pragma unroll
           for (int off = 32; off >= 1; off >>= 1) rho = fmaxf(rho, __shfl_xor(rho, off, 64));
