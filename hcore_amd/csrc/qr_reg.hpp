@@ -102,6 +102,7 @@ __device__ __forceinline__ void qr_reg_col(T (&x)[RPT][KMAX], int K, int j0, int
   constexpr int KC = 8;
 #pragma unroll
   for (int k0 = J + 1; k0 < KMAX; k0 += KC) {
+    if (j0 + k0 >= K) break;  // uniform: the columns past the stack's K are empty
     T wk[KC];
 #pragma unroll
     for (int c = 0; c < KC; ++c)
