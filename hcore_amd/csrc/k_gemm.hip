@@ -139,7 +139,7 @@ __device__ __forceinline__ void gemm_stage(T (*As)[LDT], T (*Bs)[LDT], const T (
 }
 
 // C(i, j) = alpha * acc + beta * C (or beta * C0 rows < m0, 0 below), with the
-// strict upper triangle zeroed under GF_LOWER_ZERO_UPPER.
+// strict upper triangle zeroed under GF_LOWER_ZERO_UPPER: one value.
 template <typename T>
 __device__ __forceinline__ void gemm_store1(const GemmProb<T>& P, int i, int j, int n, T acc) {
   if (i >= P.m || j >= n) return;
@@ -158,18 +158,67 @@ __device__ __forceinline__ void gemm_store1(const GemmProb<T>& P, int i, int j, 
   *c = v;
 }
 
+// The same for the NV values (i[q], j[q], acc[q]) of one lane.  Every
+// beta-term load is issued before the first store: as far as the compiler
+// knows a store through C may alias any later load, so a load after a store
+// would wait out one HBM round trip per value.
+template <typename T, int NV>
+__device__ __forceinline__ void gemm_store_n(const GemmProb<T>& P, const int (&i)[NV],
+                                             const int (&j)[NV], int n, const T (&acc)[NV]) {
+  const bool c0 = (P.flags & GF_C0) != 0, rd = !c0 && !(P.flags & GF_BETA_ZERO);
+  if (!c0 && !rd) {  // uniform: nothing to read, each value stored as computed
+#pragma unroll
+    for (int q = 0; q < NV; ++q) gemm_store1(P, i[q], j[q], n, acc[q]);
+    return;
+  }
+  const bool lo = (P.flags & GF_LOWER_ONLY) != 0, up = (P.flags & GF_UPPER_ONLY) != 0;
+  const bool lzu = (P.flags & GF_LOWER_ZERO_UPPER) != 0;
+  bool wr[NV], nd[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    wr[q] = i[q] < P.m && j[q] < n && !((lo && i[q] < j[q]) || (up && i[q] > j[q]));
+    nd[q] = wr[q] && !(lzu && i[q] < j[q]) && (c0 ? i[q] < P.m0 : rd);
+  }
+  // the loads are unconditional (a value not needed reads the source's first
+  // element), so they all issue back to back
+  const T* src = c0 ? P.C0 : P.C;
+  const size_t ld = c0 ? (size_t)P.ldc0 : (size_t)P.ldc;
+  T b[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) b[q] = src[nd[q] ? i[q] + (size_t)j[q] * ld : 0];
+  // every value before the first store: a use of a loaded value after a store
+  // would wait for the store as well (the empty asm statements keep the
+  // compiler from sinking the arithmetic into the conditional stores)
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    const T v = P.alpha * acc[q];
+    b[q] = (lzu && i[q] < j[q]) ? T(0) : (nd[q] ? fma(P.beta, b[q], v) : v);
+  }
+#pragma unroll
+  for (int q = 0; q < NV; ++q) asm volatile("" : "+v"(b[q]));
+#pragma unroll
+  for (int q = 0; q < NV; ++q)
+    if (wr[q]) P.C[i[q] + (size_t)j[q] * P.ldc] = b[q];
+}
+
 // The 2 x 2 accumulator blocks of one wavefront's 32 x 32 piece at (r0, c0).
 template <typename T>
 __device__ __forceinline__ void gemm_store_wave(const GemmProb<T>& P, const typename Mfma<T>::V (&acc)[2][2],
                                                 int r0, int c0, int n, int lane) {
+  int ii[16], jj[16];
+  T vv[16];
 #pragma unroll
   for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
-      for (int v = 0; v < 4; ++v)
-        gemm_store1(P, r0 + 16 * rb + (lane & 15), c0 + 16 * cb + Mfma<T>::m_of(lane, v), n,
-                    acc[rb][cb][v]);
+      for (int v = 0; v < 4; ++v) {
+        const int q = 8 * rb + 4 * cb + v;
+        ii[q] = r0 + 16 * rb + (lane & 15);
+        jj[q] = c0 + 16 * cb + Mfma<T>::m_of(lane, v);
+        vv[q] = acc[rb][cb][v];
+      }
+  gemm_store_n<T, 16>(P, ii, jj, n, vv);
 }
 
 // GKX/4 matrix-core steps over the staged chunk for the wavefront's 32 x 32
@@ -347,7 +396,7 @@ __global__ __launch_bounds__(64 * KS) void k_gemm_sk(const GemmProb<T>* __restri
 // k x kTallCols block of op(B) is staged in LDS in chunks of kTallK rows; A
 // goes from HBM straight into the matrix-core operand (lane l: row l % 16 of
 // the block, column k + l / 16: four coalesced 128-byte segments per load),
-// the next k step's loads in flight during the current step's matrix-core
+// the next KQ k steps' loads in flight during the current steps' matrix-core
 // work.  The many tiny output tiles of such
 // products as 2-D tiles would be bound by workgroup dispatch, not by bytes.
 template <typename T>
@@ -356,7 +405,9 @@ __global__ __launch_bounds__(256) void k_gemm_tall(const GemmProb<T>* __restrict
   using V = typename Mfma<T>::V;
   constexpr int RB = kTallRows / 4 / 16;  // 16-row blocks per wavefront
   constexpr int CBK = kTallCols / 16;     // 16-column blocks
-  static_assert(RB * 64 == kTallRows && CBK * 16 == kTallCols && kTallK % 4 == 0, "tall shape");
+  constexpr int KQ = 2;                   // matrix-core k steps per batch of A loads
+  static_assert(RB * 64 == kTallRows && CBK * 16 == kTallCols && kTallK % (4 * KQ) == 0,
+                "tall shape");
   __shared__ T Bs[kTallK][kTallCols + 1];
   const GemmTask tk = tasks[blockIdx.x];
   const GemmProb<T> P = probs[tk.prob];
@@ -386,6 +437,11 @@ __global__ __launch_bounds__(256) void k_gemm_tall(const GemmProb<T>* __restrict
   };
   for (int k0 = 0; k0 < P.k; k0 += kTallK) {
     const int kc = min(kTallK, P.k - k0);
+    // A in batches of KQ matrix-core steps, the next batch in flight during the
+    // current one's steps; the first batch is requested before op(B)'s staging
+    T av[KQ][RB], an[KQ][RB];
+#pragma unroll
+    for (int s = 0; s < KQ; ++s) load_a(k0 + 4 * s + lk, av[s]);
     for (int e = t; e < kTallK * kTallCols; e += 256) {
       int kk, j;
       if (tb) { j = e % kTallCols; kk = e / kTallCols; }
@@ -398,22 +454,30 @@ __global__ __launch_bounds__(256) void k_gemm_tall(const GemmProb<T>* __restrict
       Bs[kk][j] = v;
     }
     __syncthreads();
-    T av[RB], an[RB];
-    load_a(k0 + lk, av);
-    for (int kq = 0; kq < kc; kq += 4) {
-      if (kq + 4 < kc) load_a(k0 + kq + 4 + lk, an);  // next step in flight
-      T bv[CBK];
+    for (int kq = 0; kq < kc; kq += 4 * KQ) {
+      if (kq + 4 * KQ < kc) {
 #pragma unroll
-      for (int b = 0; b < CBK; ++b) bv[b] = Bs[kq + lk][16 * b + li];  // zero past kc
+        for (int s = 0; s < KQ; ++s) load_a(k0 + kq + 4 * (KQ + s) + lk, an[s]);
+      }
 #pragma unroll
-      for (int a = 0; a < RB; ++a)
+      for (int s = 0; s < KQ; ++s) {
+        T bv[CBK];
 #pragma unroll
-        for (int b = 0; b < CBK; ++b) acc[a][b] = Mfma<T>::mac(bv[b], av[a], acc[a][b]);
+        for (int b = 0; b < CBK; ++b) bv[b] = Bs[kq + 4 * s + lk][16 * b + li];  // zero past kc
 #pragma unroll
-      for (int a = 0; a < RB; ++a) av[a] = an[a];
+        for (int a = 0; a < RB; ++a)
+#pragma unroll
+          for (int b = 0; b < CBK; ++b) acc[a][b] = Mfma<T>::mac(bv[b], av[s][a], acc[a][b]);
+      }
+#pragma unroll
+      for (int s = 0; s < KQ; ++s)
+#pragma unroll
+        for (int a = 0; a < RB; ++a) av[s][a] = an[s][a];
     }
     __syncthreads();
   }
+  // one value at a time: only the first rows' workgroups read (C0 rows < m0),
+  // and gathering the loads costs this kernel registers (occupancy)
 #pragma unroll
   for (int a = 0; a < RB; ++a)
 #pragma unroll
@@ -622,13 +686,20 @@ __global__ __launch_bounds__(64 * KS) void k_gemm_oz(const GemmProb<double>* __r
     __syncthreads();
   }
   if (grp == 0) {
+    int ii[16], jj[16];
+    double vv[16];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b)
 #pragma unroll
-        for (int v = 0; v < 4; ++v)
-          gemm_store1(P, row0 + 16 * a + li, col0 + 16 * b + 4 * lg + v, n, acc[a][b][v]);
+        for (int v = 0; v < 4; ++v) {
+          const int q = 8 * a + 4 * b + v;
+          ii[q] = row0 + 16 * a + li;
+          jj[q] = col0 + 16 * b + 4 * lg + v;
+          vv[q] = acc[a][b][v];
+        }
+    gemm_store_n<double, 16>(P, ii, jj, n, vv);
   }
 }
 
@@ -760,7 +831,7 @@ __global__ __launch_bounds__(256) void k_gemm_oz_tall(const GemmProb<double>* __
     __syncthreads();
   }
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < 4; ++a)  // one value at a time, as k_gemm_tall
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
