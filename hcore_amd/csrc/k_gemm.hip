@@ -243,8 +243,10 @@ __device__ __forceinline__ void gemm_mfma_chunk(const T (*As)[LDT], const T (*Bs
 // 32: one wavefront), k staged through LDS in chunks (GK, or GK/2 for the
 // one-wavefront tiles) with the next chunk's global loads in flight during the
 // current chunk's matrix-core steps.
+// Three wavefronts per SIMD (the launch bound caps the registers at 168; left
+// alone the compiler takes about 200 and two fit).
 template <typename T, int TILE>
-__global__ __launch_bounds__((TILE / 4) * (TILE / 4)) void k_gemm(
+__global__ __launch_bounds__((TILE / 4) * (TILE / 4), 3) void k_gemm(
     const GemmProb<T>* __restrict__ probs, const GemmTask* __restrict__ tasks) {
   using V = typename Mfma<T>::V;
   constexpr int NT = (TILE / 4) * (TILE / 4);  // 256 or 64 threads
@@ -303,7 +305,7 @@ __global__ __launch_bounds__((TILE / 4) * (TILE / 4)) void k_gemm(
 // not depend on timing.  Every wavefront runs the same number of chunk steps
 // (chunks past k load zeros), so the block barriers line up.
 template <typename T, int TILE, int KS>
-__global__ __launch_bounds__(64 * KS) void k_gemm_sk(const GemmProb<T>* __restrict__ probs,
+__global__ __launch_bounds__(64 * KS, 3) void k_gemm_sk(const GemmProb<T>* __restrict__ probs,
                                                      const GemmTask* __restrict__ tasks) {
   using V = typename Mfma<T>::V;
   static_assert(TILE == 32, "split-k tiles are one wavefront's 32 x 32 piece");
