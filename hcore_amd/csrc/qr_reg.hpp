@@ -204,10 +204,12 @@ __global__ __launch_bounds__(NT) void k_qr_reg(const QrProb<T>* __restrict__ pro
 // X <- (I - Y T Y^T)^T X = X - Y T^T (Y^T X), with the reflectors kept in
 // the registers that held the panel.  Each thread only ever touches its own
 // rows of the stack, so the panels and the trailing columns need no
-// inter-thread ordering through global memory.
-template <typename T, int NT, int RPT, int PB>
-__global__ __launch_bounds__(NT) void k_qr_fused(const QrProb<T>* __restrict__ probs, int nb,
-                                                 const int* __restrict__ q, int qc) {
+// inter-thread ordering through global memory.  MINW > 1 caps the registers
+// for MINW wavefronts per SIMD (two 512-thread workgroups per CU at MINW = 4,
+// at the price of a few spilled registers): for launches of many stacks.
+template <typename T, int NT, int RPT, int PB, int MINW>
+__global__ __launch_bounds__(NT, MINW) void k_qr_fused(const QrProb<T>* __restrict__ probs, int nb,
+                                                       const int* __restrict__ q, int qc) {
   constexpr int NW = NT / 64;
   constexpr int CB = 4;
   constexpr int CH = 16;
@@ -332,6 +334,10 @@ constexpr int kQr16Threads = 256;  // K <= 16: 4 rows per thread at nb = 1024 (t
 constexpr int kQr32Threads = 512;  // K <= 32: 2 rows per thread at nb = 1024
 constexpr int kQrfThreads = 512;   // panels: 2 rows per thread at nb = 1024
 constexpr int kQrfPanel = 16;      // panel width of the fused QR (VGPR budget at 2 rows)
+// launches of at least this many stacks (two rounds of one workgroup per CU)
+// take the register-capped shape that fits two workgroups per CU (measured at
+// nb = 1024: 914 stacks of K 33-64 -7 %; 26 stacks of K 130-187 +5 %)
+constexpr int kQrfManyStacks = 512;
 
 template <typename T, int NT, int RPT, int KMAX>
 inline void launch_qr_reg_shape(const QrProb<T>* probs, int n, int nb, hipStream_t s,
@@ -367,13 +373,15 @@ bool launch_qr_fused(const QrProb<T>* probs, int n, int nb, hipStream_t s, const
   static_assert(kQrFusedMaxNb <= 4 * NT, "fused QR shapes");
   const int rpt = (nb + NT - 1) / NT;
   if (nb <= 256)
-    hipLaunchKernelGGL((k_qr_fused<T, 256, 1, 32>), dim3(n), dim3(256), 0, s, probs, nb, q, qc);
+    hipLaunchKernelGGL((k_qr_fused<T, 256, 1, 32, 1>), dim3(n), dim3(256), 0, s, probs, nb, q, qc);
   else if (rpt <= 1)
-    hipLaunchKernelGGL((k_qr_fused<T, NT, 1, PB>), dim3(n), dim3(NT), 0, s, probs, nb, q, qc);
+    hipLaunchKernelGGL((k_qr_fused<T, NT, 1, PB, 1>), dim3(n), dim3(NT), 0, s, probs, nb, q, qc);
+  else if (rpt <= 2 && n >= kQrfManyStacks)  // two workgroups per CU
+    hipLaunchKernelGGL((k_qr_fused<T, NT, 2, PB, 4>), dim3(n), dim3(NT), 0, s, probs, nb, q, qc);
   else if (rpt <= 2)
-    hipLaunchKernelGGL((k_qr_fused<T, NT, 2, PB>), dim3(n), dim3(NT), 0, s, probs, nb, q, qc);
+    hipLaunchKernelGGL((k_qr_fused<T, NT, 2, PB, 1>), dim3(n), dim3(NT), 0, s, probs, nb, q, qc);
   else
-    hipLaunchKernelGGL((k_qr_fused<T, NT, 4, PB>), dim3(n), dim3(NT), 0, s, probs, nb, q, qc);
+    hipLaunchKernelGGL((k_qr_fused<T, NT, 4, PB, 1>), dim3(n), dim3(NT), 0, s, probs, nb, q, qc);
   return true;
 }
 
