@@ -333,6 +333,15 @@ struct hcore_hip_plan {
     std::vector<CoreLaunch> core;  // one launch per K bucket
     hipStream_t stream = nullptr;  // lanes > 0 (lane 0 uses the caller's stream)
     hipEvent_t ev_done = nullptr;
+    // lanes with more than one core launch by K bucket (host ranks): in a
+    // captured graph the launches after the first run on a side stream,
+    // forked after the lane's previous phase and joined before its next (they
+    // serve disjoint updates).  Measured on the nb = 1024 step: graph replay
+    // -1.2 %; executed eagerly the fifth stream shares a hardware queue with
+    // another lane (+9 %), and forking the rank-queue shapes of device-rank
+    // plans doubled their replay time, so neither forks.
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   };
   std::vector<Lane> lanes;
   std::vector<int> lane_of;  // lane index of each update (HCORE_hip_plan_get_lanes)
@@ -920,6 +929,11 @@ static int plan_finalize_t(hcore_hip_plan* p) {
     auto& L = p->lanes[l];
     if (l > 0) HCHECK(lane_stream_create(&L.stream));
     HCHECK(hipEventCreateWithFlags(&L.ev_done, hipEventDisableTiming));
+    if (L.core.size() > 1 && !L.queue) {
+      HCHECK(lane_stream_create(&L.aux));
+      HCHECK(hipEventCreateWithFlags(&L.ev_fork, hipEventDisableTiming));
+      HCHECK(hipEventCreateWithFlags(&L.ev_join, hipEventDisableTiming));
+    }
   }
   HCHECK(hipEventCreateWithFlags(&p->ev_start, hipEventDisableTiming));
   p->finalized = true;
@@ -1070,14 +1084,31 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
           launch_core_queue<T>(reinterpret_cast<const CoreProb<T>*>(D(L.o_qprobs)), L.n, L.qkl,
                                L.qreg, q, ls);
       }
-      for (const auto& c : L.core) {
+      // the first (largest-K) launch on the lane's stream, the others beside it
+      // (one after the other on the side stream) when the lane's work is being
+      // captured into a graph; lanes serialized for timing keep one stream
+      hipStreamCaptureStatus cst = hipStreamCaptureStatusNone;
+      const bool fork = L.aux && !(tm && p->serial) &&
+                        hipStreamIsCapturing(ls, &cst) == hipSuccess &&
+                        cst == hipStreamCaptureStatusActive;
+      if (fork) {
+        (void)hipEventRecord(L.ev_fork, ls);
+        (void)hipStreamWaitEvent(L.aux, L.ev_fork, 0);
+      }
+      for (size_t ci = 0; ci < L.core.size(); ++ci) {
+        const auto& c = L.core[ci];
         const int* cq = c.queue ? q : nullptr;
+        hipStream_t cs = fork && ci > 0 ? L.aux : ls;
         if (c.dbl)
           launch_core<double>(reinterpret_cast<const CoreProb<double>*>(D(c.off)), c.count,
-                              c.variant, c.kmax, p->rk, p->maxrk, p->acc, ls, cq, c.base_variant);
+                              c.variant, c.kmax, p->rk, p->maxrk, p->acc, cs, cq, c.base_variant);
         else
           launch_core<T>(reinterpret_cast<const CoreProb<T>*>(D(c.off)), c.count, c.variant,
-                         c.kmax, p->rk, p->maxrk, T(p->acc), ls, cq, c.base_variant);
+                         c.kmax, p->rk, p->maxrk, T(p->acc), cs, cq, c.base_variant);
+      }
+      if (fork) {
+        (void)hipEventRecord(L.ev_join, L.aux);
+        (void)hipStreamWaitEvent(ls, L.ev_join, 0);
       }
       launch_convert(reinterpret_cast<const ConvProb*>(D(L.o_cvout)), L.n_cv, L.cv_max, false, ls);
     });
@@ -2770,6 +2801,9 @@ void HCORE_hip_plan_destroy(hcore_hip_plan* p) {
   for (auto& L : p->lanes) {
     if (L.stream) (void)hipStreamDestroy(L.stream);
     if (L.ev_done) (void)hipEventDestroy(L.ev_done);
+    if (L.aux) (void)hipStreamDestroy(L.aux);
+    if (L.ev_fork) (void)hipEventDestroy(L.ev_fork);
+    if (L.ev_join) (void)hipEventDestroy(L.ev_join);
   }
   if (p->ev_start) (void)hipEventDestroy(p->ev_start);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
