@@ -334,12 +334,13 @@ struct hcore_hip_plan {
     hipStream_t stream = nullptr;  // lanes > 0 (lane 0 uses the caller's stream)
     hipEvent_t ev_done = nullptr;
     // lanes with more than one core launch by K bucket (host ranks): in a
-    // captured graph the launches after the first run on a side stream,
-    // forked after the lane's previous phase and joined before its next (they
-    // serve disjoint updates).  Measured on the nb = 1024 step: graph replay
-    // -1.2 %; executed eagerly the fifth stream shares a hardware queue with
-    // another lane (+9 %), and forking the rank-queue shapes of device-rank
-    // plans doubled their replay time, so neither forks.
+    // captured graph the launches after the first run on a side stream (made
+    // at the first capture), forked after the lane's previous phase and joined
+    // before its next (they serve disjoint updates).  Measured on the
+    // nb = 1024 step: graph replay -1.5 %; executed eagerly the fifth stream
+    // shares a hardware queue with another lane (+9 %), and forking the
+    // rank-queue shapes of device-rank plans doubled their replay time, so
+    // neither forks.
     hipStream_t aux = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   };
@@ -395,9 +396,22 @@ static int default_slices(double acc) {
   return std::min(kOzMaxSlices, std::max(kOzMinSlices, s));
 }
 
-// Streams of the lanes after the first (default priority, non-blocking).
+// Streams of the lanes after the first (default priority, non-blocking), kept
+// in a per-thread pool of free streams: a plan takes its streams at finalize
+// and gives them back when destroyed, so a stream of plans (one per TLR
+// Cholesky step) runs on the same few streams and hardware queues instead of
+// creating two or three streams per step.
+static thread_local std::vector<hipStream_t> t_free_streams;
 static hipError_t lane_stream_create(hipStream_t* s) {
+  if (!t_free_streams.empty()) {
+    *s = t_free_streams.back();
+    t_free_streams.pop_back();
+    return hipSuccess;
+  }
   return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
+}
+static void lane_stream_release(hipStream_t s) {
+  if (s) t_free_streams.push_back(s);
 }
 
 template <typename T>
@@ -929,11 +943,6 @@ static int plan_finalize_t(hcore_hip_plan* p) {
     auto& L = p->lanes[l];
     if (l > 0) HCHECK(lane_stream_create(&L.stream));
     HCHECK(hipEventCreateWithFlags(&L.ev_done, hipEventDisableTiming));
-    if (L.core.size() > 1 && !L.queue) {
-      HCHECK(lane_stream_create(&L.aux));
-      HCHECK(hipEventCreateWithFlags(&L.ev_fork, hipEventDisableTiming));
-      HCHECK(hipEventCreateWithFlags(&L.ev_join, hipEventDisableTiming));
-    }
   }
   HCHECK(hipEventCreateWithFlags(&p->ev_start, hipEventDisableTiming));
   p->finalized = true;
@@ -1087,10 +1096,7 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
       // the first (largest-K) launch on the lane's stream, the others beside it
       // (one after the other on the side stream) when the lane's work is being
       // captured into a graph; lanes serialized for timing keep one stream
-      hipStreamCaptureStatus cst = hipStreamCaptureStatusNone;
-      const bool fork = L.aux && !(tm && p->serial) &&
-                        hipStreamIsCapturing(ls, &cst) == hipSuccess &&
-                        cst == hipStreamCaptureStatusActive;
+      const bool fork = capture && L.aux && L.core.size() > 1;
       if (fork) {
         (void)hipEventRecord(L.ev_fork, ls);
         (void)hipStreamWaitEvent(L.aux, L.ev_fork, 0);
@@ -2476,6 +2482,14 @@ int HCORE_hip_plan_execute_graph(hcore_hip_plan* p, void* stream) {
   if (!p || !p->finalized) return HCORE_HIP_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
   if (!p->gexec) {
+    // side streams of the lanes with several core launches by K bucket, made
+    // at the first capture (plans executed eagerly never pay for them)
+    for (auto& L : p->lanes)
+      if (L.core.size() > 1 && !L.queue && !L.aux) {
+        HCHECK(lane_stream_create(&L.aux));
+        HCHECK(hipEventCreateWithFlags(&L.ev_fork, hipEventDisableTiming));
+        HCHECK(hipEventCreateWithFlags(&L.ev_join, hipEventDisableTiming));
+      }
     hipStream_t cs = nullptr;
     HCHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     HCHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
@@ -2799,9 +2813,9 @@ void HCORE_hip_plan_destroy(hcore_hip_plan* p) {
         for (int j = 0; j < 2; ++j)
           if (p->ev[k][l][i][j]) (void)hipEventDestroy(p->ev[k][l][i][j]);
   for (auto& L : p->lanes) {
-    if (L.stream) (void)hipStreamDestroy(L.stream);
+    lane_stream_release(L.stream);
     if (L.ev_done) (void)hipEventDestroy(L.ev_done);
-    if (L.aux) (void)hipStreamDestroy(L.aux);
+    lane_stream_release(L.aux);
     if (L.ev_fork) (void)hipEventDestroy(L.ev_fork);
     if (L.ev_join) (void)hipEventDestroy(L.ev_join);
   }
