@@ -398,9 +398,12 @@ static int default_slices(double acc) {
 
 // Streams of the lanes after the first (default priority, non-blocking), kept
 // in a per-thread pool of free streams: a plan takes its streams at finalize
-// and gives them back when destroyed, so a stream of plans (one per TLR
+// and gives them back when destroyed, so a sequence of plans (one per TLR
 // Cholesky step) runs on the same few streams and hardware queues instead of
-// creating two or three streams per step.
+// creating two or three streams per step.  A thread keeps at most
+// kFreeStreamsMax free streams; pooled streams are not destroyed before the
+// process ends.
+constexpr size_t kFreeStreamsMax = 8;
 static thread_local std::vector<hipStream_t> t_free_streams;
 static hipError_t lane_stream_create(hipStream_t* s) {
   if (!t_free_streams.empty()) {
@@ -411,7 +414,9 @@ static hipError_t lane_stream_create(hipStream_t* s) {
   return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
 }
 static void lane_stream_release(hipStream_t s) {
-  if (s) t_free_streams.push_back(s);
+  if (!s) return;
+  if (t_free_streams.size() < kFreeStreamsMax) t_free_streams.push_back(s);
+  else (void)hipStreamDestroy(s);
 }
 
 template <typename T>
