@@ -303,7 +303,8 @@ __global__ __launch_bounds__((TILE / 4) * (TILE / 4), 3) void k_gemm(
 // g, g + KS, ... (GKS wide, staged through its own LDS buffers); the partial
 // tiles are then summed in LDS in a fixed pairwise order, so the result does
 // not depend on timing.  Every wavefront runs the same number of chunk steps
-// (chunks past k load zeros), so the block barriers line up.
+// (chunks past k load zeros), so the block barriers line up.  Registers capped
+// for three wavefronts per SIMD, as k_gemm.
 template <typename T, int TILE, int KS>
 __global__ __launch_bounds__(64 * KS, 3) void k_gemm_sk(const GemmProb<T>* __restrict__ probs,
                                                      const GemmTask* __restrict__ tasks) {
