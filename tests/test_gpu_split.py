@@ -11,7 +11,8 @@ FP64 sums) is pinned by what its arithmetic fixes:
   * for general operands the error of C(i, j) is at most
     k (S + 3) 2^(1 - 8 S) max|op(A)(i, :)| max|op(B)(:, j)| for S digits
     (the digit remainders and the dropped digit pairs), and S = 0 runs the
-    same tiles through the FP64 matrix cores.
+    same tiles through the FP64 matrix cores (exact as well on the integer
+    operands, with the alpha/beta forms reading C in the GEMM epilogues).
 Split plans are compared with the oracle (ranks within one, products within
 10*acc); the slice autotuner must pick counts that keep the core singular
 values within 0.1*acc of the FP64 ones; graph replays equal eager execution.
@@ -58,7 +59,7 @@ def test_sliced_product_exact_on_scaled_small_integers(hc, shape):
     exact = np.ldexp((Ai @ Bi).astype(float), (ra[:, None] + cb[None, :]).astype(int))
     C0 = np.ldexp(rng.integers(-1000, 1001, size=(m, n)).astype(float),
                   (ra[:, None] + cb[None, :]).astype(int))
-    for S in (2, 6):
+    for S in (0, 2, 6):  # 0: the same tiles through the FP64 matrix-core kernels
         got = hc.gemm_split(_op(A, ta), _op(B, tb), np.zeros((m, n)), S, transA=ta, transB=tb)
         assert np.array_equal(got, exact), "slices %d" % S
         got = hc.gemm_split(_op(A, ta), _op(B, tb), C0, S, alpha=2.0, beta=-1.0, transA=ta,
