@@ -400,23 +400,35 @@ static int default_slices(double acc) {
 // in a per-thread pool of free streams: a plan takes its streams at finalize
 // and gives them back when destroyed, so a sequence of plans (one per TLR
 // Cholesky step) runs on the same few streams and hardware queues instead of
-// creating two or three streams per step.  A thread keeps at most
-// kFreeStreamsMax free streams; pooled streams are not destroyed before the
-// process ends.
+// creating two or three streams per step.  A stream is handed out only on the
+// device it was made on; a thread keeps at most kFreeStreamsMax free streams;
+// pooled streams are not destroyed before the process ends.
 constexpr size_t kFreeStreamsMax = 8;
-static thread_local std::vector<hipStream_t> t_free_streams;
+struct FreeStream {
+  int device;
+  hipStream_t stream;
+};
+static thread_local std::vector<FreeStream> t_free_streams;
 static hipError_t lane_stream_create(hipStream_t* s) {
-  if (!t_free_streams.empty()) {
-    *s = t_free_streams.back();
-    t_free_streams.pop_back();
-    return hipSuccess;
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    for (size_t i = t_free_streams.size(); i-- > 0;) {
+      if (t_free_streams[i].device == dev) {
+        *s = t_free_streams[i].stream;
+        t_free_streams.erase(t_free_streams.begin() + (long)i);
+        return hipSuccess;
+      }
+    }
   }
   return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
 }
 static void lane_stream_release(hipStream_t s) {
   if (!s) return;
-  if (t_free_streams.size() < kFreeStreamsMax) t_free_streams.push_back(s);
-  else (void)hipStreamDestroy(s);
+  hipDevice_t dev = 0;
+  if (t_free_streams.size() < kFreeStreamsMax && hipStreamGetDevice(s, &dev) == hipSuccess)
+    t_free_streams.push_back({(int)dev, s});
+  else
+    (void)hipStreamDestroy(s);
 }
 
 template <typename T>
