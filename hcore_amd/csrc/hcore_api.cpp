@@ -1112,7 +1112,7 @@ static int plan_execute_t(hcore_hip_plan* p, hipStream_t s, bool capture = false
       }
       // the first (largest-K) launch on the lane's stream, the others beside it
       // (one after the other on the side stream) when the lane's work is being
-      // captured into a graph; lanes serialized for timing keep one stream
+      // captured into a graph
       const bool fork = capture && L.aux && L.core.size() > 1;
       if (fork) {
         (void)hipEventRecord(L.ev_fork, ls);
