@@ -116,7 +116,10 @@ int HCORE_hip_plan_finalize(hcore_hip_plan *plan);
 /* Enqueue the whole batch on `stream` (hipStream_t; NULL = default stream). */
 int HCORE_hip_plan_execute(hcore_hip_plan *plan, void *stream);
 
-/* Same, through a hipGraph captured on the first call and replayed after. */
+/* Same, through a hipGraph captured on the first call and replayed after.
+ * In the graph a lane's core SVD launches of different stacked-rank buckets
+ * (host-rank plans) run side by side; a side stream for them is made at the
+ * first capture. */
 int HCORE_hip_plan_execute_graph(hcore_hip_plan *plan, void *stream);
 
 /*
@@ -199,8 +202,9 @@ void HCORE_hip_plan_destroy(hcore_hip_plan *plan);
 /* Per-phase timing.  A plan of more than 64 updates splits them into up to
  * three lanes by stacked rank K (> 128, 65..128, <= 64), the first on the
  * caller's stream and the others on streams of their own (four hardware
- * queues with the null stream); the dsyrk updates follow the next-to-last
- * lane on its stream.  When timing is enabled,
+ * queues with the null stream; those streams come from a per-thread pool and
+ * return to it when the plan is destroyed); the dsyrk updates follow the
+ * next-to-last lane on its stream.  When timing is enabled,
  * HCORE_hip_plan_execute records a HIP event pair around every kernel launch
  * on the stream it runs on.  Phases: 0 stack, 1 F=AV^T BV, 2 G=BU F^T,
  * 3 geqrf, 4 T=R_A R_B^T, 5 core SVD (all K buckets of the lane, run
